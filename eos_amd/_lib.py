@@ -1,0 +1,244 @@
+"""ctypes binding of the C ABI in include/eos_c.h (libeos_amd.so, built in-tree).
+
+There is no fallback: if the HIP library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libeos_amd.so")
+
+EOS_OK, EOS_ERR_INVALID, EOS_ERR_HIP, EOS_ERR_NOMEM, EOS_ERR_STATE, EOS_ERR_LIMIT, EOS_ERR_NODEVICE = range(7)
+EOS_MEM_HOST, EOS_MEM_DEVICE = 0, 1
+DBG = {"morton_sorted": 1, "perm": 2, "kchild": 3, "kparent": 4, "ibox": 5, "nodes": 6, "tris": 7,
+       "morton": 8}
+
+# Every symbol include/eos_c.h declares.
+EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_update_materials",
+           "eos_update_lights", "eos_render", "eos_render_adjoint", "eos_render_batch",
+           "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
+           "eos_abi_version", "eos_device_name"]
+
+
+class EosError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"eos status {status}: {msg}")
+        self.status = status
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("num_triangles", C.c_int32), ("vertices", C.c_void_p), ("normals", C.c_void_p),
+                ("material_index", C.c_void_p), ("triangle_memory", C.c_int32),
+                ("num_materials", C.c_int32), ("materials", C.c_void_p),
+                ("num_lights", C.c_int32), ("lights", C.c_void_p)]
+
+
+class BuildStats(C.Structure):
+    _fields_ = [("build_ms", C.c_double), ("num_internal", C.c_int32), ("root_ref", C.c_int32),
+                ("leaf_max", C.c_int32), ("pad", C.c_int32)]
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [("depth", C.c_int32), ("output_memory", C.c_int32), ("rgb_out", C.c_void_p),
+                ("hit_out", C.c_void_p), ("keep_records", C.c_int32), ("count_visits", C.c_int32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("primary_rays", C.c_int64), ("shadow_rays", C.c_int64), ("secondary_rays", C.c_int64),
+                ("hits", C.c_int64), ("node_visits", C.c_int64), ("tri_tests", C.c_int64),
+                ("render_ms", C.c_double), ("trace_closest_ms", C.c_double),
+                ("trace_shadow_ms", C.c_double), ("shade_ms", C.c_double),
+                ("trace_closest_launches", C.c_int32), ("trace_shadow_launches", C.c_int32),
+                ("closest_visits_first", C.c_int64), ("closest_tests_first", C.c_int64),
+                ("trace_first_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AdjointOpts(C.Structure):
+    _fields_ = [("depth", C.c_int32), ("memory", C.c_int32), ("recompute", C.c_int32), ("pad", C.c_int32)]
+
+
+class AdjointStats(C.Structure):
+    _fields_ = [("adjoint_ms", C.c_double), ("retrace_ms", C.c_double), ("records", C.c_int64)]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("depth", C.c_int32), ("output_memory", C.c_int32), ("use_graph", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("render_ms", C.c_double), ("rays", C.c_int64)]
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("workgroup", C.c_int32), ("stack_lds", C.c_int32), ("waves_per_cu", C.c_int32),
+                ("refill", C.c_int32)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python eos_amd/build.py` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        vp, i32 = C.c_void_p, C.c_int32
+        L.eos_create_scene.argtypes = [vp, C.POINTER(vp)]
+        L.eos_destroy_scene.argtypes = [vp]
+        L.eos_build_lbvh.argtypes = [vp, i32, vp]
+        L.eos_update_materials.argtypes = [vp, vp, i32]
+        L.eos_update_lights.argtypes = [vp, vp, i32]
+        L.eos_render.argtypes = [vp, vp, vp, vp, vp]
+        L.eos_render_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.eos_render_batch.argtypes = [vp, vp, i32, vp, vp, vp]
+        L.eos_set_launch_config.argtypes = [vp, vp]
+        L.eos_get_launch_config.argtypes = [vp, vp]
+        L.eos_debug_read.argtypes = [vp, i32, vp, C.c_int64]
+        L.eos_last_error.restype = C.c_char_p
+        L.eos_abi_version.restype = i32
+        L.eos_device_name.argtypes = [C.c_char_p, i32]
+        for name in EXPORTS:
+            if name != "eos_last_error" and name != "eos_abi_version":
+                getattr(L, name).restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def check(status: int):
+    if status != EOS_OK:
+        raise EosError(status, lib().eos_last_error().decode())
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(lib().eos_device_name(buf, 256))
+    return buf.value.decode()
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Scene:
+    """One device scene (eos_scene*).  Arrays are copied to the device at creation."""
+
+    def __init__(self, scene, leaf_max: int = 4, build: bool = True):
+        self.data = scene
+        v = np.ascontiguousarray(scene.verts, np.float32)
+        n = np.ascontiguousarray(scene.normals, np.float32)
+        m = np.ascontiguousarray(scene.mat, np.int32)
+        self._mats = np.ascontiguousarray(scene.materials)
+        self._lights = np.ascontiguousarray(scene.lights)
+        d = SceneDesc(v.shape[0], _ptr(v), _ptr(n), _ptr(m), EOS_MEM_HOST, len(self._mats),
+                      _ptr(self._mats), len(self._lights), _ptr(self._lights))
+        h = C.c_void_p()
+        check(lib().eos_create_scene(C.byref(d), C.byref(h)))
+        self.h = h
+        self.build_stats = None
+        if build:
+            self.build(leaf_max)
+
+    @property
+    def handle(self):
+        return self.h
+
+    def build(self, leaf_max: int = 4) -> dict:
+        st = BuildStats()
+        check(lib().eos_build_lbvh(self.h, leaf_max, C.byref(st)))
+        self.build_stats = {k: getattr(st, k) for k, _ in st._fields_}
+        return self.build_stats
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().eos_destroy_scene(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_materials(self, mats):
+        self._mats = np.ascontiguousarray(mats)
+        check(lib().eos_update_materials(self.h, _ptr(self._mats), len(self._mats)))
+
+    def set_lights(self, lights):
+        self._lights = np.ascontiguousarray(lights)
+        check(lib().eos_update_lights(self.h, _ptr(self._lights), len(self._lights)))
+
+    def set_launch_config(self, workgroup=256, stack_lds=16, waves_per_cu=32, refill=16):
+        c = LaunchConfig(workgroup, stack_lds, waves_per_cu, refill)
+        check(lib().eos_set_launch_config(self.h, C.byref(c)))
+
+    def launch_config(self) -> dict:
+        c = LaunchConfig()
+        check(lib().eos_get_launch_config(self.h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in c._fields_}
+
+    def render(self, camera=None, depth=None, want_rgb=False, want_hits=False, keep_records=False,
+               count_visits=False, want_rgba=True):
+        cam = np.array(self.data.camera if camera is None else camera, copy=True)
+        depth = self.data.depth if depth is None else depth
+        W, H = int(cam["width"]), int(cam["height"])
+        rgba = np.zeros((H, W, 4), np.uint8) if want_rgba else None
+        rgb = np.zeros((H, W, 3), np.float32) if want_rgb else None
+        hits = np.zeros((H, W), np.int32) if want_hits else None
+        o = RenderOpts(depth, EOS_MEM_HOST, _ptr(rgb), _ptr(hits), int(keep_records), int(count_visits))
+        st = RenderStats()
+        check(lib().eos_render(self.h, cam.ctypes.data, C.byref(o), _ptr(rgba), C.byref(st)))
+        return {"rgba": rgba, "rgb": rgb, "hit": hits, "stats": st.as_dict()}
+
+    def render_device(self, camera, depth, rgba_dev_ptr, rgb_dev_ptr=None, hit_dev_ptr=None,
+                      keep_records=False, count_visits=False) -> dict:
+        """Render into device buffers (raw pointers, e.g. torch tensors' data_ptr())."""
+        cam = np.array(camera, copy=True)
+        o = RenderOpts(depth, EOS_MEM_DEVICE, rgb_dev_ptr, hit_dev_ptr, int(keep_records), int(count_visits))
+        st = RenderStats()
+        check(lib().eos_render(self.h, cam.ctypes.data, C.byref(o), rgba_dev_ptr, C.byref(st)))
+        return st.as_dict()
+
+    def adjoint(self, dl_drgb, camera=None, depth=None, recompute=False):
+        cam = np.array(self.data.camera if camera is None else camera, copy=True)
+        depth = self.data.depth if depth is None else depth
+        g = np.ascontiguousarray(dl_drgb, np.float32)
+        gm = np.zeros((len(self._mats), 7), np.float64)
+        gl = np.zeros((max(len(self._lights), 1), 6), np.float64)
+        o = AdjointOpts(depth, EOS_MEM_HOST, int(recompute), 0)
+        st = AdjointStats()
+        check(lib().eos_render_adjoint(self.h, cam.ctypes.data, C.byref(o), _ptr(g), _ptr(gm), _ptr(gl),
+                                       C.byref(st)))
+        return gm, gl[:len(self._lights)], {k: getattr(st, k) for k, _ in st._fields_}
+
+    def debug_read(self, what: str):
+        n = self.data.n_tri
+        m = max(n - 1, 0)
+        shapes = {"morton_sorted": ((n,), np.uint32), "perm": ((n,), np.int32),
+                  "kchild": ((m, 2), np.int32), "kparent": ((max(2 * n - 1, 0),), np.int32),
+                  "ibox": ((m, 6), np.float32), "nodes": ((m, 16), np.float32),
+                  "tris": ((n, 12), np.float32), "morton": ((n,), np.uint32)}
+        shape, dt = shapes[what]
+        out = np.zeros(shape, dt)
+        check(lib().eos_debug_read(self.h, DBG[what], _ptr(out), out.nbytes))
+        return out
+
+
+def render_batch(scenes, cameras=None, depth=2, use_graph=False):
+    cams = np.ascontiguousarray(np.stack([s.data.camera for s in scenes]) if cameras is None else cameras)
+    outs = [np.zeros((int(c["height"]), int(c["width"]), 4), np.uint8) for c in cams]
+    hs = (C.c_void_p * len(scenes))(*[s.h.value for s in scenes])
+    ptrs = (C.c_void_p * len(scenes))(*[o.ctypes.data for o in outs])
+    o = BatchOpts(depth, EOS_MEM_HOST, int(use_graph), 0)
+    st = BatchStats()
+    check(lib().eos_render_batch(hs, cams.ctypes.data, len(scenes), C.byref(o), ptrs, C.byref(st)))
+    return outs, {k: getattr(st, k) for k, _ in st._fields_}

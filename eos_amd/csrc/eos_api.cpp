@@ -1,0 +1,291 @@
+// eos_api.cpp -- C ABI entry points: scene lifetime, parameters, diagnostics.
+#include <cstring>
+#include <string>
+
+#include "eos_internal.h"
+
+namespace eos {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+eos_status fail(eos_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+eos_status DevBuf::ensure(size_t want) {
+    if (want <= bytes && ptr) return EOS_OK;
+    release();
+    if (want == 0) return EOS_OK;
+    size_t b = (want + 255) & ~size_t(255);
+    hipError_t e = hipMalloc(&ptr, b);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        return fail(EOS_ERR_NOMEM, "hipMalloc(" + std::to_string(b) + "): " + hipGetErrorString(e));
+    }
+    bytes = b;
+    return EOS_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+SceneView view_of(const eos_scene* s) {
+    SceneView v;
+    v.nodes = s->nodes.as<const Node>();
+    v.geo = s->geo.as<const float4>();
+    v.shade = s->shade.as<const float4>();
+    v.mats = s->mats.as<const DevMaterial>();
+    v.lights = s->lights.as<const DevLight>();
+    v.root = s->root;
+    v.n_tri = s->n_tri;
+    v.n_light = s->n_light;
+    v.n_mat = s->n_mat;
+    return v;
+}
+
+static eos_status copy_in(DevBuf& dst, const void* src, size_t bytes, int memory, hipStream_t st) {
+    EOS_TRY(dst.ensure(bytes));
+    if (bytes == 0) return EOS_OK;
+    EOS_HIP(hipMemcpyAsync(dst.ptr, src, bytes, memory == EOS_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                                         : hipMemcpyHostToDevice, st));
+    return EOS_OK;
+}
+
+static void free_scene(eos_scene* s) {
+    DevBuf* bufs[] = {&s->verts, &s->normals, &s->matidx, &s->mats, &s->lights, &s->nodes, &s->geo,
+                      &s->shade, &s->codes, &s->codes_sorted, &s->perm, &s->kchild, &s->kparent,
+                      &s->first_last, &s->ibox, &s->lbox, &s->flags, &s->cbounds, &s->sort_tmp_k,
+                      &s->sort_tmp_v, &s->sort_hist, &s->bounds_part, &s->fb.hit0, &s->fb.q[0], &s->fb.q[1],
+                      &s->fb.hitq, &s->fb.levcol, &s->fb.levels, &s->fb.counters, &s->fb.gstack};
+    for (DevBuf* b : bufs) b->release();
+    for (int i = 0; i < kMaxStack; ++i) {
+        s->fb.surf[i].release();
+        s->fb.vis[i].release();
+    }
+    for (hipEvent_t& e : s->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+static eos_status check_materials(const eos_material* m, int n) {
+    for (int i = 0; i < n; ++i) {
+        if (!(m[i].shininess >= 0.0f)) return fail(EOS_ERR_INVALID, "material shininess must be >= 0");
+    }
+    return EOS_OK;
+}
+
+static eos_status check_lights(const eos_light* l, int n) {
+    for (int i = 0; i < n; ++i)
+        if (l[i].position[3] != 1.0f)
+            return fail(EOS_ERR_INVALID, "light position w must be 1 (src/RayTracer.cpp:433 subtracts a w=1 point)");
+    return EOS_OK;
+}
+
+}  // namespace eos
+
+using namespace eos;
+
+extern "C" {
+
+const char* eos_last_error(void) { return g_last_error.c_str(); }
+
+int32_t eos_abi_version(void) { return EOS_ABI_VERSION; }
+
+eos_status eos_device_name(char* buf, int32_t len) {
+    if (!buf || len <= 0) return fail(EOS_ERR_INVALID, "eos_device_name: null buffer");
+    int dev = 0, count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(EOS_ERR_NODEVICE, "no HIP device");
+    EOS_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    EOS_HIP(hipGetDeviceProperties(&p, dev));
+    std::snprintf(buf, (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return EOS_OK;
+}
+
+eos_status eos_create_scene(const eos_scene_desc* d, eos_scene** out) {
+    if (!d || !out) return fail(EOS_ERR_INVALID, "eos_create_scene: null argument");
+    *out = nullptr;
+    if (d->num_triangles < 0 || d->num_triangles >= (1 << kLeafShift))
+        return fail(EOS_ERR_LIMIT, "eos_create_scene: num_triangles must be in [0, 2^28)");
+    if (d->num_triangles > 0 && (!d->vertices || !d->normals || !d->material_index))
+        return fail(EOS_ERR_INVALID, "eos_create_scene: null triangle arrays");
+    if (d->num_materials <= 0 || !d->materials)
+        return fail(EOS_ERR_INVALID, "eos_create_scene: at least one material is required");
+    if (d->num_lights < 0 || d->num_lights > EOS_MAX_LIGHTS || (d->num_lights > 0 && !d->lights))
+        return fail(EOS_ERR_LIMIT, "eos_create_scene: num_lights must be in [0, 32]");
+    EOS_TRY(check_materials(d->materials, d->num_materials));
+    EOS_TRY(check_lights(d->lights, d->num_lights));
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(EOS_ERR_NODEVICE, "eos_create_scene: no HIP device");
+    // Material indices are validated on the host when they are host memory.
+    if (d->triangle_memory == EOS_MEM_HOST)
+        for (int i = 0; i < d->num_triangles; ++i)
+            if (d->material_index[i] < 0 || d->material_index[i] >= d->num_materials)
+                return fail(EOS_ERR_INVALID, "eos_create_scene: material index out of range at triangle " +
+                                                 std::to_string(i));
+    eos_scene* s = new eos_scene();
+    EOS_HIP(hipGetDevice(&s->device));
+    hipDeviceProp_t p;
+    EOS_HIP(hipGetDeviceProperties(&p, s->device));
+    s->num_cu = p.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(EOS_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    for (hipEvent_t& ev : s->ev) {
+        if (hipEventCreate(&ev) != hipSuccess) {
+            free_scene(s);
+            delete s;
+            return fail(EOS_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    s->n_tri = d->num_triangles;
+    s->n_mat = d->num_materials;
+    s->n_light = d->num_lights;
+    const size_t n = (size_t)d->num_triangles;
+    eos_status st = EOS_OK;
+    if (st == EOS_OK) st = copy_in(s->verts, d->vertices, n * 9 * sizeof(float), d->triangle_memory, s->stream);
+    if (st == EOS_OK) st = copy_in(s->normals, d->normals, n * 9 * sizeof(float), d->triangle_memory, s->stream);
+    if (st == EOS_OK) st = copy_in(s->matidx, d->material_index, n * sizeof(int32_t), d->triangle_memory, s->stream);
+    if (st == EOS_OK) st = copy_in(s->mats, d->materials, sizeof(eos_material) * d->num_materials, EOS_MEM_HOST, s->stream);
+    if (st == EOS_OK) st = s->lights.ensure(sizeof(eos_light) * EOS_MAX_LIGHTS);
+    if (st == EOS_OK && d->num_lights > 0)
+        st = copy_in(s->lights, d->lights, sizeof(eos_light) * d->num_lights, EOS_MEM_HOST, s->stream);
+    if (st == EOS_OK && hipStreamSynchronize(s->stream) != hipSuccess)
+        st = fail(EOS_ERR_HIP, "eos_create_scene: upload failed");
+    if (st != EOS_OK) {
+        free_scene(s);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return EOS_OK;
+}
+
+eos_status eos_destroy_scene(eos_scene* s) {
+    if (!s) return EOS_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_scene(s);
+    delete s;
+    return EOS_OK;
+}
+
+eos_status eos_update_materials(eos_scene* s, const eos_material* m, int32_t n) {
+    if (!s || !m) return fail(EOS_ERR_INVALID, "eos_update_materials: null argument");
+    if (n != s->n_mat) return fail(EOS_ERR_INVALID, "eos_update_materials: material count differs from the scene's");
+    EOS_TRY(check_materials(m, n));
+    EOS_HIP(hipMemcpyAsync(s->mats.ptr, m, sizeof(eos_material) * n, hipMemcpyHostToDevice, s->stream));
+    EOS_HIP(hipStreamSynchronize(s->stream));
+    return EOS_OK;
+}
+
+eos_status eos_update_lights(eos_scene* s, const eos_light* l, int32_t n) {
+    if (!s || (!l && n > 0)) return fail(EOS_ERR_INVALID, "eos_update_lights: null argument");
+    if (n < 0 || n > EOS_MAX_LIGHTS) return fail(EOS_ERR_LIMIT, "eos_update_lights: at most 32 lights");
+    EOS_TRY(check_lights(l, n));
+    if (n > 0) EOS_HIP(hipMemcpyAsync(s->lights.ptr, l, sizeof(eos_light) * n, hipMemcpyHostToDevice, s->stream));
+    EOS_HIP(hipStreamSynchronize(s->stream));
+    if (n != s->n_light) s->fb.rec_valid = 0;
+    s->n_light = n;
+    return EOS_OK;
+}
+
+eos_status eos_build_lbvh(eos_scene* s, int32_t leaf_max, eos_build_stats* st) {
+    if (!s) return fail(EOS_ERR_INVALID, "eos_build_lbvh: null scene");
+    if (leaf_max < 1 || leaf_max > 8) return fail(EOS_ERR_INVALID, "eos_build_lbvh: leaf_max must be in [1, 8]");
+    EOS_HIP(hipSetDevice(s->device));
+    return build_lbvh(s, leaf_max, st);
+}
+
+eos_status eos_render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
+                      eos_render_stats* st) {
+    if (!s || !cam || !o) return fail(EOS_ERR_INVALID, "eos_render: null argument");
+    if (!s->built) return fail(EOS_ERR_STATE, "eos_render: call eos_build_lbvh first");
+    if (cam->width <= 0 || cam->height <= 0 || (int64_t)cam->width * cam->height > (int64_t)1 << 30)
+        return fail(EOS_ERR_INVALID, "eos_render: bad image size");
+    if (o->depth < 0 || o->depth > EOS_MAX_DEPTH) return fail(EOS_ERR_INVALID, "eos_render: depth must be in [0, 8]");
+    EOS_HIP(hipSetDevice(s->device));
+    return render(s, cam, o, rgba, st);
+}
+
+eos_status eos_render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint_opts* o, const float* dl,
+                              double* gm, double* gl, eos_adjoint_stats* st) {
+    if (!s || !cam || !o || !dl || !gm || !gl) return fail(EOS_ERR_INVALID, "eos_render_adjoint: null argument");
+    if (!s->built) return fail(EOS_ERR_STATE, "eos_render_adjoint: call eos_build_lbvh first");
+    if (o->depth < 0 || o->depth > EOS_MAX_DEPTH) return fail(EOS_ERR_INVALID, "eos_render_adjoint: bad depth");
+    EOS_HIP(hipSetDevice(s->device));
+    return render_adjoint(s, cam, o, dl, gm, gl, st);
+}
+
+eos_status eos_render_batch(eos_scene* const* scenes, const eos_camera* cams, int32_t n, const eos_batch_opts* o,
+                            uint8_t* const* out, eos_batch_stats* st) {
+    if (!scenes || !cams || !o || !out || n <= 0) return fail(EOS_ERR_INVALID, "eos_render_batch: bad argument");
+    for (int i = 0; i < n; ++i) {
+        if (!scenes[i] || !scenes[i]->built) return fail(EOS_ERR_STATE, "eos_render_batch: scene without LBVH");
+        if (cams[i].width <= 0 || cams[i].height <= 0) return fail(EOS_ERR_INVALID, "eos_render_batch: bad image size");
+        if (scenes[i]->device != scenes[0]->device) return fail(EOS_ERR_INVALID, "eos_render_batch: scenes on different devices");
+    }
+    if (o->depth < 0 || o->depth > EOS_MAX_DEPTH) return fail(EOS_ERR_INVALID, "eos_render_batch: bad depth");
+    EOS_HIP(hipSetDevice(scenes[0]->device));
+    return render_batch(scenes, cams, n, o, out, st);
+}
+
+eos_status eos_set_launch_config(eos_scene* s, const eos_launch_config* c) {
+    if (!s || !c) return fail(EOS_ERR_INVALID, "eos_set_launch_config: null argument");
+    if (c->workgroup != 64 && c->workgroup != 128 && c->workgroup != 256)
+        return fail(EOS_ERR_INVALID, "workgroup must be 64, 128 or 256");
+    if (c->stack_lds != 8 && c->stack_lds != 16 && c->stack_lds != 32)
+        return fail(EOS_ERR_INVALID, "stack_lds must be 8, 16 or 32");
+    if (c->waves_per_cu < 1 || c->waves_per_cu > 64) return fail(EOS_ERR_INVALID, "waves_per_cu must be in [1, 64]");
+    if (c->refill < 1 || c->refill > 64) return fail(EOS_ERR_INVALID, "refill must be in [1, 64]");
+    s->cfg.workgroup = c->workgroup;
+    s->cfg.stack_lds = c->stack_lds;
+    s->cfg.waves_per_cu = c->waves_per_cu;
+    s->cfg.refill = c->refill;
+    return EOS_OK;
+}
+
+eos_status eos_get_launch_config(const eos_scene* s, eos_launch_config* c) {
+    if (!s || !c) return fail(EOS_ERR_INVALID, "eos_get_launch_config: null argument");
+    c->workgroup = s->cfg.workgroup;
+    c->stack_lds = s->cfg.stack_lds;
+    c->waves_per_cu = s->cfg.waves_per_cu;
+    c->refill = s->cfg.refill;
+    return EOS_OK;
+}
+
+eos_status eos_debug_read(const eos_scene* s, int32_t what, void* dst, int64_t bytes) {
+    if (!s || !dst || bytes < 0) return fail(EOS_ERR_INVALID, "eos_debug_read: bad argument");
+    const DevBuf* b = nullptr;
+    size_t need = 0;
+    const size_t n = (size_t)s->n_tri, m = n > 0 ? n - 1 : 0;
+    switch (what) {
+        case EOS_DBG_MORTON_SORTED: b = &s->codes_sorted; need = 4 * n; break;
+        case EOS_DBG_PERM: b = &s->perm; need = 4 * n; break;
+        case EOS_DBG_KCHILD: b = &s->kchild; need = 8 * m; break;
+        case EOS_DBG_KPARENT: b = &s->kparent; need = 4 * (2 * n - (n ? 1 : 0)); break;
+        case EOS_DBG_IBOX: b = &s->ibox; need = 24 * m; break;
+        case EOS_DBG_NODES: b = &s->nodes; need = 64 * m; break;
+        case EOS_DBG_TRIS: b = &s->geo; need = 48 * n; break;
+        case EOS_DBG_MORTON: b = &s->codes; need = 4 * n; break;
+        default: return fail(EOS_ERR_INVALID, "eos_debug_read: unknown array");
+    }
+    if (!s->built) return fail(EOS_ERR_STATE, "eos_debug_read: no LBVH built");
+    if ((size_t)bytes < need) return fail(EOS_ERR_INVALID, "eos_debug_read: destination too small");
+    if (need == 0) return EOS_OK;
+    EOS_HIP(hipSetDevice(s->device));
+    EOS_HIP(hipMemcpyAsync(dst, b->ptr, need, hipMemcpyDeviceToHost, s->stream));
+    EOS_HIP(hipStreamSynchronize(s->stream));
+    return EOS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+// eos_internal.h -- device layouts and host bookkeeping shared by the HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/eos_c.h"
+
+namespace eos {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+eos_status fail(eos_status st, const std::string& msg);
+#define EOS_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return ::eos::fail(e_ == hipErrorOutOfMemory ? EOS_ERR_NOMEM : EOS_ERR_HIP,     \
+                               std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+#define EOS_TRY(expr)                          \
+    do {                                       \
+        eos_status s_ = (expr);                \
+        if (s_ != EOS_OK) return s_;           \
+    } while (0)
+
+// ---------------------------------------------------------------- layouts
+// 64-byte BVH node: the two child boxes and the two child references.
+//   b0 = c0.lo.x c0.hi.x c0.lo.y c0.hi.y
+//   b1 = c1.lo.x c1.hi.x c1.lo.y c1.hi.y
+//   b2 = c0.lo.z c0.hi.z c1.lo.z c1.hi.z
+//   c  = ref0, ref1, 0, 0
+// ref >= 0: internal node index; ref < 0: leaf, ~ref = first | (count-1) << 28.
+struct alignas(64) Node {
+    float4 b0, b1, b2;
+    int4 c;
+};
+static_assert(sizeof(Node) == 64, "node is one 64-byte line");
+
+// Triangle records in leaf (Morton) order, 48 bytes each, read as 3 x 16 B:
+//   geo:   v1.xyz e1.x | e1.yz e2.xy | e2.z orig(int) 0 0       (traversal)
+//   shade: n1.xyz mat  | n2.xyz 0    | n3.xyz 0                 (shading)
+constexpr int kLeafShift = 28;
+constexpr int32_t kDone = 0x7FFFFFFF;  // traversal sentinel
+constexpr int kMaxStack = 64;           // LBVH depth <= 64 (prefix length <= 63)
+
+struct DevMaterial {
+    float kd[4];
+    float ks[4];
+    float shininess, reflectivity;
+    int32_t reflective, pad;
+};
+struct DevLight {
+    float position[4];
+    float color[4];
+};
+struct DevCamera {
+    float position[3], view[3], right[3], up[3];
+    int32_t width, height;
+};
+static_assert(sizeof(DevMaterial) == sizeof(eos_material), "material layout");
+static_assert(sizeof(DevLight) == sizeof(eos_light), "light layout");
+static_assert(sizeof(DevCamera) == sizeof(eos_camera), "camera layout");
+
+// Device view of one scene, passed by value to kernels.
+struct SceneView {
+    const Node* nodes;
+    const float4* geo;
+    const float4* shade;
+    const DevMaterial* mats;
+    const DevLight* lights;
+    int32_t root;
+    int32_t n_tri;
+    int32_t n_light;
+    int32_t n_mat;
+};
+
+// ---------------------------------------------------------------- buffers
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    eos_status ensure(size_t want);
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct LaunchCfg {
+    int workgroup = 256;
+    int stack_lds = 16;
+    int waves_per_cu = 32;
+    int refill = 16;
+};
+
+}  // namespace eos
+
+// Per-frame device buffers of the wavefront pipeline (sized for the largest
+// frame rendered so far on this scene).
+struct eos_frame_buffers {
+    eos::DevBuf hit0;       // float4 [n_ids]      level-0 hit records (t, slot, u, v)
+    eos::DevBuf q[2];       // float4 [cap][2]     ray queues (o.xyz pixel | d.xyz 0)
+    eos::DevBuf hitq;       // float4 [cap]        hit records of queued rays
+    eos::DevBuf surf[eos::kMaxStack];  // per level: float4 [cap][2] (P.xyz pixel | N.xyz mat)
+    eos::DevBuf vis[eos::kMaxStack];   // per level: uint8 [cap * L]
+    eos::DevBuf levcol;     // float4 [(D+1) * npix]  level colour rgb + reflection weight
+    eos::DevBuf levels;     // uint8  [npix]
+    eos::DevBuf counters;   // int64  [64]
+    eos::DevBuf gstack;     // int32  overflow stacks of persistent lanes
+    size_t npix = 0, n_ids = 0;
+    int depth = -1, n_light = -1;
+    int rec_valid = 0;      // records of the last frame kept (adjoint replay)
+    eos_camera rec_cam{};
+};
+
+struct eos_scene {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_tri = 0, n_mat = 0, n_light = 0;
+    // original-order inputs (device)
+    eos::DevBuf verts, normals, matidx;
+    eos::DevBuf mats, lights;
+    // LBVH and sorted layouts
+    eos::DevBuf nodes, geo, shade;
+    eos::DevBuf codes, codes_sorted, perm, kchild, kparent, first_last, ibox, lbox, flags, cbounds;
+    eos::DevBuf sort_tmp_k, sort_tmp_v, sort_hist, bounds_part;
+    int root = 0;
+    int leaf_max = 1;
+    bool built = false;
+    eos::LaunchCfg cfg;
+    int num_cu = 256;
+    eos_frame_buffers fb;
+    hipEvent_t ev[64] = {};
+};
+
+namespace eos {
+// Implemented in lbvh.hip
+eos_status build_lbvh(eos_scene* s, int leaf_max, eos_build_stats* st);
+// Implemented in render.hip
+eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
+                  eos_render_stats* st);
+eos_status render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint_opts* o, const float* dl,
+                          double* gm, double* gl, eos_adjoint_stats* st);
+eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n, const eos_batch_opts* o,
+                        uint8_t* const* out, eos_batch_stats* st);
+SceneView view_of(const eos_scene* s);
+}  // namespace eos

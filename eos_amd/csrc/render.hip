@@ -1,0 +1,474 @@
+// render.hip -- the Whitted render path as a wavefront pipeline on gfx950.
+//
+// Reference semantics: src/RayTracer.cpp:99-174 (frame loop, quantisation),
+// :268-300 (rayTrace), :302-416 (shadow rays, Phong sum, reflection),
+// :418-458 (calcPhong, findDiffuseColor).  The reference recursion
+//     c_l = sum_{unshadowed lights} phong_l  +  r_l * c_{l+1}
+// is split into stages per bounce level l = 0..depth:
+//   trace closest  (persistent, rays of level l)          -> hit records
+//   setup          (hit point, shading normal, material)  -> hit surfaces,
+//                  reflection rays of level l+1 (compacted queue)
+//   trace any-hit  (persistent, one ray per (hit, light)) -> visibility bytes
+//   phong          (sum over visible lights, light order) -> colour of level l
+// and a final resolve evaluates c_0 = P_0 + r_0 (P_1 + r_1 (...)) innermost
+// first, exactly the reference's order, then quantises.  Every queue length
+// lives in device counters, so a frame is one stream of launches with no host
+// round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "eos_internal.h"
+#include "trace.h"
+
+namespace eos {
+namespace {
+
+struct SetupArgs {
+    SceneView sc;
+    DevCamera cam;
+    int tiles_x, n_ids, depth, level;
+    const int* count_ptr;     // level >= 1: queued rays
+    const float4* q_in;       // level >= 1
+    const float4* hit_in;
+    float4* surf_out;
+    int* hits_ctr;
+    float4* q_out;
+    int* q_out_ctr;
+    uint8_t* levels;
+    int32_t* hit_ids;         // level 0, optional
+};
+
+// src/Triangle.cpp:41-47 and src/BVH.cpp:371-381: point o + t*d; normal
+// normalize(n1*(1-u-v) + n2*u + n3*v), normalised again after the (identity)
+// inverse-transpose transform.
+template <bool L0>
+__global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
+    const int n = L0 ? a.n_ids : *a.count_ptr;
+    const int lane = __lane_id();
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+        const int i = base + lane;
+        const bool valid = i < n;
+        float4 h = valid ? a.hit_in[i] : make_float4(INFINITY, __int_as_float(-1), 0.f, 0.f);
+        const int slot = __float_as_int(h.y);
+        int pixel = 0;
+        F3 o = {0, 0, 0}, d = {0, 0, 0};
+        bool pix_ok = valid;
+        if (L0) {
+            int x = 0, y = 0;
+            pix_ok = valid && tile_pixel(i, a.tiles_x, a.cam.width, a.cam.height, x, y);
+            pixel = y * a.cam.width + x;
+            o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
+            if (pix_ok) d = camera_dir(a.cam, x, y);
+        } else if (valid) {
+            const float4 q0 = a.q_in[2 * (size_t)i], q1 = a.q_in[2 * (size_t)i + 1];
+            o = f3(q0.x, q0.y, q0.z);
+            d = f3(q1.x, q1.y, q1.z);
+            pixel = __float_as_int(q0.w);
+        }
+        const bool hit = pix_ok && slot >= 0;
+        F3 P = {0, 0, 0}, N = {0, 0, 0};
+        int mat = 0;
+        bool refl = false;
+        if (hit) {
+            const float4 s0 = a.sc.shade[3 * (size_t)slot], s1 = a.sc.shade[3 * (size_t)slot + 1],
+                         s2 = a.sc.shade[3 * (size_t)slot + 2];
+            P = add(o, scale(h.x, d));
+            const float u = h.z, v = h.w;
+            const float w = (1.0f - u) - v;
+            const F3 na = {s0.x * w, s0.y * w, s0.z * w};
+            const F3 nb = {s1.x * u, s1.y * u, s1.z * u};
+            const F3 nc = {s2.x * v, s2.y * v, s2.z * v};
+            N = normalize3(normalize3(add(add(na, nb), nc)));
+            mat = __float_as_int(s0.w);
+            refl = a.sc.mats[mat].reflective != 0 && a.level < a.depth;
+        }
+        if (L0 && pix_ok) {
+            a.levels[pixel] = hit ? 1 : 0;
+            if (a.hit_ids) a.hit_ids[pixel] = hit ? __float_as_int(a.sc.geo[3 * (size_t)slot + 2].y) : -1;
+        } else if (!L0 && hit) {
+            a.levels[pixel] = (uint8_t)(a.level + 1);
+        }
+        const int j = wave_append(a.hits_ctr, hit);
+        if (hit) {
+            a.surf_out[2 * (size_t)j] = make_float4(P.x, P.y, P.z, __int_as_float(pixel));
+            a.surf_out[2 * (size_t)j + 1] = make_float4(N.x, N.y, N.z, __int_as_float(mat));
+        }
+        const int k = wave_append(a.q_out_ctr, refl);
+        if (refl) {
+            // src/RayTracer.cpp:350-355
+            const F3 R = normalize3(reflect3(d, N));
+            const F3 ro = add(P, scale(1e-3f, R));
+            a.q_out[2 * (size_t)k] = make_float4(ro.x, ro.y, ro.z, __int_as_float(pixel));
+            a.q_out[2 * (size_t)k + 1] = make_float4(R.x, R.y, R.z, 0.0f);
+        }
+    }
+}
+
+// src/RayTracer.cpp:418-458 for one light (w = 1 makes the vec4 forms equal to
+// the vec3 ones).
+__device__ __forceinline__ void phong(const DevMaterial& m, const DevLight& l, F3 cam, F3 P, F3 N, float out[3]) {
+    const F3 L = normalize3(sub(f3(l.position[0], l.position[1], l.position[2]), P));
+    const F3 V = normalize3(sub(cam, P));
+    const float dNL = dot3(N, L);
+    const float s2 = 2.0f * dNL;
+    const F3 R = normalize3(sub(f3(s2 * N.x, s2 * N.y, s2 * N.z), L));
+    float dLN = dot3(L, N);
+    dLN = dLN > 0.0f ? dLN : 0.0f;
+    const float dVR = dot3(V, R);
+    const bool sp = dVR > 0.0f;
+    const float st = sp ? powf(dVR, m.shininess) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float diff = (dLN * m.kd[k]) * l.color[k];
+        out[k] = sp ? diff + (st * l.color[k]) * m.ks[k] : diff + 0.0f;
+    }
+}
+
+struct PhongArgs {
+    SceneView sc;
+    DevCamera cam;
+    int level, npix;
+    const int* count_ptr;
+    const float4* surf;
+    const uint8_t* vis;
+    float4* levcol;
+};
+
+__global__ __launch_bounds__(256) void k_phong(PhongArgs a) {
+    const int n = *a.count_ptr;
+    const int L = a.sc.n_light;
+    const F3 cam = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const float4 s0 = a.surf[2 * (size_t)j], s1 = a.surf[2 * (size_t)j + 1];
+        const F3 P = f3(s0.x, s0.y, s0.z), N = f3(s1.x, s1.y, s1.z);
+        const int pixel = __float_as_int(s0.w), mi = __float_as_int(s1.w);
+        const DevMaterial& m = a.sc.mats[mi];
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < L; ++i) {
+            if (!a.vis[(size_t)j * L + i]) continue;
+            float c[3];
+            phong(m, a.sc.lights[i], cam, P, N, c);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[k] = acc[k] + c[k];
+        }
+        const float w = m.reflective ? m.reflectivity : 0.0f;
+        a.levcol[(size_t)a.level * a.npix + pixel] = make_float4(acc[0], acc[1], acc[2], w);
+    }
+}
+
+// src/RayTracer.cpp:150-153: floor(c == 1 ? 255 : min(c*256, 255)); c*256 is
+// exact in float, so the float form equals the reference's double form.
+__device__ __forceinline__ uint8_t quantize(float c) {
+    if (c == 1.0f) return 255;
+    const float q = fminf(floorf(c * 256.0f), 255.0f);
+    return q > 0.0f ? (uint8_t)q : 0;
+}
+
+struct ResolveArgs {
+    int npix;
+    const uint8_t* levels;
+    const float4* levcol;
+    uchar4* rgba;
+    float* rgb;
+};
+
+__global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.npix; p += gridDim.x * blockDim.x) {
+        const int nl = a.levels[p];
+        float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+        for (int l = nl - 1; l >= 0; --l) {
+            const float4 col = a.levcol[(size_t)l * a.npix + p];
+            c0 = col.x + col.w * c0;
+            c1 = col.y + col.w * c1;
+            c2 = col.z + col.w * c2;
+        }
+        if (a.rgba) a.rgba[p] = make_uchar4(quantize(c0), quantize(c1), quantize(c2), 255);
+        if (a.rgb) {
+            a.rgb[3 * (size_t)p] = c0;
+            a.rgb[3 * (size_t)p + 1] = c1;
+            a.rgb[3 * (size_t)p + 2] = c2;
+        }
+    }
+}
+
+__global__ void k_zero_i32(int* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// ---- traversal launch table ------------------------------------------------
+template <int MODE, bool COUNT>
+eos_status launch_trace(const LaunchCfg& cfg, int grid, hipStream_t s, const TraceArgs& a) {
+#define EOS_TRACE_CASE(WG, SL)                                                                 \
+    case WG * 100 + SL:                                                                         \
+        hipLaunchKernelGGL((k_trace<MODE, WG, SL, COUNT>), dim3(grid), dim3(WG), 0, s, a);      \
+        break;
+    if (COUNT) {
+        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT>), dim3(grid), dim3(256), 0, s, a);
+    } else {
+        switch (cfg.workgroup * 100 + cfg.stack_lds) {
+            EOS_TRACE_CASE(64, 8)
+            EOS_TRACE_CASE(64, 16)
+            EOS_TRACE_CASE(64, 32)
+            EOS_TRACE_CASE(128, 8)
+            EOS_TRACE_CASE(128, 16)
+            EOS_TRACE_CASE(128, 32)
+            EOS_TRACE_CASE(256, 8)
+            EOS_TRACE_CASE(256, 16)
+            EOS_TRACE_CASE(256, 32)
+            default:
+                return fail(EOS_ERR_INVALID, "unsupported launch configuration");
+        }
+    }
+#undef EOS_TRACE_CASE
+    return EOS_OK;
+}
+
+template <int MODE>
+eos_status launch_trace_any(bool count, const LaunchCfg& cfg, int grid, hipStream_t s, const TraceArgs& a) {
+    return count ? launch_trace<MODE, true>(cfg, grid, s, a) : launch_trace<MODE, false>(cfg, grid, s, a);
+}
+
+inline unsigned blocks_for(size_t n, unsigned t, unsigned cap) {
+    size_t b = (n + t - 1) / t;
+    if (b < 1) b = 1;
+    return (unsigned)std::min<size_t>(b, cap);
+}
+
+}  // namespace
+
+// Frame driver shared by eos_render and the adjoint's re-trace.
+static eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev,
+                            float* rgb_dev, int32_t* hits_dev, eos_render_stats* st) {
+    eos_frame_buffers& fb = s->fb;
+    const DevCamera cam = *reinterpret_cast<const DevCamera*>(camp);
+    const int W = cam.width, H = cam.height;
+    const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
+    const size_t npix = (size_t)W * H;
+    const size_t n_ids = (size_t)tiles_x * tiles_y * 64;
+    const int L = s->n_light;
+    const size_t cap = npix;
+    LaunchCfg cfg = s->cfg;
+    if (count) {
+        cfg.workgroup = 256;
+        cfg.stack_lds = 16;
+    }
+    const int trace_grid = std::max(1, s->num_cu * cfg.waves_per_cu / (cfg.workgroup / 64));
+    EOS_TRY(fb.hit0.ensure(16 * n_ids));
+    EOS_TRY(fb.q[0].ensure(32 * cap));
+    EOS_TRY(fb.q[1].ensure(32 * cap));
+    EOS_TRY(fb.hitq.ensure(16 * cap));
+    for (int l = 0; l <= depth; ++l) {
+        EOS_TRY(fb.surf[l].ensure(32 * cap));
+        EOS_TRY(fb.vis[l].ensure(cap * (size_t)std::max(L, 1)));
+    }
+    EOS_TRY(fb.levcol.ensure(16 * npix * (size_t)(depth + 1)));
+    EOS_TRY(fb.levels.ensure(npix));
+    EOS_TRY(fb.counters.ensure(4 * CTR_COUNT + 8 * kStats));
+    EOS_TRY(fb.gstack.ensure((size_t)trace_grid * cfg.workgroup * (kMaxStack - cfg.stack_lds) * 4));
+    fb.npix = npix;
+    fb.n_ids = n_ids;
+    fb.depth = depth;
+    fb.n_light = L;
+
+    hipStream_t S = s->stream;
+    int* ctr = fb.counters.as<int>();
+    unsigned long long* stats = reinterpret_cast<unsigned long long*>(fb.counters.as<char>() + 4 * CTR_COUNT);
+    const SceneView sc = view_of(s);
+    const unsigned shade_grid = (unsigned)s->num_cu * 8;
+
+    int ev = 0;
+    hipEvent_t* E = s->ev;
+    auto rec = [&](void) -> int {
+        (void)hipEventRecord(E[ev], S);
+        return ev++;
+    };
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, S, ctr, CTR_COUNT + 2 * kStats);
+    const int ev_start = rec();
+    int ev_c[2 * EOS_MAX_DEPTH + 2], ev_s[2 * EOS_MAX_DEPTH + 2], ev_p[2 * EOS_MAX_DEPTH + 2];
+
+    for (int l = 0; l <= depth; ++l) {
+        float4* qin = fb.q[l & 1].as<float4>();
+        float4* qout = fb.q[(l + 1) & 1].as<float4>();
+        TraceArgs ta{};
+        ta.sc = sc;
+        ta.cam = cam;
+        ta.tiles_x = tiles_x;
+        ta.gstack = fb.gstack.as<int32_t>();
+        ta.err = ctr + CTR_ERR;
+        ta.refill = cfg.refill;
+        // closest hit
+        ta.cursor = ctr + CTR_CUR_C + l;
+        ta.stats = stats + 2 * l;
+        if (l == 0) {
+            ta.n_static = (int)n_ids;
+            ta.count_ptr = nullptr;
+            ta.count_mul = 1;
+            ta.hit_out = fb.hit0.as<float4>();
+        } else {
+            ta.count_ptr = ctr + CTR_Q + l;
+            ta.count_mul = 1;
+            ta.q = qin;
+            ta.hit_out = fb.hitq.as<float4>();
+        }
+        ev_c[2 * l] = rec();
+        if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(count, cfg, trace_grid, S, ta));
+        else EOS_TRY(launch_trace_any<MODE_QUEUE>(count, cfg, trace_grid, S, ta));
+        ev_c[2 * l + 1] = rec();
+        // hit setup + reflection rays
+        SetupArgs sa{};
+        sa.sc = sc;
+        sa.cam = cam;
+        sa.tiles_x = tiles_x;
+        sa.n_ids = (int)n_ids;
+        sa.depth = depth;
+        sa.level = l;
+        sa.count_ptr = ctr + CTR_Q + l;
+        sa.q_in = qin;
+        sa.hit_in = l == 0 ? fb.hit0.as<float4>() : fb.hitq.as<float4>();
+        sa.surf_out = fb.surf[l].as<float4>();
+        sa.hits_ctr = ctr + CTR_HITS + l;
+        sa.q_out = qout;
+        sa.q_out_ctr = ctr + CTR_Q + l + 1;
+        sa.levels = fb.levels.as<uint8_t>();
+        sa.hit_ids = l == 0 ? hits_dev : nullptr;
+        ev_p[2 * l] = rec();
+        if (l == 0)
+            hipLaunchKernelGGL(k_setup<true>, dim3(blocks_for(n_ids, 256, shade_grid)), dim3(256), 0, S, sa);
+        else
+            hipLaunchKernelGGL(k_setup<false>, dim3(shade_grid), dim3(256), 0, S, sa);
+        // shadow rays
+        ev_s[2 * l] = rec();
+        if (L > 0) {
+            TraceArgs sh = ta;
+            sh.cursor = ctr + CTR_CUR_S + l;
+            sh.stats = stats + 2 * (16 + l);
+            sh.count_ptr = ctr + CTR_HITS + l;
+            sh.count_mul = L;
+            sh.surf = fb.surf[l].as<float4>();
+            sh.vis_out = fb.vis[l].as<uint8_t>();
+            EOS_TRY(launch_trace_any<MODE_SHADOW>(count, cfg, trace_grid, S, sh));
+        }
+        ev_s[2 * l + 1] = rec();
+        PhongArgs pa{};
+        pa.sc = sc;
+        pa.cam = cam;
+        pa.level = l;
+        pa.npix = (int)npix;
+        pa.count_ptr = ctr + CTR_HITS + l;
+        pa.surf = fb.surf[l].as<float4>();
+        pa.vis = fb.vis[l].as<uint8_t>();
+        pa.levcol = fb.levcol.as<float4>();
+        hipLaunchKernelGGL(k_phong, dim3(shade_grid), dim3(256), 0, S, pa);
+        ev_p[2 * l + 1] = rec();
+    }
+    ResolveArgs ra{(int)npix, fb.levels.as<uint8_t>(), fb.levcol.as<float4>(), rgba_dev, rgb_dev};
+    const int ev_r0 = rec();
+    hipLaunchKernelGGL(k_resolve, dim3(blocks_for(npix, 256, shade_grid)), dim3(256), 0, S, ra);
+    const int ev_end = rec();
+    EOS_HIP(hipGetLastError());
+
+    int hctr[CTR_COUNT];
+    unsigned long long hstats[kStats];
+    EOS_HIP(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, S));
+    EOS_HIP(hipMemcpyAsync(hstats, stats, sizeof hstats, hipMemcpyDeviceToHost, S));
+    EOS_HIP(hipStreamSynchronize(S));
+    if (hctr[CTR_ERR] & 1) return fail(EOS_ERR_LIMIT, "traversal stack overflow (tree deeper than 64)");
+    if (hctr[CTR_ERR] & 2) return fail(EOS_ERR_STATE, "traversal exceeded its visit bound (corrupt BVH)");
+    if (st) {
+        std::memset(st, 0, sizeof *st);
+        st->primary_rays = (int64_t)npix;
+        for (int l = 0; l <= depth; ++l) {
+            st->hits += hctr[CTR_HITS + l];
+            st->shadow_rays += (int64_t)hctr[CTR_HITS + l] * L;
+            if (l >= 1) st->secondary_rays += hctr[CTR_Q + l];
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, E[ev_start], E[ev_end]);
+        st->render_ms = ms;
+        for (int l = 0; l <= depth; ++l) {
+            float a = 0.f, b = 0.f, c = 0.f, d2 = 0.f;
+            (void)hipEventElapsedTime(&a, E[ev_c[2 * l]], E[ev_c[2 * l + 1]]);
+            (void)hipEventElapsedTime(&b, E[ev_s[2 * l]], E[ev_s[2 * l + 1]]);
+            (void)hipEventElapsedTime(&c, E[ev_p[2 * l]], E[ev_s[2 * l]]);
+            (void)hipEventElapsedTime(&d2, E[ev_s[2 * l + 1]], E[ev_p[2 * l + 1]]);
+            st->trace_closest_ms += a;
+            st->trace_shadow_ms += b;
+            st->shade_ms += c + d2;
+            if (l == 0) st->trace_first_ms = a;
+        }
+        float r = 0.f;
+        (void)hipEventElapsedTime(&r, E[ev_r0], E[ev_end]);
+        st->shade_ms += r;
+        st->trace_closest_launches = depth + 1;
+        st->trace_shadow_launches = L > 0 ? depth + 1 : 0;
+        if (count) {
+            for (int l = 0; l <= depth; ++l) {
+                st->node_visits += (int64_t)(hstats[2 * l] + hstats[2 * (16 + l)]);
+                st->tri_tests += (int64_t)(hstats[2 * l + 1] + hstats[2 * (16 + l) + 1]);
+            }
+            st->closest_visits_first = (int64_t)hstats[0];
+            st->closest_tests_first = (int64_t)hstats[1];
+        }
+    }
+    return EOS_OK;
+}
+
+eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
+                  eos_render_stats* st) {
+    eos_frame_buffers& fb = s->fb;
+    const size_t npix = (size_t)cam->width * cam->height;
+    const bool dev_out = o->output_memory == EOS_MEM_DEVICE;
+    static thread_local DevBuf stage_rgba, stage_rgb, stage_hit;
+    uchar4* rgba_dev = nullptr;
+    float* rgb_dev = nullptr;
+    int32_t* hit_dev = nullptr;
+    if (rgba) {
+        if (dev_out) rgba_dev = reinterpret_cast<uchar4*>(rgba);
+        else {
+            EOS_TRY(stage_rgba.ensure(4 * npix));
+            rgba_dev = stage_rgba.as<uchar4>();
+        }
+    }
+    if (o->rgb_out) {
+        if (dev_out) rgb_dev = o->rgb_out;
+        else {
+            EOS_TRY(stage_rgb.ensure(12 * npix));
+            rgb_dev = stage_rgb.as<float>();
+        }
+    }
+    if (o->hit_out) {
+        if (dev_out) hit_dev = o->hit_out;
+        else {
+            EOS_TRY(stage_hit.ensure(4 * npix));
+            hit_dev = stage_hit.as<int32_t>();
+        }
+    }
+    fb.rec_valid = 0;
+    EOS_TRY(run_frame(s, cam, o->depth, o->count_visits != 0, rgba_dev, rgb_dev, hit_dev, st));
+    if (!dev_out) {
+        if (rgba) EOS_HIP(hipMemcpyAsync(rgba, rgba_dev, 4 * npix, hipMemcpyDeviceToHost, s->stream));
+        if (o->rgb_out) EOS_HIP(hipMemcpyAsync(o->rgb_out, rgb_dev, 12 * npix, hipMemcpyDeviceToHost, s->stream));
+        if (o->hit_out) EOS_HIP(hipMemcpyAsync(o->hit_out, hit_dev, 4 * npix, hipMemcpyDeviceToHost, s->stream));
+        EOS_HIP(hipStreamSynchronize(s->stream));
+    }
+    if (o->keep_records) {
+        fb.rec_valid = 1;
+        fb.rec_cam = *cam;
+    }
+    return EOS_OK;
+}
+
+eos_status render_adjoint(eos_scene*, const eos_camera*, const eos_adjoint_opts*, const float*, double*, double*,
+                          eos_adjoint_stats*) {
+    return fail(EOS_ERR_STATE, "eos_render_adjoint: not built yet");
+}
+
+eos_status render_batch(eos_scene* const*, const eos_camera*, int, const eos_batch_opts*, uint8_t* const*,
+                        eos_batch_stats*) {
+    return fail(EOS_ERR_STATE, "eos_render_batch: not built yet");
+}
+
+}  // namespace eos

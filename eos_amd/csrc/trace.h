@@ -1,0 +1,324 @@
+// trace.h -- device building blocks of the render path: reference arithmetic
+// (camera rays, Moller-Trumbore, shading) and the persistent traversal kernel.
+//
+// Float expressions follow the reference's glm operation order exactly and the
+// sources are compiled with -ffp-contract=off, so (apart from powf) every value
+// is bit-identical to the CPU RayTracer's:
+//   dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z   glm/detail/func_geometric.inl:52-58
+//   cross, normalize = x * (1/sqrt(dot(x,x)))   func_geometric.inl:134-158
+//   reflect(I,N) = I - N*dot(N,I)*2             func_geometric.inl:170-173
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "eos_internal.h"
+
+namespace eos {
+
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 scale(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float dot3(F3 a, F3 b) {
+    float tx = a.x * b.x, ty = a.y * b.y, tz = a.z * b.z;
+    return (tx + ty) + tz;
+}
+__device__ __forceinline__ F3 cross3(F3 x, F3 y) {
+    return {x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
+}
+__device__ __forceinline__ F3 normalize3(F3 x) {
+    float s = 1.0f / sqrtf(dot3(x, x));
+    return {x.x * s, x.y * s, x.z * s};
+}
+__device__ __forceinline__ F3 reflect3(F3 I, F3 N) {
+    float d = dot3(N, I);
+    F3 t = {(N.x * d) * 2.0f, (N.y * d) * 2.0f, (N.z * d) * 2.0f};
+    return sub(I, t);
+}
+
+// src/RayTracer.cpp:119-142: aa = (x - W/2) * (1/(W/2)), bb likewise,
+// dir = normalize(aa*right + bb*up + view).  Row 0 is the bottom row.
+__device__ __forceinline__ F3 camera_dir(const DevCamera& c, int x, int y) {
+    const float nw = (float)c.width / 2.0f, nh = (float)c.height / 2.0f;
+    const float inw = 1.0f / nw, inh = 1.0f / nh;
+    const float bb = ((float)y - nh) * inh;
+    const float aa = ((float)x - nw) * inw;
+    const F3 r = {c.right[0], c.right[1], c.right[2]};
+    const F3 u = {c.up[0], c.up[1], c.up[2]};
+    const F3 v = {c.view[0], c.view[1], c.view[2]};
+    return normalize3(add(add(scale(aa, r), scale(bb, u)), v));
+}
+
+// Primary rays are numbered in 8x8 pixel tiles (one wave per tile).
+__device__ __forceinline__ bool tile_pixel(int r, int tiles_x, int W, int H, int& x, int& y) {
+    const int t = r >> 6, i = r & 63;
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    x = tx * 8 + (i & 7);
+    y = ty * 8 + (i >> 3);
+    return x < W && y < H;
+}
+
+// src/Triangle.cpp:57-91 with e1 = v2 - v1, e2 = v3 - v1 precomputed (the same
+// float subtractions, done once at build time).
+__device__ __forceinline__ bool moller_trumbore(F3 o, F3 d, F3 v1, F3 e1, F3 e2, float& t, float& u, float& v) {
+    const F3 P = cross3(d, e2);
+    const float det = dot3(e1, P);
+    if (det > -0.00001f && det < 0.00001f) return false;
+    const float inv = 1.0f / det;
+    const F3 T = sub(o, v1);
+    const F3 Q = cross3(T, e1);
+    t = dot3(e2, Q) * inv;
+    if (t < 0.0f) return false;
+    u = dot3(T, P) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    v = dot3(d, Q) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    return true;
+}
+
+__device__ __forceinline__ float safe_inv(float d) {
+    float x = fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
+    return 1.0f / x;
+}
+
+// Ray/box slab test against [0, tmax], lo*invd - o*invd in one fma per plane.
+// Boxes are inflated at build time, so the test is conservative with respect to
+// the float Moller-Trumbore test.
+__device__ __forceinline__ bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz, float ix,
+                                     float iy, float iz, float px, float py, float pz, float tmax, float& tmin) {
+    const float x0 = __builtin_fmaf(lox, ix, -px), x1 = __builtin_fmaf(hix, ix, -px);
+    const float y0 = __builtin_fmaf(loy, iy, -py), y1 = __builtin_fmaf(hiy, iy, -py);
+    const float z0 = __builtin_fmaf(loz, iz, -pz), z1 = __builtin_fmaf(hiz, iz, -pz);
+    const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+    const float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    tmin = a;
+    return a <= b;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Wave-aggregated append: one atomic per wave, lanes get consecutive slots in
+// lane order.  Every lane of the wave must call it.
+__device__ __forceinline__ int wave_append(int* ctr, bool pred) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return -1;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (__lane_id() == leader) base = atomicAdd(ctr, __popcll(m));
+    base = __shfl(base, leader);
+    return pred ? base + __popcll(m & lanemask_lt()) : -1;
+}
+
+// Counter slots (int32) of one frame.
+enum : int {
+    CTR_Q = 0,         // + level: rays queued for level (level >= 1)
+    CTR_HITS = 16,     // + level: hits at level
+    CTR_CUR_C = 32,    // + level: fetch cursor of the closest-hit launch
+    CTR_CUR_S = 48,    // + level: fetch cursor of the any-hit launch
+    CTR_ERR = 64,      // stack overflow / invalid state
+    CTR_COUNT = 128
+};
+// int64 statistics (count_visits): [2*launch] visits, [2*launch+1] tests;
+// launch = level (closest) or 16 + level (any hit).
+constexpr int kStats = 64;
+
+enum TraceMode : int { MODE_PRIMARY = 0, MODE_QUEUE = 1, MODE_SHADOW = 2 };
+
+struct TraceArgs {
+    SceneView sc;
+    DevCamera cam;
+    int tiles_x;
+    int n_static;            // item count when count_ptr is null
+    const int* count_ptr;    // device item count (queued rays / hits)
+    int count_mul;           // items per counted entry (lights for shadow rays)
+    int* cursor;             // persistent fetch cursor
+    const float4* q;         // MODE_QUEUE: ray queue
+    float4* hit_out;         // closest: (t, slot, u, v)
+    const float4* surf;      // MODE_SHADOW: hit surfaces (P.xyz pixel | N.xyz mat)
+    uint8_t* vis_out;        // MODE_SHADOW: 1 = light visible
+    int32_t* gstack;         // overflow stacks
+    int* err;
+    unsigned long long* stats;
+    int refill;
+};
+
+// Persistent traversal.  Each wave keeps up to 64 independent rays in flight;
+// whenever at least `refill` lanes are idle it claims that many new items from
+// the fetch cursor with one atomic (ballot + mbcnt ranks the idle lanes), so
+// lanes never wait for the slowest ray of a fixed batch.  Per lane: ordered
+// near-first descent, far child pushed onto a short stack in LDS (STACK_LDS
+// entries, column per lane) that overflows into a private global area, pruning
+// against the best hit so far.  Per-ray visit order is deterministic and equal
+// to the CPU restatement in the oracle.
+template <int MODE, int WG, int STACK_LDS, bool COUNT>
+__global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
+    constexpr bool ANY = MODE == MODE_SHADOW;
+    __shared__ int32_t lds[STACK_LDS * WG];
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    int32_t* gst = a.gstack + (size_t)(blockIdx.x * WG + tid) * (kMaxStack - STACK_LDS);
+    const int n = a.count_ptr ? (*a.count_ptr) * a.count_mul : a.n_static;
+    const Node* __restrict__ nodes = a.sc.nodes;
+    const float4* __restrict__ geo = a.sc.geo;
+
+    int item = -1;
+    bool drained = false;
+    F3 o = {0, 0, 0}, d = {0, 0, 0};
+    float ix = 0, iy = 0, iz = 0, px = 0, py = 0, pz = 0;
+    float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
+    int best_slot = -1, best_orig = 0x7FFFFFFF;
+    int cur = kDone, sp = 0;
+    int budget = 0;  // node visits left for this ray: a tree visits each internal node at most once
+    const int max_visits = a.sc.n_tri + 64;
+    unsigned long long visits = 0, tests = 0;
+    const int thresh = a.refill < 1 ? 1 : (a.refill > 64 ? 64 : a.refill);
+
+    for (;;) {
+        // ---- refill idle lanes
+        const uint64_t idle = __ballot(item < 0);
+        const int nidle = __popcll(idle);
+        if (!drained && nidle >= thresh) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(a.cursor, nidle);
+            base = __shfl(base, leader);
+            if (base + nidle >= n) drained = true;
+            if (item < 0) {
+                const int my = base + __popcll(idle & lanemask_lt());
+                if (my < n) {
+                    item = my;
+                    bool valid = true;
+                    if (MODE == MODE_PRIMARY) {
+                        int x, y;
+                        valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
+                        o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
+                        d = camera_dir(a.cam, x, y);
+                    } else if (MODE == MODE_QUEUE) {
+                        const float4 q0 = a.q[2 * (size_t)my], q1 = a.q[2 * (size_t)my + 1];
+                        o = f3(q0.x, q0.y, q0.z);
+                        d = f3(q1.x, q1.y, q1.z);
+                    } else {
+                        const int j = my / a.sc.n_light, li = my - j * a.sc.n_light;
+                        const float4 s0 = a.surf[2 * (size_t)j];
+                        const F3 P = f3(s0.x, s0.y, s0.z);
+                        const DevLight& L = a.sc.lights[li];
+                        d = normalize3(sub(f3(L.position[0], L.position[1], L.position[2]), P));
+                        o = add(P, scale(1e-3f, d));
+                    }
+                    ix = safe_inv(d.x);
+                    iy = safe_inv(d.y);
+                    iz = safe_inv(d.z);
+                    px = o.x * ix;
+                    py = o.y * iy;
+                    pz = o.z * iz;
+                    best_t = INFINITY;
+                    best_slot = -1;
+                    best_orig = 0x7FFFFFFF;
+                    bu = bv = 0.0f;
+                    sp = 0;
+                    budget = max_visits;
+                    cur = valid ? a.sc.root : kDone;
+                }
+            }
+        }
+        if (__ballot(item >= 0) == 0) {
+            if (drained) break;
+            continue;
+        }
+        // ---- traverse
+        if (item >= 0) {
+            for (int step = 0; step < 32; ++step) {
+                if (cur >= 0 && cur != kDone && --budget < 0) {
+                    atomicOr(a.err, 2);
+                    cur = kDone;
+                } else if (cur >= 0 && cur != kDone) {
+                    if (COUNT) ++visits;
+                    const Node* nd = nodes + cur;
+                    const float4 b0 = nd->b0, b1 = nd->b1, b2 = nd->b2;
+                    const int4 c = nd->c;
+                    const float tmax = ANY ? INFINITY : best_t;
+                    float t0, t1;
+                    const bool h0 = slab(b0.x, b0.y, b0.z, b0.w, b2.x, b2.y, ix, iy, iz, px, py, pz, tmax, t0);
+                    const bool h1 = slab(b1.x, b1.y, b1.z, b1.w, b2.z, b2.w, ix, iy, iz, px, py, pz, tmax, t1);
+                    if (h0 && h1) {
+                        const bool swap = t1 < t0;
+                        const int nearc = swap ? c.y : c.x, farc = swap ? c.x : c.y;
+                        if (sp >= kMaxStack) {
+                            atomicOr(a.err, 1);
+                            cur = kDone;
+                        } else {
+                            if (sp < STACK_LDS) lds[sp * WG + tid] = farc;
+                            else gst[sp - STACK_LDS] = farc;
+                            ++sp;
+                            cur = nearc;
+                        }
+                    } else if (h0) {
+                        cur = c.x;
+                    } else if (h1) {
+                        cur = c.y;
+                    } else {
+                        cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
+                    }
+                } else if (cur != kDone) {
+                    const int e = ~cur;
+                    const int first = e & ((1 << kLeafShift) - 1);
+                    const int cnt = ((e >> kLeafShift) & 7) + 1;
+                    bool occluded = false;
+                    for (int s = first; s < first + cnt; ++s) {
+                        if (COUNT) ++tests;
+                        const float4 g0 = geo[3 * (size_t)s], g1 = geo[3 * (size_t)s + 1], g2 = geo[3 * (size_t)s + 2];
+                        float t, u, v;
+                        if (moller_trumbore(o, d, f3(g0.x, g0.y, g0.z), f3(g0.w, g1.x, g1.y), f3(g1.z, g1.w, g2.x), t,
+                                            u, v)) {
+                            if (ANY) {
+                                occluded = true;
+                                break;
+                            }
+                            const int orig = __float_as_int(g2.y);
+                            if (t < best_t || (t == best_t && orig < best_orig)) {
+                                best_t = t;
+                                best_orig = orig;
+                                best_slot = s;
+                                bu = u;
+                                bv = v;
+                            }
+                        }
+                    }
+                    if (ANY && occluded) {
+                        best_slot = 0;  // marks "hit" for the any-hit result
+                        cur = kDone;
+                    } else {
+                        cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
+                    }
+                }
+                if (cur == kDone) {
+                    if (ANY) {
+                        a.vis_out[item] = best_slot >= 0 ? 0 : 1;
+                    } else {
+                        a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
+                    }
+                    item = -1;
+                    break;
+                }
+            }
+        }
+    }
+    if (COUNT) {
+        // wave-reduce the counters, one atomic per wave
+        for (int off = 32; off > 0; off >>= 1) {
+            visits += __shfl_xor(visits, off);
+            tests += __shfl_xor(tests, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats[0], visits);
+            atomicAdd(&a.stats[1], tests);
+        }
+    }
+}
+
+}  // namespace eos

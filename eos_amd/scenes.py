@@ -1,0 +1,468 @@
+"""Seeded procedural scenes for the five rendering workloads.
+
+Every scene is produced from SplitMix64 streams (a counter-based generator, so
+numpy evaluates it vectorised and the output does not depend on the platform's
+``random`` implementation).  Uniform floats take the top 24 bits of a draw;
+normal deviates use double-precision Box-Muller.
+
+A scene is plain arrays in the layout the C ABI (``include/eos_c.h``) takes:
+
+* ``verts``   float32 [N, 3, 3]  triangle vertices v1, v2, v3
+  (reference ``Triangle`` m_V1..m_V3, src/Triangle.h:73-75)
+* ``normals`` float32 [N, 3, 3]  per-vertex normals n1, n2, n3 (m_N1..m_N3)
+* ``mat``     int32   [N]        material index per triangle (m_MaterialIndex)
+* ``materials`` MATERIAL_DTYPE [M]  kd, ks, shininess, reflectivity, reflective
+  (reference ``Material``, src/Material.h:43-78)
+* ``lights``  LIGHT_DTYPE [L]    position (w = 1) and colour
+  (reference ``LightSource``, src/LightSource.h:27-39)
+* ``camera``  CAMERA_DTYPE       position, viewing direction, right, up, W, H
+  (reference ``Camera``, src/Camera.h:55-63; no field of view: the lengths of
+  ``right`` and ``up`` set it, as in src/RayTracer.cpp:135-141)
+
+All transforms are the identity (the reference's per-triangle transform is a
+mat4 that the workloads never set to anything else).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MATERIAL_DTYPE = np.dtype([
+    ("kd", "<f4", (4,)), ("ks", "<f4", (4,)), ("shininess", "<f4"),
+    ("reflectivity", "<f4"), ("reflective", "<i4"), ("pad", "<i4")])
+LIGHT_DTYPE = np.dtype([("position", "<f4", (4,)), ("color", "<f4", (4,))])
+CAMERA_DTYPE = np.dtype([
+    ("position", "<f4", (3,)), ("view", "<f4", (3,)), ("right", "<f4", (3,)),
+    ("up", "<f4", (3,)), ("width", "<i4"), ("height", "<i4")])
+assert MATERIAL_DTYPE.itemsize == 48 and LIGHT_DTYPE.itemsize == 32
+assert CAMERA_DTYPE.itemsize == 56
+
+BASE_SEED = 0x456F73          # "Eos"
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """Draws ``offset+1 .. offset+n`` of the SplitMix64 sequence started at ``seed``."""
+    with np.errstate(over="ignore"):
+        i = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + i * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def mix64(z: np.ndarray) -> np.ndarray:
+    """The SplitMix64 finaliser applied to arbitrary 64-bit keys."""
+    with np.errstate(over="ignore"):
+        z = z.astype(np.uint64) + _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+class Rng:
+    """A SplitMix64 stream; each call consumes draws in order."""
+
+    def __init__(self, seed: int):
+        self.seed = seed & 0xFFFFFFFFFFFFFFFF
+        self.pos = 0
+
+    def bits(self, n: int) -> np.ndarray:
+        out = splitmix64(self.seed, n, self.pos)
+        self.pos += n
+        return out
+
+    def uniform(self, n: int) -> np.ndarray:
+        """U[0,1) from the top 24 bits (exact in float32)."""
+        return (self.bits(n) >> np.uint64(40)).astype(np.float64) * (1.0 / (1 << 24))
+
+    def normal(self, n: int) -> np.ndarray:
+        """N(0,1) by Box-Muller in float64; each pair of draws yields cos and sin branches."""
+        m = (n + 1) // 2
+        b = self.bits(2 * m)
+        u1 = ((b[0::2] >> np.uint64(11)).astype(np.float64) + 1.0) * (1.0 / (1 << 53))
+        u2 = (b[1::2] >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+        r = np.sqrt(-2.0 * np.log(u1))
+        out = np.empty(2 * m)
+        out[0::2] = r * np.cos(2.0 * math.pi * u2)
+        out[1::2] = r * np.sin(2.0 * math.pi * u2)
+        return out[:n]
+
+
+@dataclass
+class SceneData:
+    verts: np.ndarray
+    normals: np.ndarray
+    mat: np.ndarray
+    materials: np.ndarray
+    lights: np.ndarray
+    camera: np.ndarray
+    depth: int
+    name: str = ""
+    info: dict = field(default_factory=dict)
+
+    @property
+    def n_tri(self) -> int:
+        return int(self.verts.shape[0])
+
+    @property
+    def width(self) -> int:
+        return int(self.camera["width"])
+
+    @property
+    def height(self) -> int:
+        return int(self.camera["height"])
+
+    def with_camera(self, width: int, height: int) -> "SceneData":
+        """Same scene, camera re-aimed for a new resolution (same vertical FOV)."""
+        cam = self.camera.copy()
+        up_len = float(np.linalg.norm(cam["up"].astype(np.float64)))
+        r = cam["right"].astype(np.float64)
+        r = r / np.linalg.norm(r) * up_len * width / height
+        cam["right"] = r.astype(np.float32)
+        cam["width"] = width
+        cam["height"] = height
+        return SceneData(self.verts, self.normals, self.mat, self.materials, self.lights,
+                         cam, self.depth, self.name, dict(self.info))
+
+
+# ----------------------------------------------------------------- builders
+
+def make_camera(eye, target, vfov_deg: float, width: int, height: int,
+                world_up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """A reference-style camera: unit view, |up| = tan(vfov/2), |right| = aspect*|up|."""
+    eye = np.asarray(eye, np.float64)
+    view = np.asarray(target, np.float64) - eye
+    view /= np.linalg.norm(view)
+    wu = np.asarray(world_up, np.float64)
+    up = wu - np.dot(wu, view) * view
+    up /= np.linalg.norm(up)
+    right = np.cross(view, up)
+    right /= np.linalg.norm(right)
+    th = math.tan(math.radians(vfov_deg) * 0.5)
+    cam = np.zeros((), CAMERA_DTYPE)
+    cam["position"] = eye.astype(np.float32)
+    cam["view"] = view.astype(np.float32)
+    cam["up"] = (up * th).astype(np.float32)
+    cam["right"] = (right * th * width / height).astype(np.float32)
+    cam["width"] = width
+    cam["height"] = height
+    return cam
+
+
+def make_material(kd, ks, shininess, reflectivity=0.0, reflective=False) -> np.ndarray:
+    m = np.zeros((), MATERIAL_DTYPE)
+    m["kd"] = list(kd) + [1.0] if len(kd) == 3 else kd
+    m["ks"] = list(ks) + [1.0] if len(ks) == 3 else ks
+    m["shininess"] = shininess
+    m["reflectivity"] = reflectivity
+    m["reflective"] = 1 if reflective else 0
+    return m
+
+
+def make_lights(positions, colors=None) -> np.ndarray:
+    positions = np.asarray(positions, np.float64).reshape(-1, 3)
+    out = np.zeros(len(positions), LIGHT_DTYPE)
+    out["position"][:, :3] = positions
+    out["position"][:, 3] = 1.0
+    if colors is None:
+        out["color"][:] = 1.0
+    else:
+        c = np.asarray(colors, np.float64).reshape(-1, 3)
+        out["color"][:, :3] = c
+        out["color"][:, 3] = 1.0
+    return out
+
+
+def face_normals(verts: np.ndarray) -> np.ndarray:
+    v = verts.astype(np.float64)
+    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    bad = ln[:, 0] == 0.0
+    ln[bad] = 1.0
+    n = n / ln
+    n[bad] = (0.0, 0.0, 1.0)
+    return np.repeat(n[:, None, :], 3, axis=1).astype(np.float32)
+
+
+def icosphere(level: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)):
+    """Subdivided icosahedron: 20*4**level triangles, smooth normals n = v/|v|."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    vs = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+          (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in vs]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9),
+             (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2),
+             (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+             (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts)
+    F = np.array(faces)
+    tri = V[F]                              # [n,3,3] unit positions
+    nrm = tri.copy()
+    pos = tri * radius + np.asarray(center, np.float64)
+    return pos.astype(np.float32), nrm.astype(np.float32)
+
+
+def quad_grid(cells: int, half: float, y: float):
+    """cells x cells quads on the plane y over x,z in [-half, half]; normal +y."""
+    g = np.linspace(-half, half, cells + 1)
+    x0, z0 = np.meshgrid(g[:-1], g[:-1], indexing="ij")
+    x1, z1 = np.meshgrid(g[1:], g[1:], indexing="ij")
+    x0, z0, x1, z1 = (a.ravel() for a in (x0, z0, x1, z1))
+    yy = np.full_like(x0, y)
+    a = np.stack([x0, yy, z0], -1)
+    b = np.stack([x1, yy, z0], -1)
+    c = np.stack([x1, yy, z1], -1)
+    d = np.stack([x0, yy, z1], -1)
+    t1 = np.stack([a, c, b], 1)
+    t2 = np.stack([a, d, c], 1)
+    verts = np.empty((2 * len(x0), 3, 3))
+    verts[0::2] = t1
+    verts[1::2] = t2
+    normals = np.zeros_like(verts)
+    normals[..., 1] = 1.0
+    return verts.astype(np.float32), normals.astype(np.float32)
+
+
+def soup(n: int, seed: int, sigma_median: float = 0.004, sigma_log_sd: float = 0.5):
+    """n triangles: centroids U[0,1]^3, vertices c + s*N(0,I), s log-normal."""
+    rc = Rng(seed ^ 0x1)
+    rs = Rng(seed ^ 0x2)
+    rv = Rng(seed ^ 0x3)
+    c = rc.uniform(3 * n).reshape(n, 3)
+    s = np.exp(math.log(sigma_median) + sigma_log_sd * rs.normal(n))
+    off = rv.normal(9 * n).reshape(n, 3, 3)
+    verts = (c[:, None, :] + s[:, None, None] * off).astype(np.float32)
+    return verts, face_normals(verts)
+
+
+def _value_noise(p: np.ndarray, seed: int) -> np.ndarray:
+    """Trilinear value noise in [-1,1] with smoothstep fade; lattice values hashed by SplitMix64."""
+    i = np.floor(p)
+    f = p - i
+    w = f * f * (3.0 - 2.0 * f)
+    ii = i.astype(np.int64)
+    lo = ii.min(axis=0)
+    hi = ii.max(axis=0) + 1
+    ext = hi - lo + 1
+    gx, gy, gz = np.meshgrid(*(np.arange(lo[k], hi[k] + 1) for k in range(3)), indexing="ij")
+    key = ((gx & 0x1FFFFF).astype(np.uint64)
+           | ((gy & 0x1FFFFF).astype(np.uint64) << np.uint64(21))
+           | ((gz & 0x1FFFFF).astype(np.uint64) << np.uint64(42)))
+    with np.errstate(over="ignore"):
+        key = key ^ (np.uint64(seed & 0xFFFFFFFFFFFFFFFF) * _GOLDEN)
+    lat = (mix64(key) >> np.uint64(40)).astype(np.float64) * (2.0 / (1 << 24)) - 1.0
+    lat = lat.reshape(ext[0], ext[1], ext[2])
+    rel = ii - lo
+    out = np.zeros(len(p))
+    for dx in (0, 1):
+        wx = w[:, 0] if dx else 1.0 - w[:, 0]
+        for dy in (0, 1):
+            wy = w[:, 1] if dy else 1.0 - w[:, 1]
+            for dz in (0, 1):
+                wz = w[:, 2] if dz else 1.0 - w[:, 2]
+                out += wx * wy * wz * lat[rel[:, 0] + dx, rel[:, 1] + dy, rel[:, 2] + dz]
+    return out
+
+
+def displaced_sphere(nu: int, nv: int, seed: int, amplitude: float = 0.15, octaves: int = 6):
+    """UV sphere (nu around, nv pole to pole, 2*nu*nv triangles) displaced radially by
+    r = 1 + amplitude * sum_o 0.5**o * noise(2**o * u).  Returns verts, normals and the
+    per-triangle mean radius (used to band materials)."""
+    theta = np.linspace(0.0, math.pi, nv + 1)               # from +y pole
+    phi = np.arange(nu) * (2.0 * math.pi / nu)
+    th, ph = np.meshgrid(theta, phi, indexing="ij")          # [nv+1, nu]
+    u = np.stack([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)], -1)
+    u[0, :] = (0.0, 1.0, 0.0)
+    u[-1, :] = (0.0, -1.0, 0.0)
+    flat = u.reshape(-1, 3)
+    r = np.ones(len(flat))
+    for o in range(octaves):
+        r += amplitude * (0.5 ** o) * _value_noise(flat * (2.0 ** o) + 0.5, seed + 101 * o)
+    r = r.reshape(nv + 1, nu)
+    P = u * r[..., None]
+    # smooth normals from central differences on the grid
+    dphi = np.roll(P, -1, axis=1) - np.roll(P, 1, axis=1)
+    Pu = np.concatenate([P[:1], P, P[-1:]], 0)
+    dth = Pu[2:] - Pu[:-2]
+    n = np.cross(dphi, dth)
+    ln = np.linalg.norm(n, axis=-1, keepdims=True)
+    ok = ln[..., 0] > 1e-12
+    n = np.where(ok[..., None], n / np.where(ok[..., None], ln, 1.0), u)
+    flip = np.sum(n * u, axis=-1) < 0
+    n[flip] = -n[flip]
+    n[0, :] = (0.0, 1.0, 0.0)
+    n[-1, :] = (0.0, -1.0, 0.0)
+    i = np.arange(nv)[:, None]
+    j = np.arange(nu)[None, :]
+    jn = (j + 1) % nu
+    A = (i, j)
+    B = (i, jn)
+    C = (i + 1, j)
+    D = (i + 1, jn)
+
+    def g(arr, idx):
+        return arr[idx[0], idx[1]].reshape(-1, 3)
+    ntri = 2 * nu * nv
+    verts = np.empty((ntri, 3, 3))
+    norms = np.empty((ntri, 3, 3))
+    verts[0::2] = np.stack([g(P, A), g(P, C), g(P, D)], 1)
+    verts[1::2] = np.stack([g(P, A), g(P, D), g(P, B)], 1)
+    norms[0::2] = np.stack([g(n, A), g(n, C), g(n, D)], 1)
+    norms[1::2] = np.stack([g(n, A), g(n, D), g(n, B)], 1)
+    rt = np.empty(ntri)
+    rr = r
+    rt[0::2] = (rr[A] + rr[C] + rr[D]).ravel() / 3.0
+    rt[1::2] = (rr[A] + rr[D] + rr[B]).ravel() / 3.0
+    return verts.astype(np.float32), norms.astype(np.float32), rt
+
+
+def _band(values: np.ndarray, bands: int) -> np.ndarray:
+    lo, hi = float(values.min()), float(values.max())
+    b = np.floor((values - lo) / max(hi - lo, 1e-30) * bands)
+    return np.clip(b, 0, bands - 1).astype(np.int32)
+
+
+def _random_materials(rng: Rng, m: int, reflective: bool, refl_lo: float, refl_hi: float,
+                      kd_lo=0.1, kd_hi=0.9, s_lo=4.0, s_hi=128.0) -> np.ndarray:
+    mats = np.zeros(m, MATERIAL_DTYPE)
+    kd = kd_lo + (kd_hi - kd_lo) * rng.uniform(3 * m).reshape(m, 3)
+    ks = kd_lo + (kd_hi - kd_lo) * rng.uniform(3 * m).reshape(m, 3)
+    s = np.rint(s_lo + (s_hi - s_lo) * rng.uniform(m))
+    rf = refl_lo + (refl_hi - refl_lo) * rng.uniform(m)
+    mats["kd"][:, :3] = kd
+    mats["kd"][:, 3] = 1.0
+    mats["ks"][:, :3] = ks
+    mats["ks"][:, 3] = 1.0
+    mats["shininess"] = s
+    mats["reflectivity"] = rf if reflective else 0.0
+    mats["reflective"] = 1 if reflective else 0
+    return mats
+
+
+def _lights_on_sphere(rng: Rng, k: int, radius: float, center, min_y: float = 0.2) -> np.ndarray:
+    d = rng.normal(3 * k).reshape(k, 3)
+    d[:, 1] = np.abs(d[:, 1]) + min_y
+    d[:, 2] = np.abs(d[:, 2])                    # toward +z (camera side)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return make_lights(np.asarray(center) + radius * d)
+
+
+# ----------------------------------------------------------------- workloads
+
+def config_a(width: int = 512, height: int = 512) -> SceneData:
+    """Icosphere L4 (5,120 tris) over a 64x64-cell plane (8,192 tris); 1 light; depth 1."""
+    sv, sn = icosphere(4)
+    pv, pn = quad_grid(64, 5.0, -1.0)
+    verts = np.concatenate([sv, pv])
+    normals = np.concatenate([sn, pn])
+    mat = np.concatenate([np.zeros(len(sv), np.int32), np.ones(len(pv), np.int32)])
+    mats = np.stack([make_material((0.7, 0.7, 0.7), (0.3, 0.3, 0.3), 32, 0.2, True),
+                     make_material((0.7, 0.7, 0.7), (0.0, 0.0, 0.0), 32, 0.0, False)])
+    lights = make_lights([(2.0, 4.0, 3.0)])
+    cam = make_camera((0.0, 1.0, 4.0), (0.0, 0.0, 0.0), 60.0, width, height)
+    return SceneData(verts, normals, mat, mats, lights, cam, 1, "A: icosphere+plane")
+
+
+def config_b(width: int = 1920, height: int = 1080, n: int = 1_000_000) -> SceneData:
+    """1M-triangle soup in the unit cube, 2 lights, reflective material, depth 3."""
+    seed = BASE_SEED + 1
+    verts, normals = soup(n, seed)
+    mat = np.zeros(n, np.int32)
+    mats = np.stack([make_material((0.8, 0.6, 0.4), (0.5, 0.5, 0.5), 24, 0.3, True)])
+    lights = make_lights([(2.0, 3.0, 3.0), (-1.0, 2.0, 2.0)])
+    cam = make_camera((0.5, 0.5, 2.6), (0.5, 0.5, 0.5), 60.0, width, height)
+    return SceneData(verts, normals, mat, mats, lights, cam, 3, "B: 1M soup")
+
+
+def config_c(width: int = 3840, height: int = 2160, nu: int = 2048, nv: int = 1024) -> SceneData:
+    """Displaced UV sphere (2*2048*1024 = 4,194,304 tris), 16 banded materials, 4 lights, depth 3."""
+    seed = BASE_SEED + 2
+    verts, normals, rt = displaced_sphere(nu, nv, seed)
+    mat = _band(rt, 16)
+    rng = Rng(seed ^ 0x10)
+    mats = _random_materials(rng, 16, True, 0.1, 0.5)
+    lights = _lights_on_sphere(Rng(seed ^ 0x20), 4, 4.0, (0.0, 0.0, 0.0))
+    cam = make_camera((0.0, 0.4, 3.3), (0.0, 0.0, 0.0), 60.0, width, height)
+    return SceneData(verts, normals, mat, mats, lights, cam, 3, "C: displaced sphere 4M")
+
+
+def config_d(n_scenes: int = 64, scale: float = 1.0):
+    """64 soups, N log-uniform in [10k, 500k], resolution from {256,512,768,1024}^2,
+    1-4 lights, depth 2.  ``scale`` shrinks triangle counts and resolutions for tests."""
+    seed = BASE_SEED + 3
+    rng = Rng(seed)
+    out = []
+    for k in range(n_scenes):
+        u = rng.uniform(4)
+        n = int(round(math.exp(math.log(1e4) + u[0] * (math.log(5e5) - math.log(1e4)))))
+        res = (256, 512, 768, 1024)[min(int(u[1] * 4), 3)]
+        nl = 1 + min(int(u[2] * 4), 3)
+        n = max(16, int(n * scale))
+        res = max(16, int(res * math.sqrt(scale)) // 8 * 8)
+        sseed = seed + 1000 * (k + 1)
+        verts, normals = soup(n, sseed, sigma_median=0.004 * math.sqrt(1e6 / n))
+        mats = _random_materials(Rng(sseed ^ 0x10), 1, True, 0.2, 0.4, kd_lo=0.3, s_lo=8, s_hi=64)
+        lights = _lights_on_sphere(Rng(sseed ^ 0x20), nl, 3.0, (0.5, 0.5, 0.5))
+        d = Rng(sseed ^ 0x30).normal(3)
+        d[2] = abs(d[2]) + 0.5
+        d /= np.linalg.norm(d)
+        dist = 2.1 + 0.5 * u[3]
+        eye = np.array([0.5, 0.5, 0.5]) + dist * d
+        cam = make_camera(eye, (0.5, 0.5, 0.5), 60.0, res, res)
+        out.append(SceneData(verts, normals, np.zeros(n, np.int32), mats, lights, cam, 2,
+                             f"D[{k}]: soup {n}", {"seed": sseed}))
+    return out
+
+
+def config_e(width: int = 1024, height: int = 1024, nu: int = 1000, nv: int = 500,
+             n_materials: int = 256, n_lights: int = 8) -> SceneData:
+    """Displaced UV sphere (1,000,000 tris), 256 materials, 8 lights, depth 2."""
+    seed = BASE_SEED + 4
+    verts, normals, rt = displaced_sphere(nu, nv, seed)
+    rng = Rng(seed ^ 0x40)
+    # material slots by noise band, then shuffled so neighbouring bands differ
+    band = _band(rt, n_materials)
+    perm = np.argsort(rng.uniform(n_materials), kind="stable").astype(np.int32)
+    mat = perm[band]
+    mats = _random_materials(Rng(seed ^ 0x10), n_materials, True, 0.1, 0.4)
+    lights = _lights_on_sphere(Rng(seed ^ 0x20), n_lights, 4.0, (0.0, 0.0, 0.0))
+    lights["color"][:, :3] = 0.25 + 0.25 * Rng(seed ^ 0x21).uniform(3 * n_lights).reshape(-1, 3)
+    cam = make_camera((0.0, 0.3, 3.2), (0.0, 0.0, 0.0), 60.0, width, height)
+    return SceneData(verts, normals, mat, mats, lights, cam, 2, "E: adjoint sphere 1M")
+
+
+def perturb_params(scene: SceneData, seed: int, rel: float = 0.10):
+    """Materials and lights with every parameter scaled by a seeded factor in [1-rel, 1+rel]."""
+    rng = Rng(seed)
+    mats = scene.materials.copy()
+    lights = scene.lights.copy()
+    m, n_l = len(mats), len(lights)
+    f = 1.0 + rel * (2.0 * rng.uniform(7 * m + 6 * n_l) - 1.0)
+    fm = f[:7 * m].reshape(m, 7)
+    fl = f[7 * m:].reshape(n_l, 6)
+    mats["kd"][:, :3] *= fm[:, 0:3]
+    mats["ks"][:, :3] *= fm[:, 3:6]
+    mats["shininess"] *= fm[:, 6]
+    lights["position"][:, :3] *= fl[:, 0:3]
+    lights["color"][:, :3] *= fl[:, 3:6]
+    return mats, lights

@@ -1,0 +1,236 @@
+/*
+ * eos_c.h -- C ABI of the MI355X-native Eos render path (libeos_amd.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, status codes.  Every
+ * entry point names the reference interface it replaces (paths relative to
+ * the reference root).  The C++ API the reference exposes (Scene, Camera,
+ * DeviceRenderer, MultiDeviceRenderer, DeviceCameraHandler, createCudaScene)
+ * is kept by the shim in include/eos/ on top of these calls.
+ *
+ * Errors: every call returns an eos_status and, on failure, sets a
+ * thread-local message readable with eos_last_error().  Nothing calls exit().
+ * Threading: each scene handle owns one HIP stream; calls on the same handle
+ * must not overlap; different handles are independent.
+ * Memory: input/output pointers are host memory unless a *_memory field says
+ * EOS_MEM_DEVICE (then they are device pointers on the current HIP device).
+ */
+#ifndef EOS_C_H
+#define EOS_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EOS_ABI_VERSION 1
+
+typedef enum eos_status {
+    EOS_OK = 0,
+    EOS_ERR_INVALID = 1,   /* bad argument (null pointer, size, index out of range) */
+    EOS_ERR_HIP = 2,       /* a HIP runtime call failed */
+    EOS_ERR_NOMEM = 3,     /* device allocation failed */
+    EOS_ERR_STATE = 4,     /* call order (e.g. render before eos_build_lbvh) */
+    EOS_ERR_LIMIT = 5,     /* a fixed capacity was exceeded */
+    EOS_ERR_NODEVICE = 6   /* no HIP device visible */
+} eos_status;
+
+enum { EOS_MEM_HOST = 0, EOS_MEM_DEVICE = 1 };
+
+/* Camera: same fields and layout as the reference Camera (src/Camera.h:55-63):
+ * position, viewing direction, right and up vectors (their lengths set the
+ * field of view), width and height in pixels.  Row 0 of the image is the
+ * bottom row (src/RayTracer.cpp:135). */
+typedef struct eos_camera {
+    float position[3];
+    float view[3];
+    float right[3];
+    float up[3];
+    int32_t width;
+    int32_t height;
+} eos_camera;
+
+/* Material (src/Material.h:68-77): diffuse and specular colours (RGBA, A
+ * unused), Phong exponent (the reference stores an int; integer values give
+ * the reference's result, other values are used as is), reflection weight and
+ * flag.  Transparency is not part of this path. */
+typedef struct eos_material {
+    float kd[4];
+    float ks[4];
+    float shininess;
+    float reflectivity;
+    int32_t reflective;
+    int32_t pad;
+} eos_material;
+
+/* Point light (src/LightSource.h:35-36): position (w must be 1) and colour. */
+typedef struct eos_light {
+    float position[4];
+    float color[4];
+} eos_light;
+
+/* Scene description (replaces the host Scene handed to createCudaScene,
+ * src/cudaStructures.h:98).  Triangles with identity transforms. */
+typedef struct eos_scene_desc {
+    int32_t num_triangles;
+    const float* vertices;          /* [N][3][3]: v1, v2, v3 */
+    const float* normals;           /* [N][3][3]: n1, n2, n3 */
+    const int32_t* material_index;  /* [N] */
+    int32_t triangle_memory;        /* EOS_MEM_HOST / EOS_MEM_DEVICE for the three arrays */
+    int32_t num_materials;
+    const eos_material* materials;  /* host */
+    int32_t num_lights;             /* at most EOS_MAX_LIGHTS */
+    const eos_light* lights;        /* host */
+} eos_scene_desc;
+
+#define EOS_MAX_LIGHTS 32
+#define EOS_MAX_DEPTH 8
+
+typedef struct eos_scene eos_scene;
+
+typedef struct eos_build_stats {
+    double build_ms;          /* GPU time of the LBVH build (Morton, sort, Karras, refit, layout) */
+    int32_t num_internal;     /* internal nodes (N - 1) */
+    int32_t root_ref;         /* root child reference */
+    int32_t leaf_max;         /* triangles per leaf after collapsing */
+    int32_t pad;
+} eos_build_stats;
+
+/* Render options.  Output pointers are optional (NULL = not wanted). */
+typedef struct eos_render_opts {
+    int32_t depth;            /* reflection bounces (RayTracer::setTracedDepth, src/RayTracer.cpp:47-50) */
+    int32_t output_memory;    /* EOS_MEM_HOST / EOS_MEM_DEVICE for every output below */
+    float* rgb_out;           /* [H][W][3] colour before quantisation */
+    int32_t* hit_out;         /* [H][W] primary hit: original triangle index or -1 */
+    int32_t keep_records;     /* keep hit records for eos_render_adjoint */
+    int32_t count_visits;     /* run the counting traversal (node visits, triangle tests) */
+} eos_render_opts;
+
+typedef struct eos_render_stats {
+    int64_t primary_rays;
+    int64_t shadow_rays;
+    int64_t secondary_rays;
+    int64_t hits;
+    int64_t node_visits;      /* count_visits only */
+    int64_t tri_tests;        /* count_visits only */
+    double render_ms;         /* GPU time, first to last kernel of the frame */
+    double trace_closest_ms;  /* summed over the frame's closest-hit launches */
+    double trace_shadow_ms;   /* summed over the frame's any-hit launches */
+    double shade_ms;          /* hit setup + Phong + resolve */
+    int32_t trace_closest_launches;
+    int32_t trace_shadow_launches;
+    int64_t closest_visits_first;  /* count_visits: node visits of the primary-ray launch */
+    int64_t closest_tests_first;   /* count_visits: triangle tests of the primary-ray launch */
+    double trace_first_ms;         /* duration of the primary-ray closest-hit launch */
+} eos_render_stats;
+
+/* ---- scene lifetime ----------------------------------------------------- */
+
+/* Replaces createCudaScene (src/cudaStructures.h:98, .cu:483-543): copies the
+ * scene to the current device in the build's own layout.  No BVH yet. */
+eos_status eos_create_scene(const eos_scene_desc* desc, eos_scene** out);
+
+/* The reference never frees its device scene; this does. */
+eos_status eos_destroy_scene(eos_scene* scene);
+
+/* Replaces Scene::flush's BVH build (src/Scene.cpp:204-209, src/BVH.cpp:97-111)
+ * with a GPU Morton-code LBVH: radix sort + Karras hierarchy + bottom-up refit.
+ * leaf_max (1..8) collapses subtrees of at most that many triangles into one
+ * leaf.  Synchronous. */
+eos_status eos_build_lbvh(eos_scene* scene, int32_t leaf_max, eos_build_stats* stats);
+
+/* Parameter updates without a rebuild (materials / lights only). */
+eos_status eos_update_materials(eos_scene* scene, const eos_material* materials, int32_t n);
+eos_status eos_update_lights(eos_scene* scene, const eos_light* lights, int32_t n);
+
+/* ---- rendering ---------------------------------------------------------- */
+
+/* Replaces the extern "C" launcher rayTrace_MegaKernel (src/kernelWrapper.h:31)
+ * and DeviceRenderer::renderCudaSceneToHostBufferMegaKernel
+ * (src/DeviceRenderer.cu:61-109), with the CPU RayTracer's semantics
+ * (src/RayTracer.cpp:99-174,268-458): primary rays, nearest hit, one shadow ray
+ * per light, Phong, reflection up to opts->depth bounces, 8-bit quantisation.
+ * rgba_out: [H][W][4] (may be NULL when only rgb_out/hit_out are wanted).
+ * Synchronous: returns after the outputs are written. */
+eos_status eos_render(eos_scene* scene, const eos_camera* camera, const eos_render_opts* opts,
+                      uint8_t* rgba_out, eos_render_stats* stats);
+
+/* Adjoint of eos_render's pre-quantisation image with hit sets held fixed
+ * (no counterpart in the reference).  dl_drgb: [H][W][3] upstream gradient.
+ * Gradients are ACCUMULATED into grad_materials [M][7] (kd rgb, ks rgb,
+ * shininess) and grad_lights [L][6] (position xyz, colour rgb), float64.
+ * Replays the records of the last eos_render with keep_records on this scene
+ * and camera, or re-traces them when recompute != 0 or none are kept. */
+typedef struct eos_adjoint_opts {
+    int32_t depth;
+    int32_t memory;           /* EOS_MEM_HOST / EOS_MEM_DEVICE for dl_drgb and gradients */
+    int32_t recompute;        /* 1: re-trace instead of replaying stored records */
+    int32_t pad;
+} eos_adjoint_opts;
+
+typedef struct eos_adjoint_stats {
+    double adjoint_ms;        /* GPU time of the adjoint kernels */
+    double retrace_ms;        /* GPU time of the re-trace (recompute mode) */
+    int64_t records;          /* hit records replayed */
+} eos_adjoint_stats;
+
+eos_status eos_render_adjoint(eos_scene* scene, const eos_camera* camera, const eos_adjoint_opts* opts,
+                              const float* dl_drgb, double* grad_materials, double* grad_lights,
+                              eos_adjoint_stats* stats);
+
+/* Many scenes, one persistent launch per pipeline stage (no counterpart in the
+ * reference; MultiDeviceRenderer's per-GPU loop, src/MultiDeviceRenderer.cu:
+ * 112-165, is the nearest relative).  Every scene must have an LBVH.
+ * rgba_out[i]: [H_i][W_i][4] for scene i, memory per opts->output_memory. */
+typedef struct eos_batch_opts {
+    int32_t depth;
+    int32_t output_memory;
+    int32_t use_graph;        /* capture the frame in a hipGraph and replay it */
+    int32_t pad;
+} eos_batch_opts;
+
+typedef struct eos_batch_stats {
+    double render_ms;
+    int64_t rays;             /* primary + shadow + secondary over all scenes */
+} eos_batch_stats;
+
+eos_status eos_render_batch(eos_scene* const* scenes, const eos_camera* cameras, int32_t n,
+                            const eos_batch_opts* opts, uint8_t* const* rgba_out, eos_batch_stats* stats);
+
+/* ---- tuning, diagnostics ------------------------------------------------ */
+
+/* Traversal launch configuration (the autotuner's knobs). */
+typedef struct eos_launch_config {
+    int32_t workgroup;        /* 64, 128 or 256 threads */
+    int32_t stack_lds;        /* per-lane LDS stack entries: 8, 16 or 32 */
+    int32_t waves_per_cu;     /* persistent waves per CU launched */
+    int32_t refill;           /* idle lanes that trigger a refill, 1..64 */
+} eos_launch_config;
+
+eos_status eos_set_launch_config(eos_scene* scene, const eos_launch_config* cfg);
+eos_status eos_get_launch_config(const eos_scene* scene, eos_launch_config* cfg);
+
+/* Copies internal arrays to host memory for parity checks. */
+enum {
+    EOS_DBG_MORTON_SORTED = 1,   /* uint32 [N] */
+    EOS_DBG_PERM = 2,            /* int32  [N] sorted slot -> original index */
+    EOS_DBG_KCHILD = 3,          /* int32  [N-1][2] Karras children (<0: ~leaf) */
+    EOS_DBG_KPARENT = 4,         /* int32  [2N-1] */
+    EOS_DBG_IBOX = 5,            /* float  [N-1][6] exact internal boxes */
+    EOS_DBG_NODES = 6,           /* float  [N-1][16] 64-byte traversal nodes */
+    EOS_DBG_TRIS = 7,            /* float  [N][12] sorted triangle records */
+    EOS_DBG_MORTON = 8           /* uint32 [N] unsorted codes */
+};
+eos_status eos_debug_read(const eos_scene* scene, int32_t what, void* host_dst, int64_t bytes);
+
+/* Thread-local text of the last failure ("" if none). */
+const char* eos_last_error(void);
+
+/* ABI version (EOS_ABI_VERSION) and device name of the current device. */
+int32_t eos_abi_version(void);
+eos_status eos_device_name(char* buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,337 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+REC_DTYPE = np.dtype([("tri", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4"),
+                      ("p", "<f4", (3,)), ("n", "<f4", (3,)), ("mat", "<i4"),
+                      ("vis", "<u4"), ("refl", "<f4"), ("pad", "<f4", (3,))])
+assert REC_DTYPE.itemsize == 64
+
+
+class OrcScene(C.Structure):
+    _fields_ = [("n_tri", C.c_int32), ("verts", C.c_void_p), ("normals", C.c_void_p),
+                ("mat", C.c_void_p), ("n_mat", C.c_int32), ("mats", C.c_void_p),
+                ("n_light", C.c_int32), ("lights", C.c_void_p)]
+
+
+class OrcCounts(C.Structure):
+    _fields_ = [("primary", C.c_int64), ("shadow", C.c_int64), ("secondary", C.c_int64),
+                ("hits", C.c_int64), ("node_visits", C.c_int64), ("tri_tests", C.c_int64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+def build() -> None:
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        L = _LIB
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        L.orc_accel_build.restype = vp
+        L.orc_accel_build.argtypes = [vp, C.c_int]
+        L.orc_accel_free.argtypes = [vp]
+        L.orc_nearest.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp]
+        L.orc_render.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.orc_render_lbvh.argtypes = [vp, vp, i32, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
+        L.orc_mt.argtypes = [vp, vp, vp, vp]
+        L.orc_camera_ray.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.orc_quantize.restype = C.c_uint8
+        L.orc_quantize.argtypes = [C.c_float]
+        L.orc_morton3.restype = C.c_uint32
+        L.orc_morton3.argtypes = [C.c_uint32] * 3
+        L.orc_shade_phong.restype = C.c_float
+        L.orc_shade_phong.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
+        L.orc_morton_codes.argtypes = [i32, vp, vp, vp]
+        L.orc_lbvh_build.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_karras.argtypes = [i32, vp, vp, vp, vp, vp]
+        L.orc_lbvh_trace.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp]
+        L.orc_loss64.restype = C.c_double
+        L.orc_loss64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, vp]
+        L.orc_grad_fd64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, C.c_double, vp, i32, vp]
+        L.orc_grad_fwd64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, vp]
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Oracle:
+    """Holds one scene (arrays kept alive) and its exact nearest-hit accelerator."""
+
+    def __init__(self, scene, brute_force: bool = False):
+        self.scene = scene
+        self.verts = np.ascontiguousarray(scene.verts, np.float32)
+        self.normals = np.ascontiguousarray(scene.normals, np.float32)
+        self.mat = np.ascontiguousarray(scene.mat, np.int32)
+        self.materials = np.ascontiguousarray(scene.materials)
+        self.lights = np.ascontiguousarray(scene.lights)
+        self._s = OrcScene(self.verts.shape[0], _p(self.verts), _p(self.normals), _p(self.mat),
+                           len(self.materials), _p(self.materials), len(self.lights), _p(self.lights))
+        self._accel = lib().orc_accel_build(C.byref(self._s), 1 if brute_force else 0)
+
+    def set_params(self, materials=None, lights=None):
+        if materials is not None:
+            self.materials = np.ascontiguousarray(materials)
+            self._s.mats = _p(self.materials)
+            self._s.n_mat = len(self.materials)
+        if lights is not None:
+            self.lights = np.ascontiguousarray(lights)
+            self._s.lights = _p(self.lights)
+            self._s.n_light = len(self.lights)
+
+    def __del__(self):
+        try:
+            if self._accel:
+                lib().orc_accel_free(self._accel)
+        except Exception:
+            pass
+
+    def nearest(self, origins, dirs, any_hit=False):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[0]
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        P = np.zeros((n, 3), np.float32)
+        N = np.zeros((n, 3), np.float32)
+        lib().orc_nearest(C.byref(self._s), self._accel, n, _p(o), _p(d), 1 if any_hit else 0, _p(tri),
+                          _p(t), _p(u), _p(v), _p(P), _p(N))
+        if any_hit:
+            return tri.astype(bool)
+        return {"tri": tri, "t": t, "u": u, "v": v, "point": P, "normal": N}
+
+    def render(self, camera=None, depth=None, row_start=0, row_step=1, want_recs=False):
+        cam = np.array(self.scene.camera if camera is None else camera, copy=True)
+        depth = self.scene.depth if depth is None else depth
+        W, H = int(cam["width"]), int(cam["height"])
+        rgba = np.zeros((H, W, 4), np.uint8)
+        rgb = np.zeros((H, W, 3), np.float32)
+        hits = np.full((H, W), -1, np.int32)
+        recs = np.zeros((depth + 1, H, W), REC_DTYPE) if want_recs else None
+        cnt = OrcCounts()
+        lib().orc_render(C.byref(self._s), self._accel, cam.ctypes.data, depth, row_start, row_step,
+                         _p(rgba), _p(rgb), _p(hits), _p(recs), C.byref(cnt))
+        out = {"rgba": rgba, "rgb": rgb, "hit": hits, "counts": cnt.as_dict()}
+        if want_recs:
+            out["recs"] = recs
+        return out
+
+
+def morton_codes(verts):
+    v = np.ascontiguousarray(verts, np.float32)
+    n = v.shape[0]
+    codes = np.zeros(n, np.uint32)
+    cb = np.zeros(6, np.float32)
+    lib().orc_morton_codes(n, _p(v), _p(codes), _p(cb))
+    return codes, cb
+
+
+def morton3(x, y, z) -> int:
+    return int(lib().orc_morton3(x, y, z))
+
+
+def karras(sorted_codes):
+    k = np.ascontiguousarray(sorted_codes, np.uint32)
+    n = len(k)
+    kc = np.zeros(max(2 * (n - 1), 1), np.int32)
+    kp = np.zeros(max(2 * n - 1, 1), np.int32)
+    fi = np.zeros(max(n - 1, 1), np.int32)
+    la = np.zeros(max(n - 1, 1), np.int32)
+    lib().orc_karras(n, _p(k), _p(kc), _p(kp), _p(fi), _p(la))
+    return kc[:2 * (n - 1)].reshape(-1, 2), kp[:2 * n - 1], fi[:n - 1], la[:n - 1]
+
+
+def lbvh_build(verts, leaf_max=1):
+    v = np.ascontiguousarray(verts, np.float32)
+    n = v.shape[0]
+    m = max(n - 1, 1)
+    out = {
+        "codes": np.zeros(n, np.uint32), "perm": np.zeros(n, np.int32),
+        "kchild": np.zeros(2 * m, np.int32), "kparent": np.zeros(2 * n - 1 if n else 1, np.int32),
+        "ibox": np.zeros(6 * m, np.float32), "nodes": np.zeros(16 * m, np.float32),
+        "tris": np.zeros(12 * max(n, 1), np.float32), "root": np.zeros(1, np.int32)}
+    lib().orc_lbvh_build(n, _p(v), leaf_max, _p(out["codes"]), _p(out["perm"]), _p(out["kchild"]),
+                         _p(out["kparent"]), _p(out["ibox"]), _p(out["nodes"]), _p(out["tris"]),
+                         _p(out["root"]))
+    out["kchild"] = out["kchild"][:2 * (n - 1)].reshape(-1, 2)
+    out["ibox"] = out["ibox"][:6 * (n - 1)].reshape(-1, 6)
+    out["nodes"] = out["nodes"][:16 * (n - 1)].reshape(-1, 16)
+    out["tris"] = out["tris"][:12 * n].reshape(-1, 12)
+    out["root"] = int(out["root"][0])
+    return out
+
+
+def lbvh_trace(b, origins, dirs, any_hit=False):
+    o = np.ascontiguousarray(origins, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    n = o.shape[0]
+    hit = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    u = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    vis = np.zeros(1, np.int64)
+    tst = np.zeros(1, np.int64)
+    nodes = np.ascontiguousarray(b["nodes"], np.float32)
+    tris = np.ascontiguousarray(b["tris"], np.float32)
+    lib().orc_lbvh_trace(_p(nodes), b["root"], _p(tris), n, _p(o), _p(d), 1 if any_hit else 0,
+                         _p(hit), _p(t), _p(u), _p(v), _p(vis), _p(tst))
+    return {"hit": hit, "t": t, "u": u, "v": v, "visits": int(vis[0]), "tests": int(tst[0])}
+
+
+def render_lbvh(scene, b, camera=None, depth=None, row_start=0, row_step=1):
+    o = Oracle.__new__(Oracle)
+    verts = np.ascontiguousarray(scene.verts, np.float32)
+    normals = np.ascontiguousarray(scene.normals, np.float32)
+    mat = np.ascontiguousarray(scene.mat, np.int32)
+    mats = np.ascontiguousarray(scene.materials)
+    lights = np.ascontiguousarray(scene.lights)
+    s = OrcScene(verts.shape[0], _p(verts), _p(normals), _p(mat), len(mats), _p(mats),
+                 len(lights), _p(lights))
+    cam = np.array(scene.camera if camera is None else camera, copy=True)
+    depth = scene.depth if depth is None else depth
+    W, H = int(cam["width"]), int(cam["height"])
+    rgba = np.zeros((H, W, 4), np.uint8)
+    rgb = np.zeros((H, W, 3), np.float32)
+    hits = np.full((H, W), -1, np.int32)
+    cnt = OrcCounts()
+    nodes = np.ascontiguousarray(b["nodes"], np.float32)
+    tris = np.ascontiguousarray(b["tris"], np.float32)
+    lib().orc_render_lbvh(C.byref(s), _p(nodes), b["root"], _p(tris), cam.ctypes.data, depth,
+                          row_start, row_step, _p(rgba), _p(rgb), _p(hits), C.byref(cnt))
+    del o
+    return {"rgba": rgba, "rgb": rgb, "hit": hits, "counts": cnt.as_dict()}
+
+
+def params_vector(materials, lights) -> np.ndarray:
+    """Flatten materials/lights into the gradient parameter layout (float64)."""
+    m = len(materials)
+    nl = len(lights)
+    p = np.zeros(7 * m + 6 * nl)
+    pm = p[:7 * m].reshape(m, 7)
+    pm[:, 0:3] = materials["kd"][:, :3]
+    pm[:, 3:6] = materials["ks"][:, :3]
+    pm[:, 6] = materials["shininess"]
+    pl = p[7 * m:].reshape(nl, 6)
+    pl[:, 0:3] = lights["position"][:, :3]
+    pl[:, 3:6] = lights["color"][:, :3]
+    return p
+
+
+def loss64(recs, cam_pos, n_mat, n_light, params, target_rgb):
+    r = np.ascontiguousarray(recs)
+    lev = r.shape[0]
+    npix = r[0].size
+    cp = np.ascontiguousarray(cam_pos, np.float32)
+    pr = np.ascontiguousarray(params, np.float64)
+    tg = np.ascontiguousarray(target_rgb, np.float32)
+    img = np.zeros((npix, 3))
+    loss = lib().orc_loss64(_p(r), lev, npix, _p(cp), n_mat, n_light, _p(pr), _p(tg), _p(img))
+    return loss, img
+
+
+def grad_fd64(recs, cam_pos, n_mat, n_light, params, target_rgb, which, h=1e-3):
+    r = np.ascontiguousarray(recs)
+    w = np.ascontiguousarray(which, np.int32)
+    out = np.zeros(len(w))
+    cp = np.ascontiguousarray(cam_pos, np.float32)
+    pr = np.ascontiguousarray(params, np.float64)
+    tg = np.ascontiguousarray(target_rgb, np.float32)
+    lib().orc_grad_fd64(_p(r), r.shape[0], r[0].size, _p(cp), n_mat, n_light, _p(pr), _p(tg), h,
+                        _p(w), len(w), _p(out))
+    return out
+
+
+def grad_fwd64(recs, cam_pos, n_mat, n_light, params, target_rgb):
+    r = np.ascontiguousarray(recs)
+    cp = np.ascontiguousarray(cam_pos, np.float32)
+    pr = np.ascontiguousarray(params, np.float64)
+    tg = np.ascontiguousarray(target_rgb, np.float32)
+    out = np.zeros(len(pr))
+    lib().orc_grad_fwd64(_p(r), r.shape[0], r[0].size, _p(cp), n_mat, n_light, _p(pr), _p(tg), _p(out))
+    return out
+
+
+# --------------------------------------------------------------------- refport
+class RpCounts(C.Structure):
+    _fields_ = [("primary", C.c_int64), ("shadow", C.c_int64), ("secondary", C.c_int64),
+                ("hits", C.c_int64), ("seconds", C.c_double), ("threads", C.c_int32), ("pad", C.c_int32)]
+
+
+_RP = {}
+
+
+def refport_lib(parity: bool = False):
+    name = "librefport_parity.so" if parity else "librefport.so"
+    if name not in _RP:
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", _HERE, name])
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.rp_build.restype = vp
+        L.rp_build.argtypes = [vp, vp]
+        L.rp_free.argtypes = [vp]
+        L.rp_num_nodes.argtypes = [vp]
+        L.rp_num_nodes.restype = C.c_int32
+        L.rp_render.argtypes = [vp, vp, C.c_int, vp, C.c_int64, C.c_int32, vp, C.c_int32, vp]
+        _RP[name] = L
+    return _RP[name]
+
+
+class RefPort:
+    """Timing-faithful port of the reference CPU path (oracle/refport.h)."""
+
+    def __init__(self, scene, parity: bool = False):
+        self.L = refport_lib(parity)
+        self.scene = scene
+        self.verts = np.ascontiguousarray(scene.verts, np.float32)
+        self.normals = np.ascontiguousarray(scene.normals, np.float32)
+        self.mat = np.ascontiguousarray(scene.mat, np.int32)
+        self.materials = np.ascontiguousarray(scene.materials)
+        self.lights = np.ascontiguousarray(scene.lights)
+        self._s = OrcScene(self.verts.shape[0], _p(self.verts), _p(self.normals), _p(self.mat),
+                           len(self.materials), _p(self.materials), len(self.lights), _p(self.lights))
+        bs = np.zeros(1, np.float64)
+        self.h = self.L.rp_build(C.byref(self._s), bs.ctypes.data)
+        self.build_seconds = float(bs[0])
+
+    def __del__(self):
+        try:
+            self.L.rp_free(self.h)
+        except Exception:
+            pass
+
+    def render(self, pixels=None, threads=0, camera=None, depth=None):
+        cam = np.array(self.scene.camera if camera is None else camera, copy=True)
+        depth = self.scene.depth if depth is None else depth
+        W, H = int(cam["width"]), int(cam["height"])
+        px = np.arange(W * H, dtype=np.int32) if pixels is None else np.ascontiguousarray(pixels, np.int32)
+        rgba = np.zeros((len(px), 4), np.uint8)
+        cnt = RpCounts()
+        self.L.rp_render(self.h, cam.ctypes.data, depth, px.ctypes.data, len(px), W, rgba.ctypes.data,
+                         threads, C.byref(cnt))
+        counts = {k: getattr(cnt, k) for k, _ in cnt._fields_}
+        return rgba, counts

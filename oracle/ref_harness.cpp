@@ -1,0 +1,141 @@
+// ref_harness.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// Drives the reference's own CPU renderer through its public C++ API
+// (Scene::addSurface/addLightSource/addMaterial/flush, Camera setters,
+// RayTracer::setTracedDepth/render; src/Scene.h:46-86, src/Camera.h:40-45,
+// src/RayTracer.h:43-51) so the oracle's restatement can be checked against
+// the real thing.  Compiled together with the reference's translation units,
+// read in place from the reference tree by oracle/ref.mk; the result lives in
+// oracle/_ref/ and is never part of the product.
+#include <omp.h>
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "Camera.h"
+#include "LightSource.h"
+#include "Material.h"
+#include "RayIntersection.h"
+#include "RayTracer.h"
+#include "Scene.h"
+#include "Triangle.h"
+
+#define REF_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+struct RefScene {
+    Scene scene;
+    std::vector<Material> mats;       // stable storage Triangle::setMaterial points into
+    glm::mat4 ident{1.0f}, inv{1.0f}, invT{1.0f};
+    std::vector<Triangle*> tris;
+    std::vector<LightSource*> lights;
+};
+}  // namespace
+
+// Build a reference Scene.  Materials: kd[4] ks[4] shininess reflectivity
+// reflective pad (48 bytes each); lights: position[4] color[4].
+REF_API void* ref_scene_create(int32_t n_tri, const float* verts, const float* normals,
+                               const int32_t* mat, int32_t n_mat, const void* mats_raw,
+                               int32_t n_light, const void* lights_raw, int32_t build_threads) {
+    RefScene* R = new RefScene();
+    R->scene.setMaxTracedDepth(5);
+    R->scene.setAmbientRefractiveIndex(1.0f);
+    R->scene.useBvh(true);
+    R->scene.setAASamples(1);
+    const char* mb = static_cast<const char*>(mats_raw);
+    for (int i = 0; i < n_mat; ++i) {
+        const float* f = reinterpret_cast<const float*>(mb + 48 * i);
+        int32_t refl;
+        std::memcpy(&refl, mb + 48 * i + 40, 4);
+        Material m(0.0f, glm::vec4(f[0], f[1], f[2], f[3]), glm::vec4(f[4], f[5], f[6], f[7]), (int)f[8]);
+        m.setReflective(refl != 0);
+        m.setReflectionIntensity(f[9]);
+        m.setTransparent(false);
+        m.setRefractiveIndex(1.0f);
+        R->mats.push_back(m);
+        R->scene.addMaterial(m);
+    }
+    for (int i = 0; i < n_tri; ++i) {
+        const float* v = verts + 9 * (size_t)i;
+        const float* n = normals + 9 * (size_t)i;
+        Triangle* t = new Triangle(glm::vec3(v[0], v[1], v[2]), glm::vec3(v[3], v[4], v[5]),
+                                   glm::vec3(v[6], v[7], v[8]), glm::vec3(n[0], n[1], n[2]),
+                                   glm::vec3(n[3], n[4], n[5]), glm::vec3(n[6], n[7], n[8]));
+        t->setTransformation(R->ident);
+        t->setInverseTransformation(R->inv);
+        t->setInverseTransposeTransformation(R->invT);
+        t->setMaterial(R->mats[mat[i]]);
+        t->setMaterialIndex(mat[i]);
+        t->setTransformationIndex(0);
+        R->tris.push_back(t);
+        R->scene.addSurface(t);
+    }
+    const char* lb = static_cast<const char*>(lights_raw);
+    for (int i = 0; i < n_light; ++i) {
+        const float* f = reinterpret_cast<const float*>(lb + 32 * i);
+        LightSource* l = new LightSource(glm::vec4(f[0], f[1], f[2], f[3]), glm::vec4(f[4], f[5], f[6], f[7]));
+        R->lights.push_back(l);
+        R->scene.addLightSource(l);
+    }
+    int saved = omp_get_max_threads();
+    if (build_threads > 0) omp_set_num_threads(build_threads);  // the SAH sweep races (no determinism otherwise)
+    R->scene.flush();
+    omp_set_num_threads(saved);
+    return R;
+}
+
+// RayTracer::render into rgba[H*W*4].  Camera: position, view, right, up, W, H.
+REF_API void ref_render(void* h, const float* cam, int32_t width, int32_t height, int32_t depth,
+                        uint8_t* rgba) {
+    RefScene* R = static_cast<RefScene*>(h);
+    Camera c;
+    c.setPosition(glm::vec3(cam[0], cam[1], cam[2]));
+    c.setViewingDirection(glm::vec3(cam[3], cam[4], cam[5]));
+    c.setRightVector(glm::vec3(cam[6], cam[7], cam[8]));
+    c.setUpVector(glm::vec3(cam[9], cam[10], cam[11]));
+    c.setWidth(width);
+    c.setHeight(height);
+    RayTracer rt;
+    rt.setAASamples(RAYTRACER_NO_AA);
+    rt.setTracedDepth(depth);
+    rt.render(R->scene, c, rgba);
+}
+
+// Scene::findMinDistanceIntersectionBVH for n rays: hit flag, point, normal, material.
+REF_API void ref_nearest(void* h, int32_t n, const float* orig, const float* dirs, int32_t* hit,
+                         float* point, float* normal, int32_t* mat) {
+    RefScene* R = static_cast<RefScene*>(h);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        Ray r;
+        r.setOrigin(glm::vec3(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]));
+        r.setDirection(glm::vec3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        RayIntersection x;
+        bool got = R->scene.findMinDistanceIntersectionBVH(r, x);
+        hit[i] = got ? 1 : 0;
+        if (got) {
+            for (int k = 0; k < 3; ++k) { point[3 * i + k] = x.getPoint()[k]; normal[3 * i + k] = x.getNormal()[k]; }
+            mat[i] = x.getMaterialIndex();
+        }
+    }
+}
+
+// Scene::shadowRayVisibilityBVH for n rays.
+REF_API void ref_occluded(void* h, int32_t n, const float* orig, const float* dirs, int32_t* occ) {
+    RefScene* R = static_cast<RefScene*>(h);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        Ray r;
+        r.setOrigin(glm::vec3(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]));
+        r.setDirection(glm::vec3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        occ[i] = R->scene.shadowRayVisibilityBVH(r) ? 1 : 0;
+    }
+}
+
+REF_API void ref_scene_free(void* h) {
+    RefScene* R = static_cast<RefScene*>(h);
+    for (Triangle* t : R->tris) delete t;
+    for (LightSource* l : R->lights) delete l;
+    delete R;
+}

@@ -1,0 +1,400 @@
+"""Pins the CPU oracle with closed forms, textbook cases and brute force.
+
+Each check is chosen so a plausible mistake (dropped term, wrong sign or
+index, swapped operand, off-by-one in a tie rule) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from eos_amd import scenes
+from oracle import pyoracle as po
+
+
+# ------------------------------------------------------------------ Morton
+def _interleave(x, y, z):
+    r = 0
+    for b in range(10):
+        r |= ((x >> b) & 1) << (3 * b) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b + 2)
+    return r
+
+
+def test_morton_known_answers():
+    assert po.morton3(1, 0, 0) == 1
+    assert po.morton3(0, 1, 0) == 2
+    assert po.morton3(0, 0, 1) == 4
+    assert po.morton3(1023, 1023, 1023) == 0x3FFFFFFF
+    assert po.morton3(1024, 0, 0) == 0           # 10-bit mask (src/morton.h:33)
+
+
+def test_morton_exhaustive_per_axis():
+    for v in range(1024):
+        assert po.morton3(v, 0, 0) == _interleave(v, 0, 0)
+        assert po.morton3(0, v, 0) == _interleave(0, v, 0)
+        assert po.morton3(0, 0, v) == _interleave(0, 0, v)
+    rng = np.random.default_rng(1)
+    for x, y, z in rng.integers(0, 1024, size=(2000, 3)):
+        assert po.morton3(int(x), int(y), int(z)) == _interleave(int(x), int(y), int(z))
+
+
+def test_morton_codes_quantisation():
+    # centroids on a regular 4x4x4 grid in [0,3]^3 map to q = c * 1024/3 clamped to 1023
+    g = np.arange(4, dtype=np.float32)
+    cx, cy, cz = np.meshgrid(g, g, g, indexing="ij")
+    c = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], 1)
+    tri = np.repeat(c[:, None, :], 3, axis=1).astype(np.float32)  # degenerate: centroid = point
+    codes, cb = po.morton_codes(tri)
+    assert np.array_equal(cb, [0, 0, 0, 3, 3, 3])
+    q = np.minimum((c * np.float32(1024.0 / 3.0)).astype(np.int64), 1023)
+    q[c == 3] = 1023
+    expect = [_interleave(*map(int, r)) for r in q]
+    assert np.array_equal(codes, expect)
+
+
+# ------------------------------------------------------------------ Karras
+def test_karras_paper_example():
+    """Karras 2012, Figure 3: keys 00001 00010 00100 00101 10011 11000 11001 11110."""
+    keys = np.array([0b00001, 0b00010, 0b00100, 0b00101, 0b10011, 0b11000, 0b11001, 0b11110], np.uint32)
+    kc, kp, fi, la = po.karras(keys)
+    L = lambda i: ~i  # noqa: E731
+    expect = [(3, 4), (L(0), L(1)), (L(2), L(3)), (1, 2), (L(4), 5), (6, L(7)), (L(5), L(6))]
+    assert [tuple(r) for r in kc] == expect
+    assert list(zip(fi, la)) == [(0, 7), (0, 1), (2, 3), (0, 3), (4, 7), (5, 7), (5, 6)]
+    assert kp[0] == -1
+
+
+def _topdown(keys):
+    """Independent radix-tree build: split [lo,hi] after the last key sharing a
+    longer (key, index) prefix with keys[lo] than keys[hi] does; node = split index."""
+    n = len(keys)
+    k64 = [(int(k) << 32) | i for i, k in enumerate(keys)]
+
+    def cpl(a, b):
+        x = k64[a] ^ k64[b]
+        return 64 - x.bit_length()
+    children = {}
+    stack = [(0, n - 1, 0)]
+    while stack:
+        lo, hi, node = stack.pop()
+        d = cpl(lo, hi)
+        g = max(j for j in range(lo, hi) if cpl(lo, j) > d)
+        left = ~g if g == lo else g
+        right = ~(g + 1) if g + 1 == hi else g + 1
+        children[node] = (left, right)
+        if left >= 0:
+            stack.append((lo, g, g))
+        if right >= 0:
+            stack.append((g + 1, hi, g + 1))
+    return children
+
+
+@pytest.mark.parametrize("seed,n,bits", [(1, 50, 30), (2, 200, 6), (3, 333, 3), (4, 2, 30), (5, 64, 1)])
+def test_karras_matches_topdown(seed, n, bits):
+    rng = np.random.default_rng(seed)
+    keys = np.sort(rng.integers(0, 1 << bits, n).astype(np.uint32))   # many duplicates when bits small
+    kc, kp, fi, la = po.karras(keys)
+    td = _topdown(keys)
+    assert len(td) == n - 1
+    for i in range(n - 1):
+        assert tuple(kc[i]) == td[i]
+    # parents consistent, each leaf exactly once
+    leaves = sorted(~c for r in kc for c in r if c < 0)
+    assert leaves == list(range(n))
+
+
+def test_lbvh_ranges_and_boxes():
+    s = scenes.config_b(8, 8, n=4000)
+    b = po.lbvh_build(s.verts, 1)
+    n = s.n_tri
+    v = s.verts[b["perm"]]
+    lo = v.min(1)
+    hi = v.max(1)
+    kc = b["kchild"]
+    # exact internal boxes = min/max over their leaf range
+    _, _, fi, la = po.karras(b["codes"])
+    for i in range(0, n - 1, 97):
+        assert np.array_equal(b["ibox"][i, :3], lo[fi[i]:la[i] + 1].min(0))
+        assert np.array_equal(b["ibox"][i, 3:], hi[fi[i]:la[i] + 1].max(0))
+    # child boxes in the 64-byte node contain the exact box (inflated outward)
+    nd = b["nodes"]
+    for i in range(0, n - 1, 97):
+        for c, (xs, ys, zs) in enumerate([((0, 1), (2, 3), (8, 9)), ((4, 5), (6, 7), (10, 11))]):
+            ch = kc[i, c]
+            box = b["ibox"][ch] if ch >= 0 else np.concatenate([lo[~ch], hi[~ch]])
+            assert nd[i, xs[0]] < box[0] <= box[3] < nd[i, xs[1]]
+            assert nd[i, ys[0]] < box[1] <= box[4] < nd[i, ys[1]]
+            assert nd[i, zs[0]] < box[2] <= box[5] < nd[i, zs[1]]
+
+
+# ------------------------------------------------------------------ MT
+def _mt(o, d, tri):
+    tuv = np.zeros(3, np.float32)
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    tri = np.ascontiguousarray(tri, np.float32)
+    h = po.lib().orc_mt(o.ctypes.data, d.ctypes.data, tri.ctypes.data, tuv.ctypes.data)
+    return bool(h), tuv
+
+
+UNIT = [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
+
+
+def test_mt_closed_forms():
+    h, tuv = _mt([0.25, 0.25, 1], [0, 0, -1], UNIT)
+    assert h and np.allclose(tuv, [1.0, 0.25, 0.25])
+    assert _mt([0, 0, 1], [0, 0, -1], UNIT)[0]            # vertex, u = v = 0
+    assert _mt([0.5, 0.5, 1], [0, 0, -1], UNIT)[0]        # edge u + v = 1 (inclusive)
+    assert not _mt([0.6, 0.6, 1], [0, 0, -1], UNIT)[0]
+    assert not _mt([-0.01, 0.2, 1], [0, 0, -1], UNIT)[0]  # u < 0 side
+    h, tuv = _mt([0.2, 0.2, 0], [0, 0, -1], UNIT)          # t = 0 accepted
+    assert h and tuv[0] == 0.0
+    assert not _mt([0.2, 0.2, -1], [0, 0, -1], UNIT)[0]   # behind: t < 0
+    assert _mt([0.2, 0.2, 1], [0, 0, 1], UNIT)[0] is False
+    # two-sided: reversed winding still hits
+    assert _mt([0.25, 0.25, 1], [0, 0, -1], [UNIT[0], UNIT[2], UNIT[1]])[0]
+
+
+def test_mt_absolute_det_threshold():
+    # triangle (0,0,0),(s,0,0),(0,s,0) with d=(0,0,-1): det = s^2 exactly for these s
+    for s, hit in [(0.003, False), (0.0032, True), (0.004, True)]:
+        tri = [[0, 0, 0], [s, 0, 0], [0, s, 0]]
+        assert _mt([s / 4, s / 4, 1], [0, 0, -1], tri)[0] == hit
+
+
+def test_mt_random_against_float64():
+    rng = np.random.default_rng(7)
+    agree = 0
+    for _ in range(3000):
+        tri = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        o = rng.uniform(-2, 2, 3).astype(np.float32)
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        h, tuv = _mt(o, d, tri)
+        # float64 closed form (Cramer's rule)
+        t64 = tri.astype(np.float64)
+        e1, e2 = t64[1] - t64[0], t64[2] - t64[0]
+        M = np.stack([-d.astype(np.float64), e1, e2], 1)
+        det = np.linalg.det(M)
+        if abs(det) < 1e-3:
+            continue
+        t, u, v = np.linalg.solve(M, o.astype(np.float64) - t64[0])
+        margin = min(abs(u), abs(v), abs(1 - u - v), abs(t))
+        if margin < 1e-4:
+            continue
+        h64 = t >= 0 and u >= 0 and v >= 0 and u + v <= 1
+        assert h == h64
+        if h:
+            assert np.allclose(tuv, [t, u, v], atol=1e-4)
+        agree += 1
+    assert agree > 1500
+
+
+# ------------------------------------------------------------------ camera, quantisation
+def test_camera_ray_formula():
+    cam = scenes.make_camera((0, 1, 4), (0, 0, 0), 60.0, 64, 48)
+    d = np.zeros(3, np.float32)
+    po.lib().orc_camera_ray(cam.ctypes.data, 32, 24, d.ctypes.data)   # centre: aa = bb = 0
+    v = cam["view"].astype(np.float64)
+    assert np.allclose(d, v / np.linalg.norm(v), atol=1e-7)
+    po.lib().orc_camera_ray(cam.ctypes.data, 0, 0, d.ctypes.data)     # bottom-left: aa = bb = -1
+    e = (-cam["right"] - cam["up"] + cam["view"]).astype(np.float64)
+    assert np.allclose(d, e / np.linalg.norm(e), atol=1e-7)
+    # the vertical half-angle is atan(|up|) = 30 degrees
+    po.lib().orc_camera_ray(cam.ctypes.data, 32, 0, d.ctypes.data)
+    ang = math.degrees(math.acos(float(np.dot(d, v / np.linalg.norm(v)))))
+    assert abs(ang - 30.0) < 1e-3
+
+
+def test_quantisation():
+    q = po.lib().orc_quantize
+    assert q(1.0) == 255 and q(0.0) == 0 and q(0.5) == 128
+    assert q(0.999) == 255 and q(2.0) == 255
+    assert q(1.0 / 256) == 1 and q(0.99 / 256) == 0
+    assert q(np.float32(254.999 / 256)) == 254
+
+
+# ------------------------------------------------------------------ shading closed forms
+def _quad(y, half, z0=0.0):
+    v, n = scenes.quad_grid(1, half, y)
+    v[..., 2] += z0
+    return v, n
+
+
+def test_phong_closed_form_on_a_quad():
+    v, n = _quad(0.0, 2.0)
+    mat = scenes.make_material((0.6, 0.5, 0.4), (0.3, 0.2, 0.1), 8, 0.0, False)[None]
+    light = scenes.make_lights([(0.3, 2.0, 0.4)], [(0.9, 0.8, 0.7)])
+    cam = scenes.make_camera((0.0, 3.0, 2.0), (0.0, 0.0, 0.0), 40.0, 24, 24)
+    s = scenes.SceneData(v, n, np.zeros(2, np.int32), mat, light, cam, 1, "quad")
+    out = po.Oracle(s).render()
+    rgb, hit = out["rgb"], out["hit"]
+    assert (hit >= 0).sum() > 200
+    ys, xs = np.nonzero(hit >= 0)
+    d = np.zeros(3, np.float32)
+    for y, x in list(zip(ys, xs))[::17]:
+        po.lib().orc_camera_ray(cam.ctypes.data, int(x), int(y), d.ctypes.data)
+        o = cam["position"].astype(np.float64)
+        t = -o[1] / float(d[1])
+        P = o + t * d.astype(np.float64)
+        N = np.array([0, 1.0, 0])
+        L = light["position"][0, :3] - P
+        L /= np.linalg.norm(L)
+        V = o - P
+        V /= np.linalg.norm(V)
+        R = 2 * np.dot(N, L) * N - L
+        R /= np.linalg.norm(R)
+        diff = max(0.0, np.dot(L, N)) * mat["kd"][0, :3] * light["color"][0, :3]
+        spec = (np.dot(V, R) ** 8) * light["color"][0, :3] * mat["ks"][0, :3] if np.dot(V, R) > 0 else 0
+        assert np.allclose(rgb[y, x], diff + spec, atol=2e-6)
+
+
+def test_shadow_occluder():
+    ground_v, ground_n = _quad(0.0, 3.0)
+    occ_v, occ_n = _quad(1.0, 0.5)          # 1x1 square at y=1 over the origin
+    v = np.concatenate([ground_v, occ_v])
+    n = np.concatenate([ground_n, occ_n])
+    mats = np.stack([scenes.make_material((0.5, 0.5, 0.5), (0, 0, 0), 4),
+                     scenes.make_material((0.2, 0.2, 0.2), (0, 0, 0), 4)])
+    light = scenes.make_lights([(0.0, 3.0, 0.0)])
+    cam = scenes.make_camera((0.0, 6.0, 0.01), (0.0, 0.0, 0.0), 60.0, 48, 48, world_up=(0, 0, -1))
+    s = scenes.SceneData(v, n, np.array([0, 0, 1, 1], np.int32), mats, light, cam, 0, "shadow")
+    out = po.Oracle(s, brute_force=True).render(want_recs=True)
+    rec = out["recs"][0]
+    ground = rec["mat"] == 0
+    ground &= rec["tri"] >= 0
+    P = rec["p"]
+    # umbra of a 1x1 square at height 1 from a point light at height 3 on y=0: |x|,|z| < 0.75
+    inside = (np.abs(P[..., 0]) < 0.74) & (np.abs(P[..., 2]) < 0.74)
+    outside = (np.abs(P[..., 0]) > 0.76) | (np.abs(P[..., 2]) > 0.76)
+    assert (ground & inside).sum() > 0 and (ground & outside).sum() > 0
+    assert (rec["vis"][ground & inside] == 0).all()
+    assert (rec["vis"][ground & outside] == 1).all()
+    assert (out["rgb"][ground & inside] == 0).all()     # no ambient term (src/RayTracer.cpp:412-413)
+
+
+def test_shadow_counts_geometry_beyond_the_light():
+    """src/BVH.cpp:661-716 has no upper t bound: a blocker behind the light still shadows."""
+    ground_v, ground_n = _quad(0.0, 3.0)
+    roof_v, roof_n = _quad(5.0, 10.0)       # above the light (light at y = 3)
+    v = np.concatenate([ground_v, roof_v])
+    n = np.concatenate([ground_n, roof_n])
+    mats = scenes.make_material((0.5, 0.5, 0.5), (0, 0, 0), 4)[None]
+    cam = scenes.make_camera((0.0, 2.0, 4.0), (0.0, 0.0, 0.0), 60.0, 16, 16)
+    s = scenes.SceneData(v, n, np.zeros(4, np.int32), mats, scenes.make_lights([(0, 3.0, 0)]), cam, 0, "roof")
+    out = po.Oracle(s, brute_force=True).render(want_recs=True)
+    rec = out["recs"][0]
+    g = (rec["tri"] >= 0) & (rec["tri"] < 2)
+    assert g.sum() > 0 and (rec["vis"][g] == 0).all()
+
+
+def _phong64(P, N, cam, lp, lc, kd, ks, sh):
+    L = lp - P
+    L /= np.linalg.norm(L, axis=-1, keepdims=True)
+    V = cam - P
+    V /= np.linalg.norm(V, axis=-1, keepdims=True)
+    dNL = np.sum(N * L, -1, keepdims=True)
+    R = 2 * dNL * N - L
+    R /= np.linalg.norm(R, axis=-1, keepdims=True)
+    dVR = np.sum(V * R, -1, keepdims=True)
+    spec = np.where(dVR > 0, np.abs(dVR) ** sh[..., None], 0.0) * lc * ks
+    return np.maximum(dNL, 0.0) * kd * lc + spec
+
+
+def test_reflection_recursion_closed_form():
+    """colour = sum_k (prod_{j<k} r_j) * sum_{visible lights} phong_k, evaluated in
+    float64 from the recorded hits (src/RayTracer.cpp:302-416, :348-357)."""
+    a = scenes.config_a(40, 40)
+    mats = a.materials.copy()
+    mats["reflective"] = 1
+    mats["reflectivity"] = (0.4, 0.3)
+    mats["ks"][1, :3] = 0.25
+    lights = scenes.make_lights([(2.0, 4.0, 3.0), (-3.0, 2.0, 1.0)], [(0.7, 0.6, 0.5), (0.3, 0.4, 0.5)])
+    s = scenes.SceneData(a.verts, a.normals, a.mat, mats, lights, a.camera, 3, "mirror-sphere")
+    out = po.Oracle(s).render(want_recs=True)
+    recs = out["recs"]
+    cam = s.camera["position"].astype(np.float64)
+    col = np.zeros((40, 40, 3))
+    w = np.ones((40, 40, 1))
+    alive = np.ones((40, 40), bool)
+    for level in range(s.depth + 1):
+        r = recs[level]
+        alive &= r["tri"] >= 0
+        m = s.materials[r["mat"]]
+        P = r["p"].astype(np.float64)
+        N = r["n"].astype(np.float64)
+        acc = np.zeros((40, 40, 3))
+        for i in range(len(s.lights)):
+            vis = ((r["vis"] >> i) & 1).astype(bool) & alive
+            ph = _phong64(P, N, cam, s.lights["position"][i, :3], s.lights["color"][i, :3],
+                          m["kd"][..., :3], m["ks"][..., :3], m["shininess"])
+            acc += np.where(vis[..., None], ph, 0.0)
+        col += w * acc
+        w = w * np.where(alive, m["reflectivity"], 0.0)[..., None]
+    assert np.abs(col - out["rgb"]).max() < 2e-5
+    assert (recs[1]["tri"] >= 0).sum() > 100 and (recs[3]["tri"] >= 0).sum() > 10   # reflections exercised
+
+
+# ------------------------------------------------------------------ exact nearest hit
+@pytest.mark.parametrize("seed", [1, 2])
+def test_accelerated_nearest_equals_brute_force(seed):
+    s = scenes.config_b(40, 30, n=1500)
+    s = scenes.SceneData(s.verts * np.float32(1.0 + seed), s.normals, s.mat, s.materials, s.lights,
+                         s.camera, 2, "soup")
+    a = po.Oracle(s).render()
+    b = po.Oracle(s, brute_force=True).render()
+    assert np.array_equal(a["hit"], b["hit"])
+    assert np.array_equal(a["rgba"], b["rgba"])
+    assert a["counts"] == b["counts"]
+
+
+def test_lbvh_traversal_restatement_is_exact():
+    s = scenes.config_c(48, 32, nu=48, nv=24)
+    bf = po.Oracle(s, brute_force=True)
+    rng = np.random.default_rng(3)
+    o = rng.uniform(-2, 2, (4000, 3)).astype(np.float32)
+    d = rng.normal(size=(4000, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    ref = bf.nearest(o, d)
+    for leaf_max in (1, 4, 8):
+        b = po.lbvh_build(s.verts, leaf_max)
+        r = po.lbvh_trace(b, o, d)
+        assert np.array_equal(r["hit"], ref["tri"])
+        a = po.lbvh_trace(b, o, d, any_hit=True)
+        assert np.array_equal(a["hit"] >= 0, bf.nearest(o, d, any_hit=True))
+        assert r["visits"] > 0 and r["tests"] > 0
+
+
+def test_counts_definition():
+    s = scenes.config_a(32, 32)
+    out = po.Oracle(s).render(want_recs=True)
+    c = out["counts"]
+    recs = out["recs"]
+    hits = sum(int((recs[level]["tri"] >= 0).sum()) for level in range(s.depth + 1))
+    assert c["primary"] == 32 * 32
+    assert c["hits"] == hits
+    assert c["shadow"] == hits * len(s.lights)
+    # secondary = reflective hits below the maximum depth (sphere is reflective, plane is not)
+    assert c["secondary"] == int(((recs[0]["tri"] >= 0) & (recs[0]["mat"] == 0)).sum())
+
+
+# ------------------------------------------------------------------ FP64 gradients
+def test_fp64_forward_matches_fp32_image_and_gradients_agree():
+    s = scenes.config_e(24, 24, nu=40, nv=20, n_materials=6, n_lights=3)
+    orc = po.Oracle(s)
+    out = orc.render(want_recs=True)
+    prm = po.params_vector(s.materials, s.lights)
+    tm, tl = scenes.perturb_params(s, 99)
+    orc.set_params(tm, tl)
+    target = orc.render()["rgb"]
+    orc.set_params(s.materials, s.lights)
+    loss, img = po.loss64(out["recs"], s.camera["position"], len(s.materials), len(s.lights), prm, target)
+    assert np.allclose(img.reshape(out["rgb"].shape), out["rgb"], atol=3e-6)
+    g_fwd = po.grad_fwd64(out["recs"], s.camera["position"], len(s.materials), len(s.lights), prm, target)
+    which = np.arange(len(prm))
+    g_fd = po.grad_fd64(out["recs"], s.camera["position"], len(s.materials), len(s.lights), prm, target, which,
+                        h=1e-4)
+    scale = np.abs(g_fwd).max()
+    assert scale > 0
+    err = np.abs(g_fwd - g_fd) / np.maximum(np.abs(g_fwd), 1e-3 * scale)
+    assert err.max() < 1e-3, err.max()
