@@ -78,6 +78,10 @@ def cpu_same_algorithm(scene, stride: int):
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "sample": f"every {stride}th row", "seconds": dt}
 
 
+def _progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +90,9 @@ def main() -> int:
     ap.add_argument("--leaf-max", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-stride", type=int, default=32)
+    ap.add_argument("--profile-copy-mib", type=int, default=0,
+                    help="before the frames, run one device-to-device copy of this many MiB "
+                         "(a kernel of known traffic for calibrating rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's full frame; --gpus {args.gpus} is not supported "
@@ -101,6 +108,7 @@ def main() -> int:
     t0 = time.perf_counter()
     scene = scenes.config_c()
     gen_s = time.perf_counter() - t0
+    _progress(f"scene generated in {gen_s:.1f} s ({scene.n_tri} triangles)")
     dev = torch.device("cuda:0")
     torch.cuda.init()
     g = E.Scene(scene, leaf_max=args.leaf_max)
@@ -109,7 +117,18 @@ def main() -> int:
     out = torch.empty(W * H * 4, dtype=torch.uint8, device=dev)
     cam = scene.camera
 
+    _progress(f"LBVH built in {build['build_ms']:.1f} ms")
+    if args.profile_copy_mib > 0:
+        n = args.profile_copy_mib << 20
+        src = torch.ones(n, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        _progress(f"calibration copy of {args.profile_copy_mib} MiB done")
     counted = g.render_device(cam, depth, out.data_ptr(), count_visits=True)
+    _progress(f"counting frame done: {counted['render_ms']:.1f} ms")
     for _ in range(args.warmup):
         g.render_device(cam, depth, out.data_ptr())
     torch.cuda.synchronize()
@@ -119,6 +138,7 @@ def main() -> int:
         per.append(g.render_device(cam, depth, out.data_ptr()))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+    _progress(f"{args.steps} timed frames: {wall * 1e3 / args.steps:.2f} ms/frame")
 
     st = per[-1]
     rays = st["primary_rays"] + st["shadow_rays"] + st["secondary_rays"]
@@ -168,10 +188,13 @@ def main() -> int:
         pass
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        _progress(f"CPU baseline (reference port, {threads} threads)")
         cb = cpu_baseline(scene, args.cpu_stride, threads)
+        _progress(f"CPU baseline: {cb['value']:.6f} Mrays/s in {cb['seconds']:.1f} s")
         result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         result["gpu_vs_cpu_baseline"] = round(value / cb["value"], 1)
         sa = cpu_same_algorithm(scene, 64)
+        _progress(f"CPU same-algorithm: {sa['value']:.3f} Mrays/s")
         result["cpu_same_algorithm"] = {"value": round(sa["value"], 4), "unit": "Mrays/s",
                                         "sample": sa["sample"], "cores": threads,
                                         "note": "oracle restatement of the GPU's LBVH traversal, -O2, OpenMP"}

@@ -220,6 +220,15 @@ class Scene:
                                        C.byref(st)))
         return gm, gl[:len(self._lights)], {k: getattr(st, k) for k, _ in st._fields_}
 
+    def adjoint_device(self, camera, depth, dl_dev_ptr, gm_dev_ptr, gl_dev_ptr, recompute=False) -> dict:
+        """Adjoint with device buffers: dl [H][W][3] float32, gm [M][7] and gl [L][6] float64 (accumulated)."""
+        cam = np.array(camera, copy=True)
+        o = AdjointOpts(depth, EOS_MEM_DEVICE, int(recompute), 0)
+        st = AdjointStats()
+        check(lib().eos_render_adjoint(self.h, cam.ctypes.data, C.byref(o), dl_dev_ptr, gm_dev_ptr, gl_dev_ptr,
+                                       C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
     def debug_read(self, what: str):
         n = self.data.n_tri
         m = max(n - 1, 0)

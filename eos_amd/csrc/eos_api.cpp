@@ -185,6 +185,7 @@ eos_status eos_update_materials(eos_scene* s, const eos_material* m, int32_t n) 
     EOS_TRY(check_materials(m, n));
     EOS_HIP(hipMemcpyAsync(s->mats.ptr, m, sizeof(eos_material) * n, hipMemcpyHostToDevice, s->stream));
     EOS_HIP(hipStreamSynchronize(s->stream));
+    s->fb.rec_valid = 0;  // reflection flags and weights shape the stored records
     return EOS_OK;
 }
 
@@ -194,7 +195,7 @@ eos_status eos_update_lights(eos_scene* s, const eos_light* l, int32_t n) {
     EOS_TRY(check_lights(l, n));
     if (n > 0) EOS_HIP(hipMemcpyAsync(s->lights.ptr, l, sizeof(eos_light) * n, hipMemcpyHostToDevice, s->stream));
     EOS_HIP(hipStreamSynchronize(s->stream));
-    if (n != s->n_light) s->fb.rec_valid = 0;
+    s->fb.rec_valid = 0;  // light positions decide the stored shadow masks
     s->n_light = n;
     return EOS_OK;
 }

@@ -143,4 +143,7 @@ eos_status render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint
 eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n, const eos_batch_opts* o,
                         uint8_t* const* out, eos_batch_stats* st);
 SceneView view_of(const eos_scene* s);
+// One frame of the wavefront pipeline (render.hip); outputs may be null.
+eos_status run_frame(eos_scene* s, const eos_camera* cam, int depth, bool count, uchar4* rgba_dev, float* rgb_dev,
+                     int32_t* hits_dev, eos_render_stats* st);
 }  // namespace eos

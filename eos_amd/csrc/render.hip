@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "eos_internal.h"
 #include "trace.h"
@@ -28,7 +29,10 @@ namespace {
 struct SetupArgs {
     SceneView sc;
     DevCamera cam;
-    int tiles_x, n_ids, depth, level;
+    int tiles_x;
+    const BatchScene* table;
+    int n_scenes;
+    int n_ids, depth, level;
     const int* count_ptr;     // level >= 1: queued rays
     const float4* q_in;       // level >= 1
     const float4* hit_in;
@@ -36,6 +40,7 @@ struct SetupArgs {
     int* hits_ctr;
     float4* q_out;
     int* q_out_ctr;
+    int* shadow_ctr;
     uint8_t* levels;
     int32_t* hit_ids;         // level 0, optional
 };
@@ -43,7 +48,7 @@ struct SetupArgs {
 // src/Triangle.cpp:41-47 and src/BVH.cpp:371-381: point o + t*d; normal
 // normalize(n1*(1-u-v) + n2*u + n3*v), normalised again after the (identity)
 // inverse-transpose transform.
-template <bool L0>
+template <bool L0, bool BATCH>
 __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
     const int n = L0 ? a.n_ids : *a.count_ptr;
     const int lane = __lane_id();
@@ -53,28 +58,51 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
         const bool valid = i < n;
         float4 h = valid ? a.hit_in[i] : make_float4(INFINITY, __int_as_float(-1), 0.f, 0.f);
         const int slot = __float_as_int(h.y);
-        int pixel = 0;
+        int pixel = 0, scene = 0;
         F3 o = {0, 0, 0}, d = {0, 0, 0};
         bool pix_ok = valid;
+        const float4* shade = a.sc.shade;
+        const float4* geo = a.sc.geo;
+        const DevMaterial* mats = a.sc.mats;
         if (L0) {
             int x = 0, y = 0;
-            pix_ok = valid && tile_pixel(i, a.tiles_x, a.cam.width, a.cam.height, x, y);
-            pixel = y * a.cam.width + x;
-            o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-            if (pix_ok) d = camera_dir(a.cam, x, y);
+            if (BATCH) {
+                if (valid) {
+                    scene = scene_of_id(a.table, a.n_scenes, i);
+                    const BatchScene& B = a.table[scene];
+                    pix_ok = tile_pixel(i - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
+                    pixel = B.pix_off + y * B.cam.width + x;
+                    o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
+                    if (pix_ok) d = camera_dir(B.cam, x, y);
+                    shade = B.sc.shade;
+                    geo = B.sc.geo;
+                    mats = B.sc.mats;
+                }
+            } else {
+                pix_ok = valid && tile_pixel(i, a.tiles_x, a.cam.width, a.cam.height, x, y);
+                pixel = y * a.cam.width + x;
+                o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
+                if (pix_ok) d = camera_dir(a.cam, x, y);
+            }
         } else if (valid) {
             const float4 q0 = a.q_in[2 * (size_t)i], q1 = a.q_in[2 * (size_t)i + 1];
             o = f3(q0.x, q0.y, q0.z);
             d = f3(q1.x, q1.y, q1.z);
             pixel = __float_as_int(q0.w);
+            if (BATCH) {
+                scene = __float_as_int(q1.w);
+                const BatchScene& B = a.table[scene];
+                shade = B.sc.shade;
+                geo = B.sc.geo;
+                mats = B.sc.mats;
+            }
         }
         const bool hit = pix_ok && slot >= 0;
         F3 P = {0, 0, 0}, N = {0, 0, 0};
         int mat = 0;
         bool refl = false;
         if (hit) {
-            const float4 s0 = a.sc.shade[3 * (size_t)slot], s1 = a.sc.shade[3 * (size_t)slot + 1],
-                         s2 = a.sc.shade[3 * (size_t)slot + 2];
+            const float4 s0 = shade[3 * (size_t)slot], s1 = shade[3 * (size_t)slot + 1], s2 = shade[3 * (size_t)slot + 2];
             P = add(o, scale(h.x, d));
             const float u = h.z, v = h.w;
             const float w = (1.0f - u) - v;
@@ -83,18 +111,19 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
             const F3 nc = {s2.x * v, s2.y * v, s2.z * v};
             N = normalize3(normalize3(add(add(na, nb), nc)));
             mat = __float_as_int(s0.w);
-            refl = a.sc.mats[mat].reflective != 0 && a.level < a.depth;
+            refl = mats[mat].reflective != 0 && a.level < a.depth;
         }
         if (L0 && pix_ok) {
             a.levels[pixel] = hit ? 1 : 0;
-            if (a.hit_ids) a.hit_ids[pixel] = hit ? __float_as_int(a.sc.geo[3 * (size_t)slot + 2].y) : -1;
+            if (a.hit_ids) a.hit_ids[pixel] = hit ? __float_as_int(geo[3 * (size_t)slot + 2].y) : -1;
         } else if (!L0 && hit) {
             a.levels[pixel] = (uint8_t)(a.level + 1);
         }
         const int j = wave_append(a.hits_ctr, hit);
+        wave_sum_add(a.shadow_ctr, hit ? (BATCH ? a.table[scene].sc.n_light : a.sc.n_light) : 0);
         if (hit) {
             a.surf_out[2 * (size_t)j] = make_float4(P.x, P.y, P.z, __int_as_float(pixel));
-            a.surf_out[2 * (size_t)j + 1] = make_float4(N.x, N.y, N.z, __int_as_float(mat));
+            a.surf_out[2 * (size_t)j + 1] = make_float4(N.x, N.y, N.z, __int_as_float(pack_mat(mat, scene)));
         }
         const int k = wave_append(a.q_out_ctr, refl);
         if (refl) {
@@ -102,7 +131,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
             const F3 R = normalize3(reflect3(d, N));
             const F3 ro = add(P, scale(1e-3f, R));
             a.q_out[2 * (size_t)k] = make_float4(ro.x, ro.y, ro.z, __int_as_float(pixel));
-            a.q_out[2 * (size_t)k + 1] = make_float4(R.x, R.y, R.z, 0.0f);
+            a.q_out[2 * (size_t)k + 1] = make_float4(R.x, R.y, R.z, __int_as_float(scene));
         }
     }
 }
@@ -130,27 +159,31 @@ __device__ __forceinline__ void phong(const DevMaterial& m, const DevLight& l, F
 struct PhongArgs {
     SceneView sc;
     DevCamera cam;
-    int level, npix;
+    const BatchScene* table;
+    int level, npix, lstride;
     const int* count_ptr;
     const float4* surf;
     const uint8_t* vis;
     float4* levcol;
 };
 
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_phong(PhongArgs a) {
     const int n = *a.count_ptr;
-    const int L = a.sc.n_light;
-    const F3 cam = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const float4 s0 = a.surf[2 * (size_t)j], s1 = a.surf[2 * (size_t)j + 1];
         const F3 P = f3(s0.x, s0.y, s0.z), N = f3(s1.x, s1.y, s1.z);
-        const int pixel = __float_as_int(s0.w), mi = __float_as_int(s1.w);
-        const DevMaterial& m = a.sc.mats[mi];
+        const int pixel = __float_as_int(s0.w), packed = __float_as_int(s1.w);
+        const SceneView& sc = BATCH ? a.table[unpack_scene(packed)].sc : a.sc;
+        const DevCamera& cm = BATCH ? a.table[unpack_scene(packed)].cam : a.cam;
+        const F3 cam = f3(cm.position[0], cm.position[1], cm.position[2]);
+        const DevMaterial& m = sc.mats[unpack_mat(packed)];
+        const int L = sc.n_light;
         float acc[3] = {0.0f, 0.0f, 0.0f};
         for (int i = 0; i < L; ++i) {
-            if (!a.vis[(size_t)j * L + i]) continue;
+            if (!a.vis[(size_t)j * a.lstride + i]) continue;
             float c[3];
-            phong(m, a.sc.lights[i], cam, P, N, c);
+            phong(m, sc.lights[i], cam, P, N, c);
 #pragma unroll
             for (int k = 0; k < 3; ++k) acc[k] = acc[k] + c[k];
         }
@@ -199,14 +232,14 @@ __global__ void k_zero_i32(int* p, int n) {
 }
 
 // ---- traversal launch table ------------------------------------------------
-template <int MODE, bool COUNT>
+template <int MODE, bool COUNT, bool BATCH>
 eos_status launch_trace(const LaunchCfg& cfg, int grid, hipStream_t s, const TraceArgs& a) {
-#define EOS_TRACE_CASE(WG, SL)                                                                 \
-    case WG * 100 + SL:                                                                         \
-        hipLaunchKernelGGL((k_trace<MODE, WG, SL, COUNT>), dim3(grid), dim3(WG), 0, s, a);      \
+#define EOS_TRACE_CASE(WG, SL)                                                                        \
+    case WG * 100 + SL:                                                                                \
+        hipLaunchKernelGGL((k_trace<MODE, WG, SL, COUNT, BATCH>), dim3(grid), dim3(WG), 0, s, a);      \
         break;
     if (COUNT) {
-        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT>), dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH>), dim3(grid), dim3(256), 0, s, a);
     } else {
         switch (cfg.workgroup * 100 + cfg.stack_lds) {
             EOS_TRACE_CASE(64, 8)
@@ -227,8 +260,10 @@ eos_status launch_trace(const LaunchCfg& cfg, int grid, hipStream_t s, const Tra
 }
 
 template <int MODE>
-eos_status launch_trace_any(bool count, const LaunchCfg& cfg, int grid, hipStream_t s, const TraceArgs& a) {
-    return count ? launch_trace<MODE, true>(cfg, grid, s, a) : launch_trace<MODE, false>(cfg, grid, s, a);
+eos_status launch_trace_any(bool count, bool batch, const LaunchCfg& cfg, int grid, hipStream_t s,
+                            const TraceArgs& a) {
+    if (batch) return launch_trace<MODE, false, true>(cfg, grid, s, a);
+    return count ? launch_trace<MODE, true, false>(cfg, grid, s, a) : launch_trace<MODE, false, false>(cfg, grid, s, a);
 }
 
 inline unsigned blocks_for(size_t n, unsigned t, unsigned cap) {
@@ -237,73 +272,83 @@ inline unsigned blocks_for(size_t n, unsigned t, unsigned cap) {
     return (unsigned)std::min<size_t>(b, cap);
 }
 
-}  // namespace
+// One frame: a single scene, or a batch of scenes described by a device table.
+struct Frame {
+    bool batch = false;
+    SceneView sc{};
+    DevCamera cam{};
+    int tiles_x = 0;
+    const BatchScene* table = nullptr;
+    int n_scenes = 1;
+    size_t npix = 0, n_ids = 0;
+    int depth = 0;
+    int lstride = 0;       // light slots per hit (max lights over the batch)
+    bool count = false;
+    LaunchCfg cfg;
+    int num_cu = 256;
+    uchar4* rgba = nullptr;
+    float* rgb = nullptr;
+    int32_t* hits = nullptr;
+};
 
-// Frame driver shared by eos_render and the adjoint's re-trace.
-static eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev,
-                            float* rgb_dev, int32_t* hits_dev, eos_render_stats* st) {
-    eos_frame_buffers& fb = s->fb;
-    const DevCamera cam = *reinterpret_cast<const DevCamera*>(camp);
-    const int W = cam.width, H = cam.height;
-    const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
-    const size_t npix = (size_t)W * H;
-    const size_t n_ids = (size_t)tiles_x * tiles_y * 64;
-    const int L = s->n_light;
-    const size_t cap = npix;
-    LaunchCfg cfg = s->cfg;
-    if (count) {
-        cfg.workgroup = 256;
-        cfg.stack_lds = 16;
-    }
-    const int trace_grid = std::max(1, s->num_cu * cfg.waves_per_cu / (cfg.workgroup / 64));
-    EOS_TRY(fb.hit0.ensure(16 * n_ids));
+struct FrameEvents {
+    int start, end, resolve0;
+    int c[2 * EOS_MAX_DEPTH + 2], s[2 * EOS_MAX_DEPTH + 2], p[2 * EOS_MAX_DEPTH + 2];
+};
+
+eos_status ensure_buffers(eos_frame_buffers& fb, const Frame& f, int trace_grid) {
+    const size_t cap = f.npix;
+    EOS_TRY(fb.hit0.ensure(16 * f.n_ids));
     EOS_TRY(fb.q[0].ensure(32 * cap));
     EOS_TRY(fb.q[1].ensure(32 * cap));
     EOS_TRY(fb.hitq.ensure(16 * cap));
-    for (int l = 0; l <= depth; ++l) {
+    for (int l = 0; l <= f.depth; ++l) {
         EOS_TRY(fb.surf[l].ensure(32 * cap));
-        EOS_TRY(fb.vis[l].ensure(cap * (size_t)std::max(L, 1)));
+        EOS_TRY(fb.vis[l].ensure(cap * (size_t)std::max(f.lstride, 1)));
     }
-    EOS_TRY(fb.levcol.ensure(16 * npix * (size_t)(depth + 1)));
-    EOS_TRY(fb.levels.ensure(npix));
+    EOS_TRY(fb.levcol.ensure(16 * f.npix * (size_t)(f.depth + 1)));
+    EOS_TRY(fb.levels.ensure(f.npix));
     EOS_TRY(fb.counters.ensure(4 * CTR_COUNT + 8 * kStats));
-    EOS_TRY(fb.gstack.ensure((size_t)trace_grid * cfg.workgroup * (kMaxStack - cfg.stack_lds) * 4));
-    fb.npix = npix;
-    fb.n_ids = n_ids;
-    fb.depth = depth;
-    fb.n_light = L;
+    EOS_TRY(fb.gstack.ensure((size_t)trace_grid * f.cfg.workgroup * (kMaxStack - f.cfg.stack_lds) * 4));
+    fb.npix = f.npix;
+    fb.n_ids = f.n_ids;
+    fb.depth = f.depth;
+    fb.n_light = f.lstride;
+    return EOS_OK;
+}
 
-    hipStream_t S = s->stream;
+// Enqueues every launch of one frame on stream S.  Events are recorded when
+// E is non-null (not while a graph is being captured).
+eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hipEvent_t* E, FrameEvents* fe) {
+    const int trace_grid = std::max(1, f.num_cu * f.cfg.waves_per_cu / (f.cfg.workgroup / 64));
     int* ctr = fb.counters.as<int>();
     unsigned long long* stats = reinterpret_cast<unsigned long long*>(fb.counters.as<char>() + 4 * CTR_COUNT);
-    const SceneView sc = view_of(s);
-    const unsigned shade_grid = (unsigned)s->num_cu * 8;
-
+    const unsigned shade_grid = (unsigned)f.num_cu * 8;
     int ev = 0;
-    hipEvent_t* E = s->ev;
-    auto rec = [&](void) -> int {
-        (void)hipEventRecord(E[ev], S);
+    auto rec = [&]() -> int {
+        if (E) (void)hipEventRecord(E[ev], S);
         return ev++;
     };
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, S, ctr, CTR_COUNT + 2 * kStats);
-    const int ev_start = rec();
-    int ev_c[2 * EOS_MAX_DEPTH + 2], ev_s[2 * EOS_MAX_DEPTH + 2], ev_p[2 * EOS_MAX_DEPTH + 2];
-
-    for (int l = 0; l <= depth; ++l) {
+    FrameEvents local;
+    FrameEvents& F = fe ? *fe : local;
+    F.start = rec();
+    for (int l = 0; l <= f.depth; ++l) {
         float4* qin = fb.q[l & 1].as<float4>();
         float4* qout = fb.q[(l + 1) & 1].as<float4>();
         TraceArgs ta{};
-        ta.sc = sc;
-        ta.cam = cam;
-        ta.tiles_x = tiles_x;
+        ta.sc = f.sc;
+        ta.cam = f.cam;
+        ta.tiles_x = f.tiles_x;
+        ta.table = f.table;
+        ta.n_scenes = f.n_scenes;
         ta.gstack = fb.gstack.as<int32_t>();
         ta.err = ctr + CTR_ERR;
-        ta.refill = cfg.refill;
-        // closest hit
+        ta.refill = f.cfg.refill;
         ta.cursor = ctr + CTR_CUR_C + l;
         ta.stats = stats + 2 * l;
         if (l == 0) {
-            ta.n_static = (int)n_ids;
+            ta.n_static = (int)f.n_ids;
             ta.count_ptr = nullptr;
             ta.count_mul = 1;
             ta.hit_out = fb.hit0.as<float4>();
@@ -313,17 +358,18 @@ static eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, boo
             ta.q = qin;
             ta.hit_out = fb.hitq.as<float4>();
         }
-        ev_c[2 * l] = rec();
-        if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(count, cfg, trace_grid, S, ta));
-        else EOS_TRY(launch_trace_any<MODE_QUEUE>(count, cfg, trace_grid, S, ta));
-        ev_c[2 * l + 1] = rec();
-        // hit setup + reflection rays
+        F.c[2 * l] = rec();
+        if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(f.count, f.batch, f.cfg, trace_grid, S, ta));
+        else EOS_TRY(launch_trace_any<MODE_QUEUE>(f.count, f.batch, f.cfg, trace_grid, S, ta));
+        F.c[2 * l + 1] = rec();
         SetupArgs sa{};
-        sa.sc = sc;
-        sa.cam = cam;
-        sa.tiles_x = tiles_x;
-        sa.n_ids = (int)n_ids;
-        sa.depth = depth;
+        sa.sc = f.sc;
+        sa.cam = f.cam;
+        sa.tiles_x = f.tiles_x;
+        sa.table = f.table;
+        sa.n_scenes = f.n_scenes;
+        sa.n_ids = (int)f.n_ids;
+        sa.depth = f.depth;
         sa.level = l;
         sa.count_ptr = ctr + CTR_Q + l;
         sa.q_in = qin;
@@ -332,88 +378,135 @@ static eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, boo
         sa.hits_ctr = ctr + CTR_HITS + l;
         sa.q_out = qout;
         sa.q_out_ctr = ctr + CTR_Q + l + 1;
+        sa.shadow_ctr = ctr + CTR_SHADOW;
         sa.levels = fb.levels.as<uint8_t>();
-        sa.hit_ids = l == 0 ? hits_dev : nullptr;
-        ev_p[2 * l] = rec();
-        if (l == 0)
-            hipLaunchKernelGGL(k_setup<true>, dim3(blocks_for(n_ids, 256, shade_grid)), dim3(256), 0, S, sa);
-        else
-            hipLaunchKernelGGL(k_setup<false>, dim3(shade_grid), dim3(256), 0, S, sa);
-        // shadow rays
-        ev_s[2 * l] = rec();
-        if (L > 0) {
+        sa.hit_ids = l == 0 ? f.hits : nullptr;
+        F.p[2 * l] = rec();
+        if (l == 0) {
+            if (f.batch)
+                hipLaunchKernelGGL((k_setup<true, true>), dim3(blocks_for(f.n_ids, 256, shade_grid)), dim3(256), 0, S, sa);
+            else
+                hipLaunchKernelGGL((k_setup<true, false>), dim3(blocks_for(f.n_ids, 256, shade_grid)), dim3(256), 0, S, sa);
+        } else {
+            if (f.batch) hipLaunchKernelGGL((k_setup<false, true>), dim3(shade_grid), dim3(256), 0, S, sa);
+            else hipLaunchKernelGGL((k_setup<false, false>), dim3(shade_grid), dim3(256), 0, S, sa);
+        }
+        F.s[2 * l] = rec();
+        if (f.lstride > 0) {
             TraceArgs sh = ta;
             sh.cursor = ctr + CTR_CUR_S + l;
             sh.stats = stats + 2 * (16 + l);
             sh.count_ptr = ctr + CTR_HITS + l;
-            sh.count_mul = L;
+            sh.count_mul = f.lstride;
             sh.surf = fb.surf[l].as<float4>();
             sh.vis_out = fb.vis[l].as<uint8_t>();
-            EOS_TRY(launch_trace_any<MODE_SHADOW>(count, cfg, trace_grid, S, sh));
+            EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.cfg, trace_grid, S, sh));
         }
-        ev_s[2 * l + 1] = rec();
+        F.s[2 * l + 1] = rec();
         PhongArgs pa{};
-        pa.sc = sc;
-        pa.cam = cam;
+        pa.sc = f.sc;
+        pa.cam = f.cam;
+        pa.table = f.table;
         pa.level = l;
-        pa.npix = (int)npix;
+        pa.npix = (int)f.npix;
+        pa.lstride = f.lstride;
         pa.count_ptr = ctr + CTR_HITS + l;
         pa.surf = fb.surf[l].as<float4>();
         pa.vis = fb.vis[l].as<uint8_t>();
         pa.levcol = fb.levcol.as<float4>();
-        hipLaunchKernelGGL(k_phong, dim3(shade_grid), dim3(256), 0, S, pa);
-        ev_p[2 * l + 1] = rec();
+        if (f.batch) hipLaunchKernelGGL(k_phong<true>, dim3(shade_grid), dim3(256), 0, S, pa);
+        else hipLaunchKernelGGL(k_phong<false>, dim3(shade_grid), dim3(256), 0, S, pa);
+        F.p[2 * l + 1] = rec();
     }
-    ResolveArgs ra{(int)npix, fb.levels.as<uint8_t>(), fb.levcol.as<float4>(), rgba_dev, rgb_dev};
-    const int ev_r0 = rec();
-    hipLaunchKernelGGL(k_resolve, dim3(blocks_for(npix, 256, shade_grid)), dim3(256), 0, S, ra);
-    const int ev_end = rec();
+    ResolveArgs ra{(int)f.npix, fb.levels.as<uint8_t>(), fb.levcol.as<float4>(), f.rgba, f.rgb};
+    F.resolve0 = rec();
+    hipLaunchKernelGGL(k_resolve, dim3(blocks_for(f.npix, 256, shade_grid)), dim3(256), 0, S, ra);
+    F.end = rec();
     EOS_HIP(hipGetLastError());
+    return EOS_OK;
+}
 
+// Reads the frame's counters (synchronises the stream) into stats.
+eos_status finish_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hipEvent_t* E, const FrameEvents* F,
+                        eos_render_stats* st) {
     int hctr[CTR_COUNT];
     unsigned long long hstats[kStats];
+    int* ctr = fb.counters.as<int>();
+    unsigned long long* stats = reinterpret_cast<unsigned long long*>(fb.counters.as<char>() + 4 * CTR_COUNT);
     EOS_HIP(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, S));
     EOS_HIP(hipMemcpyAsync(hstats, stats, sizeof hstats, hipMemcpyDeviceToHost, S));
     EOS_HIP(hipStreamSynchronize(S));
     if (hctr[CTR_ERR] & 1) return fail(EOS_ERR_LIMIT, "traversal stack overflow (tree deeper than 64)");
     if (hctr[CTR_ERR] & 2) return fail(EOS_ERR_STATE, "traversal exceeded its visit bound (corrupt BVH)");
-    if (st) {
-        std::memset(st, 0, sizeof *st);
-        st->primary_rays = (int64_t)npix;
-        for (int l = 0; l <= depth; ++l) {
-            st->hits += hctr[CTR_HITS + l];
-            st->shadow_rays += (int64_t)hctr[CTR_HITS + l] * L;
-            if (l >= 1) st->secondary_rays += hctr[CTR_Q + l];
-        }
+    if (!st) return EOS_OK;
+    std::memset(st, 0, sizeof *st);
+    st->primary_rays = (int64_t)f.npix;
+    for (int l = 0; l <= f.depth; ++l) {
+        st->hits += hctr[CTR_HITS + l];
+        if (l >= 1) st->secondary_rays += hctr[CTR_Q + l];
+    }
+    st->shadow_rays = hctr[CTR_SHADOW];
+    if (E && F) {
         float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, E[ev_start], E[ev_end]);
+        (void)hipEventElapsedTime(&ms, E[F->start], E[F->end]);
         st->render_ms = ms;
-        for (int l = 0; l <= depth; ++l) {
+        for (int l = 0; l <= f.depth; ++l) {
             float a = 0.f, b = 0.f, c = 0.f, d2 = 0.f;
-            (void)hipEventElapsedTime(&a, E[ev_c[2 * l]], E[ev_c[2 * l + 1]]);
-            (void)hipEventElapsedTime(&b, E[ev_s[2 * l]], E[ev_s[2 * l + 1]]);
-            (void)hipEventElapsedTime(&c, E[ev_p[2 * l]], E[ev_s[2 * l]]);
-            (void)hipEventElapsedTime(&d2, E[ev_s[2 * l + 1]], E[ev_p[2 * l + 1]]);
+            (void)hipEventElapsedTime(&a, E[F->c[2 * l]], E[F->c[2 * l + 1]]);
+            (void)hipEventElapsedTime(&b, E[F->s[2 * l]], E[F->s[2 * l + 1]]);
+            (void)hipEventElapsedTime(&c, E[F->p[2 * l]], E[F->s[2 * l]]);
+            (void)hipEventElapsedTime(&d2, E[F->s[2 * l + 1]], E[F->p[2 * l + 1]]);
             st->trace_closest_ms += a;
             st->trace_shadow_ms += b;
             st->shade_ms += c + d2;
             if (l == 0) st->trace_first_ms = a;
         }
         float r = 0.f;
-        (void)hipEventElapsedTime(&r, E[ev_r0], E[ev_end]);
+        (void)hipEventElapsedTime(&r, E[F->resolve0], E[F->end]);
         st->shade_ms += r;
-        st->trace_closest_launches = depth + 1;
-        st->trace_shadow_launches = L > 0 ? depth + 1 : 0;
-        if (count) {
-            for (int l = 0; l <= depth; ++l) {
-                st->node_visits += (int64_t)(hstats[2 * l] + hstats[2 * (16 + l)]);
-                st->tri_tests += (int64_t)(hstats[2 * l + 1] + hstats[2 * (16 + l) + 1]);
-            }
-            st->closest_visits_first = (int64_t)hstats[0];
-            st->closest_tests_first = (int64_t)hstats[1];
+    }
+    st->trace_closest_launches = f.depth + 1;
+    st->trace_shadow_launches = f.lstride > 0 ? f.depth + 1 : 0;
+    if (f.count) {
+        for (int l = 0; l <= f.depth; ++l) {
+            st->node_visits += (int64_t)(hstats[2 * l] + hstats[2 * (16 + l)]);
+            st->tri_tests += (int64_t)(hstats[2 * l + 1] + hstats[2 * (16 + l) + 1]);
         }
+        st->closest_visits_first = (int64_t)hstats[0];
+        st->closest_tests_first = (int64_t)hstats[1];
     }
     return EOS_OK;
+}
+
+}  // namespace
+
+// Frame driver shared by eos_render and the adjoint's re-trace.
+eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev,
+                     float* rgb_dev, int32_t* hits_dev, eos_render_stats* st) {
+    Frame f;
+    f.cam = *reinterpret_cast<const DevCamera*>(camp);
+    f.sc = view_of(s);
+    f.tiles_x = (f.cam.width + 7) / 8;
+    const int tiles_y = (f.cam.height + 7) / 8;
+    f.npix = (size_t)f.cam.width * f.cam.height;
+    f.n_ids = (size_t)f.tiles_x * tiles_y * 64;
+    f.depth = depth;
+    f.lstride = s->n_light;
+    f.count = count;
+    f.cfg = s->cfg;
+    if (count) {
+        f.cfg.workgroup = 256;
+        f.cfg.stack_lds = 16;
+    }
+    f.num_cu = s->num_cu;
+    f.rgba = rgba_dev;
+    f.rgb = rgb_dev;
+    f.hits = hits_dev;
+    const int trace_grid = std::max(1, f.num_cu * f.cfg.waves_per_cu / (f.cfg.workgroup / 64));
+    EOS_TRY(ensure_buffers(s->fb, f, trace_grid));
+    FrameEvents fe;
+    EOS_TRY(enqueue_frame(s->fb, f, s->stream, s->ev, &fe));
+    return finish_frame(s->fb, f, s->stream, s->ev, &fe, st);
 }
 
 eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
@@ -461,14 +554,135 @@ eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o,
     return EOS_OK;
 }
 
-eos_status render_adjoint(eos_scene*, const eos_camera*, const eos_adjoint_opts*, const float*, double*, double*,
-                          eos_adjoint_stats*) {
-    return fail(EOS_ERR_STATE, "eos_render_adjoint: not built yet");
+// ---- batched frames ----------------------------------------------------------
+namespace {
+
+// Buffers, scene table and captured graph of the last batch (per process;
+// eos_render_batch calls must not overlap).
+struct BatchState {
+    eos_frame_buffers fb;
+    DevBuf table, rgba;
+    std::vector<BatchScene> host_table;
+    std::vector<eos_scene*> scenes;
+    std::vector<eos_camera> cams;
+    int depth = -1;
+    LaunchCfg cfg;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    Frame frame;
+};
+BatchState g_batch;
+
+bool same_batch(eos_scene* const* scenes, const eos_camera* cams, int n, int depth, const LaunchCfg& cfg) {
+    const BatchState& b = g_batch;
+    if ((int)b.scenes.size() != n || b.depth != depth) return false;
+    if (b.cfg.workgroup != cfg.workgroup || b.cfg.stack_lds != cfg.stack_lds || b.cfg.waves_per_cu != cfg.waves_per_cu ||
+        b.cfg.refill != cfg.refill)
+        return false;
+    for (int i = 0; i < n; ++i) {
+        if (b.scenes[i] != scenes[i]) return false;
+        if (std::memcmp(&b.cams[i], &cams[i], sizeof(eos_camera)) != 0) return false;
+        const SceneView v = view_of(scenes[i]);   // rebuilt BVH or changed light count
+        if (std::memcmp(&v, &b.host_table[i].sc, sizeof(SceneView)) != 0) return false;
+    }
+    return true;
 }
 
-eos_status render_batch(eos_scene* const*, const eos_camera*, int, const eos_batch_opts*, uint8_t* const*,
-                        eos_batch_stats*) {
-    return fail(EOS_ERR_STATE, "eos_render_batch: not built yet");
+}  // namespace
+
+eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n, const eos_batch_opts* o,
+                        uint8_t* const* out, eos_batch_stats* st) {
+    BatchState& b = g_batch;
+    eos_scene* s0 = scenes[0];
+    hipStream_t S = s0->stream;
+    const LaunchCfg cfg = s0->cfg;
+    if (!b.ev[0]) {
+        EOS_HIP(hipEventCreate(&b.ev[0]));
+        EOS_HIP(hipEventCreate(&b.ev[1]));
+    }
+    const bool same = same_batch(scenes, cams, n, o->depth, cfg);
+    if (!same) {
+        if (b.exec) {
+            (void)hipGraphExecDestroy(b.exec);
+            b.exec = nullptr;
+        }
+        b.host_table.assign(n, BatchScene{});
+        size_t ids = 0, pix = 0;
+        int lmax = 0;
+        for (int i = 0; i < n; ++i) {
+            BatchScene& B = b.host_table[i];
+            B.sc = view_of(scenes[i]);
+            B.cam = *reinterpret_cast<const DevCamera*>(&cams[i]);
+            B.tiles_x = (B.cam.width + 7) / 8;
+            B.id_off = (int)ids;
+            B.pix_off = (int)pix;
+            B.n_ids = B.tiles_x * ((B.cam.height + 7) / 8) * 64;
+            ids += (size_t)B.n_ids;
+            pix += (size_t)B.cam.width * B.cam.height;
+            lmax = std::max(lmax, scenes[i]->n_light);
+        }
+        if (ids >= ((size_t)1 << 31) || pix >= ((size_t)1 << 31)) return fail(EOS_ERR_LIMIT, "batch too large");
+        EOS_TRY(b.table.ensure(sizeof(BatchScene) * n));
+        EOS_HIP(hipMemcpyAsync(b.table.ptr, b.host_table.data(), sizeof(BatchScene) * n, hipMemcpyHostToDevice, S));
+        EOS_TRY(b.rgba.ensure(4 * pix));
+        Frame& f = b.frame;
+        f = Frame();
+        f.batch = true;
+        f.table = b.table.as<const BatchScene>();
+        f.n_scenes = n;
+        f.npix = pix;
+        f.n_ids = ids;
+        f.depth = o->depth;
+        f.lstride = lmax;
+        f.cfg = cfg;
+        f.num_cu = s0->num_cu;
+        f.rgba = b.rgba.as<uchar4>();
+        f.sc = view_of(s0);
+        f.cam = b.host_table[0].cam;
+        const int trace_grid = std::max(1, f.num_cu * f.cfg.waves_per_cu / (f.cfg.workgroup / 64));
+        EOS_TRY(ensure_buffers(b.fb, f, trace_grid));
+        b.scenes.assign(scenes, scenes + n);
+        b.cams.assign(cams, cams + n);
+        b.depth = o->depth;
+        b.cfg = cfg;
+    }
+    const Frame& f = b.frame;
+    if (o->use_graph) {
+        if (!b.exec) {
+            hipGraph_t graph = nullptr;
+            EOS_HIP(hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
+            eos_status es = enqueue_frame(b.fb, f, S, nullptr, nullptr);
+            hipError_t ce = hipStreamEndCapture(S, &graph);
+            if (es != EOS_OK) return es;
+            if (ce != hipSuccess) return fail(EOS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&b.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(EOS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
+        EOS_HIP(hipEventRecord(b.ev[0], S));
+        EOS_HIP(hipGraphLaunch(b.exec, S));
+        EOS_HIP(hipEventRecord(b.ev[1], S));
+    } else {
+        EOS_HIP(hipEventRecord(b.ev[0], S));
+        EOS_TRY(enqueue_frame(b.fb, f, S, nullptr, nullptr));
+        EOS_HIP(hipEventRecord(b.ev[1], S));
+    }
+    eos_render_stats rs;
+    EOS_TRY(finish_frame(b.fb, f, S, nullptr, nullptr, &rs));
+    for (int i = 0; i < n; ++i) {
+        const BatchScene& B = b.host_table[i];
+        const size_t bytes = 4 * (size_t)B.cam.width * B.cam.height;
+        EOS_HIP(hipMemcpyAsync(out[i], b.rgba.as<uchar4>() + B.pix_off, bytes,
+                               o->output_memory == EOS_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, S));
+    }
+    EOS_HIP(hipStreamSynchronize(S));
+    if (st) {
+        float ms = 0.f;
+        EOS_HIP(hipEventElapsedTime(&ms, b.ev[0], b.ev[1]));
+        st->render_ms = ms;
+        st->rays = rs.primary_rays + rs.secondary_rays + rs.shadow_rays;
+    }
+    return EOS_OK;
 }
 
 }  // namespace eos

@@ -114,6 +114,12 @@ __device__ __forceinline__ int wave_append(int* ctr, bool pred) {
     return pred ? base + __popcll(m & lanemask_lt()) : -1;
 }
 
+// Adds the wave's sum of v to *ctr with one atomic.  Every lane must call it.
+__device__ __forceinline__ void wave_sum_add(int* ctr, int v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v != 0) atomicAdd(ctr, v);
+}
+
 // Counter slots (int32) of one frame.
 enum : int {
     CTR_Q = 0,         // + level: rays queued for level (level >= 1)
@@ -121,6 +127,7 @@ enum : int {
     CTR_CUR_C = 32,    // + level: fetch cursor of the closest-hit launch
     CTR_CUR_S = 48,    // + level: fetch cursor of the any-hit launch
     CTR_ERR = 64,      // stack overflow / invalid state
+    CTR_SHADOW = 65,   // shadow rays (sum over hits of the hit scene's light count)
     CTR_COUNT = 128
 };
 // int64 statistics (count_visits): [2*launch] visits, [2*launch+1] tests;
@@ -129,13 +136,42 @@ constexpr int kStats = 64;
 
 enum TraceMode : int { MODE_PRIMARY = 0, MODE_QUEUE = 1, MODE_SHADOW = 2 };
 
+// One scene of a batched frame (device table, one entry per scene).
+struct BatchScene {
+    SceneView sc;
+    DevCamera cam;
+    int tiles_x;
+    int id_off;      // first primary ray id of this scene
+    int pix_off;     // first pixel of this scene in the concatenated frame
+    int n_ids;
+};
+
+// Hit surfaces carry the material index and, in a batch, the scene index.
+constexpr int kSceneShift = 20;
+__device__ __forceinline__ int pack_mat(int mat, int scene) { return mat | (scene << kSceneShift); }
+__device__ __forceinline__ int unpack_mat(int v) { return v & ((1 << kSceneShift) - 1); }
+__device__ __forceinline__ int unpack_scene(int v) { return (int)((unsigned)v >> kSceneShift); }
+
+// Scene owning primary ray id r: the last entry with id_off <= r.
+__device__ __forceinline__ int scene_of_id(const BatchScene* t, int n, int r) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t[mid].id_off <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 struct TraceArgs {
     SceneView sc;
     DevCamera cam;
     int tiles_x;
+    const BatchScene* table;  // batched frames
+    int n_scenes;
     int n_static;            // item count when count_ptr is null
     const int* count_ptr;    // device item count (queued rays / hits)
-    int count_mul;           // items per counted entry (lights for shadow rays)
+    int count_mul;           // items per counted entry (light slots for shadow rays)
     int* cursor;             // persistent fetch cursor
     const float4* q;         // MODE_QUEUE: ray queue
     float4* hit_out;         // closest: (t, slot, u, v)
@@ -154,8 +190,9 @@ struct TraceArgs {
 // near-first descent, far child pushed onto a short stack in LDS (STACK_LDS
 // entries, column per lane) that overflows into a private global area, pruning
 // against the best hit so far.  Per-ray visit order is deterministic and equal
-// to the CPU restatement in the oracle.
-template <int MODE, int WG, int STACK_LDS, bool COUNT>
+// to the CPU restatement in the oracle.  BATCH: every ray looks up its own
+// scene (BVH, triangles, lights, camera) in the batch table.
+template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     constexpr bool ANY = MODE == MODE_SHADOW;
     __shared__ int32_t lds[STACK_LDS * WG];
@@ -174,7 +211,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     int best_slot = -1, best_orig = 0x7FFFFFFF;
     int cur = kDone, sp = 0;
     int budget = 0;  // node visits left for this ray: a tree visits each internal node at most once
-    const int max_visits = a.sc.n_tri + 64;
+    int max_visits = a.sc.n_tri + 64;
     unsigned long long visits = 0, tests = 0;
     const int thresh = a.refill < 1 ? 1 : (a.refill > 64 ? 64 : a.refill);
 
@@ -193,22 +230,58 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                 if (my < n) {
                     item = my;
                     bool valid = true;
+                    int root = a.sc.root;
                     if (MODE == MODE_PRIMARY) {
                         int x, y;
-                        valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
-                        o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-                        d = camera_dir(a.cam, x, y);
+                        if (BATCH) {
+                            const BatchScene& B = a.table[scene_of_id(a.table, a.n_scenes, my)];
+                            nodes = B.sc.nodes;
+                            geo = B.sc.geo;
+                            root = B.sc.root;
+                            max_visits = B.sc.n_tri + 64;
+                            valid = tile_pixel(my - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
+                            o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
+                            d = camera_dir(B.cam, x, y);
+                        } else {
+                            valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
+                            o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
+                            d = camera_dir(a.cam, x, y);
+                        }
                     } else if (MODE == MODE_QUEUE) {
                         const float4 q0 = a.q[2 * (size_t)my], q1 = a.q[2 * (size_t)my + 1];
                         o = f3(q0.x, q0.y, q0.z);
                         d = f3(q1.x, q1.y, q1.z);
+                        if (BATCH) {
+                            const BatchScene& B = a.table[__float_as_int(q1.w)];
+                            nodes = B.sc.nodes;
+                            geo = B.sc.geo;
+                            root = B.sc.root;
+                            max_visits = B.sc.n_tri + 64;
+                        }
                     } else {
-                        const int j = my / a.sc.n_light, li = my - j * a.sc.n_light;
+                        const int j = my / a.count_mul, li = my - j * a.count_mul;
                         const float4 s0 = a.surf[2 * (size_t)j];
                         const F3 P = f3(s0.x, s0.y, s0.z);
-                        const DevLight& L = a.sc.lights[li];
-                        d = normalize3(sub(f3(L.position[0], L.position[1], L.position[2]), P));
-                        o = add(P, scale(1e-3f, d));
+                        const DevLight* lights = a.sc.lights;
+                        int nl = a.sc.n_light;
+                        if (BATCH) {
+                            const BatchScene& B = a.table[unpack_scene(__float_as_int(a.surf[2 * (size_t)j + 1].w))];
+                            nodes = B.sc.nodes;
+                            geo = B.sc.geo;
+                            root = B.sc.root;
+                            max_visits = B.sc.n_tri + 64;
+                            lights = B.sc.lights;
+                            nl = B.sc.n_light;
+                        }
+                        if (li < nl) {
+                            const DevLight& L = lights[li];
+                            d = normalize3(sub(f3(L.position[0], L.position[1], L.position[2]), P));
+                            o = add(P, scale(1e-3f, d));
+                        } else {
+                            valid = false;  // light slot beyond this scene's lights
+                            d = f3(0.0f, 0.0f, 1.0f);
+                            o = P;
+                        }
                     }
                     ix = safe_inv(d.x);
                     iy = safe_inv(d.y);
@@ -222,7 +295,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     bu = bv = 0.0f;
                     sp = 0;
                     budget = max_visits;
-                    cur = valid ? a.sc.root : kDone;
+                    cur = valid ? root : kDone;
                 }
             }
         }

@@ -1,0 +1,91 @@
+"""HIP adjoint vs FP64 CPU references (hit sets held fixed).
+
+Bar: relative error <= 1e-3 per parameter, where the relative error of a
+parameter is |g - g_ref| / max(|g_ref|, 1e-3 * max_j |g_ref_j|) (the floor keeps
+parameters whose gradient is ~0 from dividing by zero).
+"""
+import numpy as np
+import pytest
+
+from eos_amd import scenes
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-3
+
+
+def _rel_err(g, ref):
+    scale = np.abs(ref).max()
+    return np.abs(g - ref) / np.maximum(np.abs(ref), 1e-3 * scale)
+
+
+def _setup(width=96, n_materials=16, n_lights=8):
+    from eos_amd import _lib as E
+    s = scenes.config_e(width, width, nu=100, nv=50, n_materials=n_materials, n_lights=n_lights)
+    g = E.Scene(s, leaf_max=4)
+    tm, tl = scenes.perturb_params(s, 1234)
+    g.set_materials(tm)
+    g.set_lights(tl)
+    target = g.render(want_rgb=True)["rgb"]
+    g.set_materials(s.materials)
+    g.set_lights(s.lights)
+    return s, g, target
+
+
+def test_adjoint_matches_fp64_forward_mode_and_central_differences():
+    s, g, target = _setup()
+    print(f"adjoint: {s.n_tri} tris {s.width}x{s.height} M={len(s.materials)} L={len(s.lights)}", flush=True)
+    fwd = g.render(want_rgb=True, want_hits=True, keep_records=True)
+    dl = 2.0 * (fwd["rgb"] - target)
+    gm, gl, st = g.adjoint(dl)
+    g_gpu = np.concatenate([gm.ravel(), gl.ravel()])
+    assert st["records"] > 0 and st["retrace_ms"] == 0.0          # replayed, not re-traced
+    orc = po.Oracle(s)
+    out = orc.render(want_recs=True)
+    assert (out["hit"] == fwd["hit"]).mean() >= 0.9999
+    prm = po.params_vector(s.materials, s.lights)
+    ref = po.grad_fwd64(out["recs"], s.camera["position"], len(s.materials), len(s.lights), prm, target)
+    err = _rel_err(g_gpu, ref)
+    print(f"  max rel err vs forward mode {err.max():.2e}", flush=True)
+    assert err.max() <= REL_TOL
+    rng = np.random.default_rng(5)
+    which = np.sort(rng.choice(len(prm), 64, replace=False))
+    fd = po.grad_fd64(out["recs"], s.camera["position"], len(s.materials), len(s.lights), prm, target, which,
+                      h=1e-3)
+    scale = np.abs(ref).max()
+    e2 = np.abs(g_gpu[which] - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
+    print(f"  max rel err vs central differences (64 params, h=1e-3) {e2.max():.2e}", flush=True)
+    assert e2.max() <= REL_TOL
+
+
+def test_adjoint_replay_equals_recompute_and_accumulates():
+    s, g, target = _setup(64, 8, 4)
+    fwd = g.render(want_rgb=True, keep_records=True)
+    dl = 2.0 * (fwd["rgb"] - target)
+    gm1, gl1, st1 = g.adjoint(dl)
+    gm2, gl2, st2 = g.adjoint(dl, recompute=True)
+    assert st2["retrace_ms"] > 0.0
+    assert np.allclose(gm1, gm2, rtol=1e-6, atol=1e-9) and np.allclose(gl1, gl2, rtol=1e-6, atol=1e-9)
+    # records are dropped when parameters change: the next adjoint re-traces
+    g.set_lights(s.lights)
+    gm3, gl3, st3 = g.adjoint(dl)
+    assert st3["retrace_ms"] > 0.0 and np.allclose(gm1, gm3, rtol=1e-6, atol=1e-9)
+
+
+def test_autograd_function():
+    import torch
+    from eos_amd import autograd as A
+    s, g, target = _setup(48, 6, 3)
+    params = [p.requires_grad_(True) for p in A.params_of(s)]
+    img = A.render(g, *params)
+    tgt = torch.tensor(target, device=img.device)
+    loss = ((img - tgt) ** 2).sum()
+    loss.backward()
+    got = np.concatenate([params[0].grad.cpu().numpy().ravel(), params[1].grad.cpu().numpy().ravel(),
+                          params[2].grad.cpu().numpy().ravel()])
+    gm, gl, _ = g.adjoint(2.0 * (img.detach().cpu().numpy() - target))
+    want = np.concatenate([gm[:, 0:3].ravel(), gm[:, 3:6].ravel(), gm[:, 6]])
+    assert np.allclose(got, want, rtol=1e-5, atol=1e-6)
+    assert np.allclose(params[3].grad.cpu().numpy(), gl[:, 0:3], rtol=1e-5, atol=1e-6)
+    assert np.allclose(params[4].grad.cpu().numpy(), gl[:, 3:6], rtol=1e-5, atol=1e-6)
