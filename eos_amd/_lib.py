@@ -15,10 +15,10 @@ LIB_PATH = os.path.join(_HERE, "libeos_amd.so")
 EOS_OK, EOS_ERR_INVALID, EOS_ERR_HIP, EOS_ERR_NOMEM, EOS_ERR_STATE, EOS_ERR_LIMIT, EOS_ERR_NODEVICE = range(7)
 EOS_MEM_HOST, EOS_MEM_DEVICE = 0, 1
 DBG = {"morton_sorted": 1, "perm": 2, "kchild": 3, "kparent": 4, "ibox": 5, "nodes": 6, "tris": 7,
-       "morton": 8}
+       "morton": 8, "nodes4": 9}
 
 # Every symbol include/eos_c.h declares.
-EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_update_materials",
+EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_build_lbvh_ex", "eos_update_materials",
            "eos_update_lights", "eos_render", "eos_render_adjoint", "eos_render_batch",
            "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
            "eos_abi_version", "eos_device_name"]
@@ -39,7 +39,11 @@ class SceneDesc(C.Structure):
 
 class BuildStats(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("num_internal", C.c_int32), ("root_ref", C.c_int32),
-                ("leaf_max", C.c_int32), ("pad", C.c_int32)]
+                ("leaf_max", C.c_int32), ("width", C.c_int32), ("num_nodes4", C.c_int32), ("root4", C.c_int32)]
+
+
+class BuildOpts(C.Structure):
+    _fields_ = [("leaf_max", C.c_int32), ("width", C.c_int32)]
 
 
 class RenderOpts(C.Structure):
@@ -96,6 +100,7 @@ def lib():
         L.eos_create_scene.argtypes = [vp, C.POINTER(vp)]
         L.eos_destroy_scene.argtypes = [vp]
         L.eos_build_lbvh.argtypes = [vp, i32, vp]
+        L.eos_build_lbvh_ex.argtypes = [vp, vp, vp]
         L.eos_update_materials.argtypes = [vp, vp, i32]
         L.eos_update_lights.argtypes = [vp, vp, i32]
         L.eos_render.argtypes = [vp, vp, vp, vp, vp]
@@ -132,7 +137,7 @@ def _ptr(a):
 class Scene:
     """One device scene (eos_scene*).  Arrays are copied to the device at creation."""
 
-    def __init__(self, scene, leaf_max: int = 4, build: bool = True):
+    def __init__(self, scene, leaf_max: int = 4, build: bool = True, width: int = 4):
         self.data = scene
         v = np.ascontiguousarray(scene.verts, np.float32)
         n = np.ascontiguousarray(scene.normals, np.float32)
@@ -146,15 +151,16 @@ class Scene:
         self.h = h
         self.build_stats = None
         if build:
-            self.build(leaf_max)
+            self.build(leaf_max, width)
 
     @property
     def handle(self):
         return self.h
 
-    def build(self, leaf_max: int = 4) -> dict:
+    def build(self, leaf_max: int = 4, width: int = 4) -> dict:
         st = BuildStats()
-        check(lib().eos_build_lbvh(self.h, leaf_max, C.byref(st)))
+        o = BuildOpts(leaf_max, width)
+        check(lib().eos_build_lbvh_ex(self.h, C.byref(o), C.byref(st)))
         self.build_stats = {k: getattr(st, k) for k, _ in st._fields_}
         return self.build_stats
 
@@ -235,7 +241,8 @@ class Scene:
         shapes = {"morton_sorted": ((n,), np.uint32), "perm": ((n,), np.int32),
                   "kchild": ((m, 2), np.int32), "kparent": ((max(2 * n - 1, 0),), np.int32),
                   "ibox": ((m, 6), np.float32), "nodes": ((m, 16), np.float32),
-                  "tris": ((n, 12), np.float32), "morton": ((n,), np.uint32)}
+                  "tris": ((n, 12), np.float32), "morton": ((n,), np.uint32),
+                  "nodes4": ((self.build_stats["num_nodes4"] if self.build_stats else 0, 32), np.float32)}
         shape, dt = shapes[what]
         out = np.zeros(shape, dt)
         check(lib().eos_debug_read(self.h, DBG[what], _ptr(out), out.nbytes))

@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libeos_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("EOS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["eos_api.cpp", "lbvh.hip", "render.hip", "adjoint.hip"]
+SOURCES = ["eos_api.cpp", "eos_shim.cpp", "lbvh.hip", "render.hip", "adjoint.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function", "-Wno-unused-result",
@@ -50,6 +50,13 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the C++ example that drives the reference-compatible API (include/eos/eos.h)
+    demo = os.path.join(HERE, "examples", "render_scene")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(HERE, "examples", "render_scene.cpp"), "-L", HERE, "-leos_amd",
+                        "-Wl,-rpath,$ORIGIN/..", "-o", demo], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"example build failed:\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -39,6 +39,9 @@ void DevBuf::release() {
 SceneView view_of(const eos_scene* s) {
     SceneView v;
     v.nodes = s->nodes.as<const Node>();
+    v.nodes4 = s->nodes4.as<const Node4>();
+    v.root4 = s->root4;
+    v.width = s->width;
     v.geo = s->geo.as<const float4>();
     v.shade = s->shade.as<const float4>();
     v.mats = s->mats.as<const DevMaterial>();
@@ -62,10 +65,11 @@ static void free_scene(eos_scene* s) {
     DevBuf* bufs[] = {&s->verts, &s->normals, &s->matidx, &s->mats, &s->lights, &s->nodes, &s->geo,
                       &s->shade, &s->codes, &s->codes_sorted, &s->perm, &s->kchild, &s->kparent,
                       &s->first_last, &s->ibox, &s->lbox, &s->flags, &s->cbounds, &s->sort_tmp_k,
-                      &s->sort_tmp_v, &s->sort_hist, &s->bounds_part, &s->fb.hit0, &s->fb.q[0], &s->fb.q[1],
+                      &s->sort_tmp_v, &s->sort_hist, &s->bounds_part, &s->nodes4, &s->flags4, &s->scan_tmp,
+                      &s->fb.hit0, &s->fb.q[0], &s->fb.q[1],
                       &s->fb.hitq, &s->fb.levcol, &s->fb.levels, &s->fb.counters, &s->fb.gstack};
     for (DevBuf* b : bufs) b->release();
-    for (int i = 0; i < kMaxStack; ++i) {
+    for (int i = 0; i <= EOS_MAX_DEPTH; ++i) {
         s->fb.surf[i].release();
         s->fb.vis[i].release();
     }
@@ -200,11 +204,17 @@ eos_status eos_update_lights(eos_scene* s, const eos_light* l, int32_t n) {
     return EOS_OK;
 }
 
-eos_status eos_build_lbvh(eos_scene* s, int32_t leaf_max, eos_build_stats* st) {
-    if (!s) return fail(EOS_ERR_INVALID, "eos_build_lbvh: null scene");
-    if (leaf_max < 1 || leaf_max > 8) return fail(EOS_ERR_INVALID, "eos_build_lbvh: leaf_max must be in [1, 8]");
+eos_status eos_build_lbvh_ex(eos_scene* s, const eos_build_opts* o, eos_build_stats* st) {
+    if (!s || !o) return fail(EOS_ERR_INVALID, "eos_build_lbvh_ex: null argument");
+    if (o->leaf_max < 1 || o->leaf_max > 8) return fail(EOS_ERR_INVALID, "eos_build_lbvh: leaf_max must be in [1, 8]");
+    if (o->width != 2 && o->width != 4) return fail(EOS_ERR_INVALID, "eos_build_lbvh: width must be 2 or 4");
     EOS_HIP(hipSetDevice(s->device));
-    return build_lbvh(s, leaf_max, st);
+    return build_lbvh(s, o->leaf_max, o->width, st);
+}
+
+eos_status eos_build_lbvh(eos_scene* s, int32_t leaf_max, eos_build_stats* st) {
+    eos_build_opts o{leaf_max, 4};
+    return eos_build_lbvh_ex(s, &o, st);
 }
 
 eos_status eos_render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
@@ -278,6 +288,7 @@ eos_status eos_debug_read(const eos_scene* s, int32_t what, void* dst, int64_t b
         case EOS_DBG_NODES: b = &s->nodes; need = 64 * m; break;
         case EOS_DBG_TRIS: b = &s->geo; need = 48 * n; break;
         case EOS_DBG_MORTON: b = &s->codes; need = 4 * n; break;
+        case EOS_DBG_NODES4: b = &s->nodes4; need = 128 * (size_t)s->n_nodes4; break;
         default: return fail(EOS_ERR_INVALID, "eos_debug_read: unknown array");
     }
     if (!s->built) return fail(EOS_ERR_STATE, "eos_debug_read: no LBVH built");

@@ -38,12 +38,25 @@ struct alignas(64) Node {
 };
 static_assert(sizeof(Node) == 64, "node is one 64-byte line");
 
+// 128-byte 4-wide node (two 64-byte lines), children in structure-of-arrays
+// form so one 16-byte load brings one bound of all four children:
+//   lox, hix, loy, hiy, loz, hiz (float4 each), ref (int4), pad.
+// Empty slots: ref kEmpty, bounds +inf (the slab test rejects them).
+struct alignas(64) Node4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 ref;
+    int4 pad;
+};
+static_assert(sizeof(Node4) == 128, "4-wide node is two 64-byte lines");
+constexpr int32_t kEmpty = 0x7FFFFFFE;
+
 // Triangle records in leaf (Morton) order, 48 bytes each, read as 3 x 16 B:
 //   geo:   v1.xyz e1.x | e1.yz e2.xy | e2.z orig(int) 0 0       (traversal)
 //   shade: n1.xyz mat  | n2.xyz 0    | n3.xyz 0                 (shading)
 constexpr int kLeafShift = 28;
 constexpr int32_t kDone = 0x7FFFFFFF;  // traversal sentinel
-constexpr int kMaxStack = 64;           // LBVH depth <= 64 (prefix length <= 63)
+constexpr int kMaxStack = 64;           // binary: LBVH depth <= 64 (prefix length <= 63)
+constexpr int kMaxStack4 = 96;          // 4-wide: <= 32 levels, <= 3 pushes each
 
 struct DevMaterial {
     float kd[4];
@@ -66,11 +79,14 @@ static_assert(sizeof(DevCamera) == sizeof(eos_camera), "camera layout");
 // Device view of one scene, passed by value to kernels.
 struct SceneView {
     const Node* nodes;
+    const Node4* nodes4;
     const float4* geo;
     const float4* shade;
     const DevMaterial* mats;
     const DevLight* lights;
     int32_t root;
+    int32_t root4;
+    int32_t width;            // 2 or 4: which tree the traversal walks
     int32_t n_tri;
     int32_t n_light;
     int32_t n_mat;
@@ -100,8 +116,8 @@ struct eos_frame_buffers {
     eos::DevBuf hit0;       // float4 [n_ids]      level-0 hit records (t, slot, u, v)
     eos::DevBuf q[2];       // float4 [cap][2]     ray queues (o.xyz pixel | d.xyz 0)
     eos::DevBuf hitq;       // float4 [cap]        hit records of queued rays
-    eos::DevBuf surf[eos::kMaxStack];  // per level: float4 [cap][2] (P.xyz pixel | N.xyz mat)
-    eos::DevBuf vis[eos::kMaxStack];   // per level: uint8 [cap * L]
+    eos::DevBuf surf[EOS_MAX_DEPTH + 1];  // per level: float4 [cap][2] (P.xyz pixel | N.xyz mat)
+    eos::DevBuf vis[EOS_MAX_DEPTH + 1];   // per level: uint8 [cap * L]
     eos::DevBuf levcol;     // float4 [(D+1) * npix]  level colour rgb + reflection weight
     eos::DevBuf levels;     // uint8  [npix]
     eos::DevBuf counters;   // int64  [64]
@@ -123,7 +139,11 @@ struct eos_scene {
     eos::DevBuf nodes, geo, shade;
     eos::DevBuf codes, codes_sorted, perm, kchild, kparent, first_last, ibox, lbox, flags, cbounds;
     eos::DevBuf sort_tmp_k, sort_tmp_v, sort_hist, bounds_part;
+    eos::DevBuf nodes4, flags4, scan_tmp;
     int root = 0;
+    int root4 = 0;
+    int n_nodes4 = 0;
+    int width = 4;
     int leaf_max = 1;
     bool built = false;
     eos::LaunchCfg cfg;
@@ -134,7 +154,7 @@ struct eos_scene {
 
 namespace eos {
 // Implemented in lbvh.hip
-eos_status build_lbvh(eos_scene* s, int leaf_max, eos_build_stats* st);
+eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st);
 // Implemented in render.hip
 eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
                   eos_render_stats* st);

@@ -13,6 +13,8 @@
 // bit-identical to the CPU restatement of the same algorithm.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "eos_internal.h"
 
 namespace eos {
@@ -345,16 +347,159 @@ __global__ __launch_bounds__(256) void k_tris(const float* __restrict__ verts, c
     shade[3 * (size_t)s + 2] = make_float4(nn[6], nn[7], nn[8], 0.0f);
 }
 
+// ---- exclusive scan of int32 (three phases: blocks of 1024, block sums, add) --
+constexpr int kScanBlock = 1024;
+
+__global__ __launch_bounds__(256) void k_scan_blocks(const int32_t* __restrict__ in, int32_t* __restrict__ out, int n,
+                                                     int32_t* __restrict__ sums) {
+    __shared__ int32_t s[256];
+    const int base = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    int32_t v[4], t = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        t += v[k];
+    }
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int32_t x = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int32_t run = s[threadIdx.x] - t;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) sums[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(int32_t* __restrict__ sums, int nb) {
+    __shared__ int32_t s[1024];
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += 1024) {
+        const int i = b0 + threadIdx.x;
+        const int32_t v = i < nb ? sums[i] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int32_t x = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+            __syncthreads();
+            s[threadIdx.x] += x;
+            __syncthreads();
+        }
+        if (i < nb) sums[i] = carry + s[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += s[1023];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int32_t* __restrict__ out, int n, const int32_t* __restrict__ sums) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] += sums[i / kScanBlock];
+}
+
+// ---- 4-wide collapse -----------------------------------------------------------
+// A 4-wide node stands for every binary node that stays internal (range larger
+// than leaf_max) at even depth; nodes are numbered in binary index order.
+__global__ __launch_bounds__(256) void k_flags4(const int32_t* __restrict__ kparent, const int2* __restrict__ fl, int n,
+                                                int leaf_max, int32_t* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int2 r = fl[i];
+    int f = 0;
+    if (r.y - r.x + 1 > leaf_max) {
+        int depth = 0;
+        for (int p = kparent[i]; p >= 0 && depth < 128; p = kparent[p]) ++depth;
+        f = (depth & 1) == 0;
+    }
+    flags[i] = f;
+}
+
+__global__ __launch_bounds__(256) void k_nodes4(const int2* __restrict__ kchild, const int2* __restrict__ fl,
+                                                const float* __restrict__ ibox, const float* __restrict__ lbox,
+                                                const float* __restrict__ cb, const int32_t* __restrict__ flags,
+                                                const int32_t* __restrict__ idx, int n, int leaf_max,
+                                                Node4* __restrict__ nodes4) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1 || !flags[i]) return;
+    float eps = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) eps = fmaxf(eps, fabsf(cb[k]));
+    eps = eps * 0x1p-22f;
+    int slots[4] = {0, 0, 0, 0};
+    int ns = 0;
+    const int2 c = kchild[i];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int ch = j ? c.y : c.x;
+        bool internal = false;
+        if (ch >= 0) {
+            const int2 r = fl[ch];
+            internal = r.y - r.x + 1 > leaf_max;
+        }
+        if (internal) {
+            const int2 g = kchild[ch];
+            slots[ns++] = g.x;
+            slots[ns++] = g.y;
+        } else {
+            slots[ns++] = ch;
+        }
+    }
+    float lo[3][4], hi[3][4];
+    int32_t ref[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ref[k] = kEmpty;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) lo[a][k] = hi[a][k] = INFINITY;
+        if (k < ns) {
+            const int g = slots[k];
+            const float* b = g >= 0 ? ibox + 6 * (size_t)g : lbox + 6 * (size_t)~g;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float pad = inflate_pad(b[a], b[3 + a], eps);
+                lo[a][k] = b[a] - pad;
+                hi[a][k] = b[3 + a] + pad;
+            }
+            if (g < 0) {
+                ref[k] = leaf_ref(~g, 1);
+            } else {
+                const int2 r = fl[g];
+                ref[k] = (r.y - r.x + 1 <= leaf_max) ? leaf_ref(r.x, r.y - r.x + 1) : idx[g];
+            }
+        }
+    }
+    Node4 nd;
+    nd.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    nd.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    nd.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    nd.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    nd.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    nd.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    nd.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+    nd.pad = make_int4(0, 0, 0, 0);
+    nodes4[idx[i]] = nd;
+}
+
 inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
 
-eos_status build_lbvh(eos_scene* s, int leaf_max, eos_build_stats* st) {
+eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st) {
     const int n = s->n_tri;
     hipStream_t S = s->stream;
     s->built = false;
     s->fb.rec_valid = 0;
     s->leaf_max = leaf_max;
+    s->width = width;
+    s->n_nodes4 = 0;
     const size_t un = (size_t)n, m = n > 0 ? un - 1 : 0;
     const int nblocks = n > 0 ? (int)blocks(un, kSortTile) : 1;
     const int nparts = n > 0 ? (int)std::min<size_t>(blocks(un, 256), 1024) : 1;
@@ -417,6 +562,29 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, eos_build_stats* st) {
         hipLaunchKernelGGL(k_tris, dim3(blocks(un, 256)), dim3(256), 0, S, V, s->normals.as<const float>(),
                            s->matidx.as<const int32_t>(), vA, n, s->n_mat, s->geo.as<float4>(), s->shade.as<float4>(),
                            err);
+        if (n > leaf_max) {
+            // 4-wide collapse: flags, their exclusive scan (node numbers), node count, nodes
+            const int nb = (int)blocks(m, kScanBlock);
+            EOS_TRY(s->flags4.ensure(8 * m + 8));
+            EOS_TRY(s->scan_tmp.ensure(4 * (size_t)nb + 4));
+            int32_t* flags = s->flags4.as<int32_t>();
+            int32_t* idx = flags + m;
+            hipLaunchKernelGGL(k_flags4, dim3(blocks(m, 256)), dim3(256), 0, S, s->kparent.as<const int32_t>(),
+                               s->first_last.as<const int2>(), n, leaf_max, flags);
+            hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, S, flags, idx, (int)m, s->scan_tmp.as<int32_t>());
+            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, S, s->scan_tmp.as<int32_t>(), nb);
+            hipLaunchKernelGGL(k_scan_add, dim3(blocks(m, 256)), dim3(256), 0, S, idx, (int)m,
+                               s->scan_tmp.as<const int32_t>());
+            int32_t tail[2] = {0, 0};
+            EOS_HIP(hipMemcpyAsync(&tail[0], flags + (m - 1), 4, hipMemcpyDeviceToHost, S));
+            EOS_HIP(hipMemcpyAsync(&tail[1], idx + (m - 1), 4, hipMemcpyDeviceToHost, S));
+            EOS_HIP(hipStreamSynchronize(S));
+            s->n_nodes4 = tail[0] + tail[1];
+            EOS_TRY(s->nodes4.ensure(128 * (size_t)std::max(s->n_nodes4, 1)));
+            hipLaunchKernelGGL(k_nodes4, dim3(blocks(m, 256)), dim3(256), 0, S, s->kchild.as<const int2>(),
+                               s->first_last.as<const int2>(), s->ibox.as<const float>(), s->lbox.as<const float>(),
+                               cb, flags, idx, n, leaf_max, s->nodes4.as<Node4>());
+        }
     }
     EOS_HIP(hipGetLastError());
     EOS_HIP(hipEventRecord(s->ev[1], S));
@@ -430,13 +598,16 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, eos_build_stats* st) {
     if (n == 0) s->root = kDone;
     else if (n <= leaf_max) s->root = ~(0 | ((n - 1) << kLeafShift));
     else s->root = 0;
+    s->root4 = s->root;
     s->built = true;
     if (st) {
         st->build_ms = ms;
         st->num_internal = n > 0 ? n - 1 : 0;
         st->root_ref = s->root;
         st->leaf_max = leaf_max;
-        st->pad = 0;
+        st->width = width;
+        st->num_nodes4 = s->n_nodes4;
+        st->root4 = s->root4;
     }
     return EOS_OK;
 }

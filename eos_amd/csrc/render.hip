@@ -232,14 +232,18 @@ __global__ void k_zero_i32(int* p, int n) {
 }
 
 // ---- traversal launch table ------------------------------------------------
+// 4-wide trees: every workgroup / LDS-stack configuration; binary trees: the
+// default configuration only (they exist for comparison and parity checks).
 template <int MODE, bool COUNT, bool BATCH>
-eos_status launch_trace(const LaunchCfg& cfg, int grid, hipStream_t s, const TraceArgs& a) {
-#define EOS_TRACE_CASE(WG, SL)                                                                        \
-    case WG * 100 + SL:                                                                                \
-        hipLaunchKernelGGL((k_trace<MODE, WG, SL, COUNT, BATCH>), dim3(grid), dim3(WG), 0, s, a);      \
+eos_status launch_trace(const LaunchCfg& cfg, int width, int grid, hipStream_t s, const TraceArgs& a) {
+#define EOS_TRACE_CASE(WG, SL)                                                                           \
+    case WG * 100 + SL:                                                                                   \
+        hipLaunchKernelGGL((k_trace<MODE, WG, SL, COUNT, BATCH, 4>), dim3(grid), dim3(WG), 0, s, a);      \
         break;
-    if (COUNT) {
-        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH>), dim3(grid), dim3(256), 0, s, a);
+    if (width == 2) {
+        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 2>), dim3(grid), dim3(256), 0, s, a);
+    } else if (COUNT) {
+        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 4>), dim3(grid), dim3(256), 0, s, a);
     } else {
         switch (cfg.workgroup * 100 + cfg.stack_lds) {
             EOS_TRACE_CASE(64, 8)
@@ -260,10 +264,11 @@ eos_status launch_trace(const LaunchCfg& cfg, int grid, hipStream_t s, const Tra
 }
 
 template <int MODE>
-eos_status launch_trace_any(bool count, bool batch, const LaunchCfg& cfg, int grid, hipStream_t s,
+eos_status launch_trace_any(bool count, bool batch, int width, const LaunchCfg& cfg, int grid, hipStream_t s,
                             const TraceArgs& a) {
-    if (batch) return launch_trace<MODE, false, true>(cfg, grid, s, a);
-    return count ? launch_trace<MODE, true, false>(cfg, grid, s, a) : launch_trace<MODE, false, false>(cfg, grid, s, a);
+    if (batch) return launch_trace<MODE, false, true>(cfg, width, grid, s, a);
+    return count ? launch_trace<MODE, true, false>(cfg, width, grid, s, a)
+                 : launch_trace<MODE, false, false>(cfg, width, grid, s, a);
 }
 
 inline unsigned blocks_for(size_t n, unsigned t, unsigned cap) {
@@ -283,6 +288,7 @@ struct Frame {
     size_t npix = 0, n_ids = 0;
     int depth = 0;
     int lstride = 0;       // light slots per hit (max lights over the batch)
+    int width = 4;         // tree width walked by the traversal
     bool count = false;
     LaunchCfg cfg;
     int num_cu = 256;
@@ -309,7 +315,8 @@ eos_status ensure_buffers(eos_frame_buffers& fb, const Frame& f, int trace_grid)
     EOS_TRY(fb.levcol.ensure(16 * f.npix * (size_t)(f.depth + 1)));
     EOS_TRY(fb.levels.ensure(f.npix));
     EOS_TRY(fb.counters.ensure(4 * CTR_COUNT + 8 * kStats));
-    EOS_TRY(fb.gstack.ensure((size_t)trace_grid * f.cfg.workgroup * (kMaxStack - f.cfg.stack_lds) * 4));
+    const int maxs = f.width == 4 ? kMaxStack4 : kMaxStack;
+    EOS_TRY(fb.gstack.ensure((size_t)trace_grid * f.cfg.workgroup * (maxs - f.cfg.stack_lds) * 4));
     fb.npix = f.npix;
     fb.n_ids = f.n_ids;
     fb.depth = f.depth;
@@ -359,8 +366,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, h
             ta.hit_out = fb.hitq.as<float4>();
         }
         F.c[2 * l] = rec();
-        if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(f.count, f.batch, f.cfg, trace_grid, S, ta));
-        else EOS_TRY(launch_trace_any<MODE_QUEUE>(f.count, f.batch, f.cfg, trace_grid, S, ta));
+        if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(f.count, f.batch, f.width, f.cfg, trace_grid, S, ta));
+        else EOS_TRY(launch_trace_any<MODE_QUEUE>(f.count, f.batch, f.width, f.cfg, trace_grid, S, ta));
         F.c[2 * l + 1] = rec();
         SetupArgs sa{};
         sa.sc = f.sc;
@@ -400,7 +407,7 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, h
             sh.count_mul = f.lstride;
             sh.surf = fb.surf[l].as<float4>();
             sh.vis_out = fb.vis[l].as<uint8_t>();
-            EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.cfg, trace_grid, S, sh));
+            EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.width, f.cfg, trace_grid, S, sh));
         }
         F.s[2 * l + 1] = rec();
         PhongArgs pa{};
@@ -494,7 +501,8 @@ eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count
     f.lstride = s->n_light;
     f.count = count;
     f.cfg = s->cfg;
-    if (count) {
+    f.width = s->width;
+    if (count || f.width == 2) {
         f.cfg.workgroup = 256;
         f.cfg.stack_lds = 16;
     }
@@ -609,6 +617,8 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
         b.host_table.assign(n, BatchScene{});
         size_t ids = 0, pix = 0;
         int lmax = 0;
+        for (int i = 0; i < n; ++i)
+            if (scenes[i]->width != s0->width) return fail(EOS_ERR_INVALID, "eos_render_batch: scenes built with different tree widths");
         for (int i = 0; i < n; ++i) {
             BatchScene& B = b.host_table[i];
             B.sc = view_of(scenes[i]);
@@ -635,6 +645,11 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
         f.depth = o->depth;
         f.lstride = lmax;
         f.cfg = cfg;
+        f.width = s0->width;
+        if (f.width == 2) {
+            f.cfg.workgroup = 256;
+            f.cfg.stack_lds = 16;
+        }
         f.num_cu = s0->num_cu;
         f.rgba = b.rgba.as<uchar4>();
         f.sc = view_of(s0);

@@ -192,15 +192,30 @@ struct TraceArgs {
 // against the best hit so far.  Per-ray visit order is deterministic and equal
 // to the CPU restatement in the oracle.  BATCH: every ray looks up its own
 // scene (BVH, triangles, lights, camera) in the batch table.
-template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH>
+// Compare-exchange on (entry distance, slot): a total order, so the result does
+// not depend on the network's shape.
+__device__ __forceinline__ void cmpswap(float& ta, int& ka, int& ra, float& tb, int& kb, int& rb) {
+    if (tb < ta || (tb == ta && kb < ka)) {
+        float t = ta; ta = tb; tb = t;
+        int k = ka; ka = kb; kb = k;
+        int r = ra; ra = rb; rb = r;
+    }
+}
+
+template <int WIDTH>
+constexpr int max_stack() { return WIDTH == 4 ? kMaxStack4 : kMaxStack; }
+
+template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH, int WIDTH>
 __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     constexpr bool ANY = MODE == MODE_SHADOW;
+    constexpr int MAXS = max_stack<WIDTH>();
     __shared__ int32_t lds[STACK_LDS * WG];
     const int tid = threadIdx.x;
     const int lane = __lane_id();
-    int32_t* gst = a.gstack + (size_t)(blockIdx.x * WG + tid) * (kMaxStack - STACK_LDS);
+    int32_t* gst = a.gstack + (size_t)(blockIdx.x * WG + tid) * (MAXS - STACK_LDS);
     const int n = a.count_ptr ? (*a.count_ptr) * a.count_mul : a.n_static;
     const Node* __restrict__ nodes = a.sc.nodes;
+    const Node4* __restrict__ nodes4 = a.sc.nodes4;
     const float4* __restrict__ geo = a.sc.geo;
 
     int item = -1;
@@ -230,14 +245,15 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                 if (my < n) {
                     item = my;
                     bool valid = true;
-                    int root = a.sc.root;
+                    int root = WIDTH == 4 ? a.sc.root4 : a.sc.root;
                     if (MODE == MODE_PRIMARY) {
                         int x, y;
                         if (BATCH) {
                             const BatchScene& B = a.table[scene_of_id(a.table, a.n_scenes, my)];
                             nodes = B.sc.nodes;
+                            nodes4 = B.sc.nodes4;
                             geo = B.sc.geo;
-                            root = B.sc.root;
+                            root = WIDTH == 4 ? B.sc.root4 : B.sc.root;
                             max_visits = B.sc.n_tri + 64;
                             valid = tile_pixel(my - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
                             o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
@@ -254,8 +270,9 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                         if (BATCH) {
                             const BatchScene& B = a.table[__float_as_int(q1.w)];
                             nodes = B.sc.nodes;
+                            nodes4 = B.sc.nodes4;
                             geo = B.sc.geo;
-                            root = B.sc.root;
+                            root = WIDTH == 4 ? B.sc.root4 : B.sc.root;
                             max_visits = B.sc.n_tri + 64;
                         }
                     } else {
@@ -267,8 +284,9 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                         if (BATCH) {
                             const BatchScene& B = a.table[unpack_scene(__float_as_int(a.surf[2 * (size_t)j + 1].w))];
                             nodes = B.sc.nodes;
+                            nodes4 = B.sc.nodes4;
                             geo = B.sc.geo;
-                            root = B.sc.root;
+                            root = WIDTH == 4 ? B.sc.root4 : B.sc.root;
                             max_visits = B.sc.n_tri + 64;
                             lights = B.sc.lights;
                             nl = B.sc.n_light;
@@ -311,31 +329,75 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     cur = kDone;
                 } else if (cur >= 0 && cur != kDone) {
                     if (COUNT) ++visits;
-                    const Node* nd = nodes + cur;
-                    const float4 b0 = nd->b0, b1 = nd->b1, b2 = nd->b2;
-                    const int4 c = nd->c;
                     const float tmax = ANY ? INFINITY : best_t;
-                    float t0, t1;
-                    const bool h0 = slab(b0.x, b0.y, b0.z, b0.w, b2.x, b2.y, ix, iy, iz, px, py, pz, tmax, t0);
-                    const bool h1 = slab(b1.x, b1.y, b1.z, b1.w, b2.z, b2.w, ix, iy, iz, px, py, pz, tmax, t1);
-                    if (h0 && h1) {
-                        const bool swap = t1 < t0;
-                        const int nearc = swap ? c.y : c.x, farc = swap ? c.x : c.y;
-                        if (sp >= kMaxStack) {
+                    if (WIDTH == 4) {
+                        const Node4* nd = nodes4 + cur;
+                        const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+                        const int4 rf = nd->ref;
+                        float t0, t1, t2, t3;
+                        const bool h0 = rf.x != kEmpty && slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ix, iy, iz, px, py, pz, tmax, t0);
+                        const bool h1 = rf.y != kEmpty && slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ix, iy, iz, px, py, pz, tmax, t1);
+                        const bool h2 = rf.z != kEmpty && slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ix, iy, iz, px, py, pz, tmax, t2);
+                        const bool h3 = rf.w != kEmpty && slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ix, iy, iz, px, py, pz, tmax, t3);
+                        const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+                        if (nh == 0) {
+                            cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
+                        } else if (sp + nh - 1 > MAXS) {
                             atomicOr(a.err, 1);
                             cur = kDone;
                         } else {
-                            if (sp < STACK_LDS) lds[sp * WG + tid] = farc;
-                            else gst[sp - STACK_LDS] = farc;
-                            ++sp;
-                            cur = nearc;
+                            float e0 = h0 ? t0 : INFINITY, e1 = h1 ? t1 : INFINITY, e2 = h2 ? t2 : INFINITY,
+                                  e3 = h3 ? t3 : INFINITY;
+                            int k0 = 0, k1 = 1, k2 = 2, k3 = 3;
+                            int r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+                            cmpswap(e0, k0, r0, e1, k1, r1);
+                            cmpswap(e2, k2, r2, e3, k3, r3);
+                            cmpswap(e0, k0, r0, e2, k2, r2);
+                            cmpswap(e1, k1, r1, e3, k3, r3);
+                            cmpswap(e1, k1, r1, e2, k2, r2);
+                            if (nh > 3) {
+                                if (sp < STACK_LDS) lds[sp * WG + tid] = r3;
+                                else gst[sp - STACK_LDS] = r3;
+                                ++sp;
+                            }
+                            if (nh > 2) {
+                                if (sp < STACK_LDS) lds[sp * WG + tid] = r2;
+                                else gst[sp - STACK_LDS] = r2;
+                                ++sp;
+                            }
+                            if (nh > 1) {
+                                if (sp < STACK_LDS) lds[sp * WG + tid] = r1;
+                                else gst[sp - STACK_LDS] = r1;
+                                ++sp;
+                            }
+                            cur = r0;
                         }
-                    } else if (h0) {
-                        cur = c.x;
-                    } else if (h1) {
-                        cur = c.y;
                     } else {
-                        cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
+                        const Node* nd = nodes + cur;
+                        const float4 b0 = nd->b0, b1 = nd->b1, b2 = nd->b2;
+                        const int4 c = nd->c;
+                        float t0, t1;
+                        const bool h0 = slab(b0.x, b0.y, b0.z, b0.w, b2.x, b2.y, ix, iy, iz, px, py, pz, tmax, t0);
+                        const bool h1 = slab(b1.x, b1.y, b1.z, b1.w, b2.z, b2.w, ix, iy, iz, px, py, pz, tmax, t1);
+                        if (h0 && h1) {
+                            const bool swap = t1 < t0;
+                            const int nearc = swap ? c.y : c.x, farc = swap ? c.x : c.y;
+                            if (sp >= MAXS) {
+                                atomicOr(a.err, 1);
+                                cur = kDone;
+                            } else {
+                                if (sp < STACK_LDS) lds[sp * WG + tid] = farc;
+                                else gst[sp - STACK_LDS] = farc;
+                                ++sp;
+                                cur = nearc;
+                            }
+                        } else if (h0) {
+                            cur = c.x;
+                        } else if (h1) {
+                            cur = c.y;
+                        } else {
+                            cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
+                        }
                     }
                 } else if (cur != kDone) {
                     const int e = ~cur;

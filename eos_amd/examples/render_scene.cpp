@@ -1,0 +1,114 @@
+// render_scene.cpp -- drives the GPU path through the reference-compatible C++
+// API exactly as the reference's driver does (src/main.cpp:355-481): build a
+// Scene from Triangle surfaces, materials and lights, construct a
+// MultiDeviceRenderer and render once to a host buffer; then the single-GPU
+// route (createCudaScene + DeviceCameraHandler + DeviceRenderer).
+//
+// usage: render_scene <scene.bin> <out_multi.rgba> <out_single.rgba>
+// scene.bin: int32 n_tri, n_mat, n_light, width, height, depth; float32
+// verts[n][9], normals[n][9]; int32 mat[n]; 48-byte materials; 32-byte lights;
+// float32 camera position, view, right, up (12 floats).
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "eos/eos.h"
+
+template <class T>
+static void rd(FILE* f, T* p, size_t n) {
+    if (std::fread(p, sizeof(T), n, f) != n) {
+        std::fprintf(stderr, "short read\n");
+        std::exit(2);
+    }
+}
+
+int main(int argc, char** argv) {
+    if (argc != 4) {
+        std::fprintf(stderr, "usage: %s scene.bin out_multi.rgba out_single.rgba\n", argv[0]);
+        return 2;
+    }
+    FILE* f = std::fopen(argv[1], "rb");
+    if (!f) return 2;
+    int32_t hdr[6];
+    rd(f, hdr, 6);
+    const int n = hdr[0], nm = hdr[1], nl = hdr[2], W = hdr[3], H = hdr[4], depth = hdr[5];
+    std::vector<float> v(9 * (size_t)n), nn(9 * (size_t)n), cam(12);
+    std::vector<int32_t> mat((size_t)n);
+    std::vector<eos_material> mats((size_t)nm);
+    std::vector<eos_light> lights((size_t)nl);
+    rd(f, v.data(), v.size());
+    rd(f, nn.data(), nn.size());
+    rd(f, mat.data(), mat.size());
+    rd(f, mats.data(), mats.size());
+    rd(f, lights.data(), lights.size());
+    rd(f, cam.data(), cam.size());
+    std::fclose(f);
+
+    Scene scene;
+    scene.setMaxTracedDepth(5);
+    scene.useBvh(true);
+    scene.setAASamples(1);
+    std::vector<int> matIndex;
+    for (const eos_material& m : mats) {
+        Material mm(0.0f, eos::vec4(m.kd[0], m.kd[1], m.kd[2], m.kd[3]), eos::vec4(m.ks[0], m.ks[1], m.ks[2], m.ks[3]),
+                    (int)m.shininess);
+        mm.setTransparent(false);
+        mm.setReflective(m.reflective != 0);
+        mm.setReflectionIntensity(m.reflectivity);
+        matIndex.push_back(scene.addMaterial(mm));
+    }
+    int ident = scene.addTransformation(eos::mat4(1.0f));
+    std::vector<Triangle*> tris;
+    for (int i = 0; i < n; ++i) {
+        const float* p = &v[9 * (size_t)i];
+        const float* q = &nn[9 * (size_t)i];
+        Triangle* t = new Triangle(eos::vec3(p[0], p[1], p[2]), eos::vec3(p[3], p[4], p[5]), eos::vec3(p[6], p[7], p[8]),
+                                   eos::vec3(q[0], q[1], q[2]), eos::vec3(q[3], q[4], q[5]), eos::vec3(q[6], q[7], q[8]));
+        t->setMaterialIndex(matIndex[mat[i]]);
+        t->setTransformationIndex(ident);
+        scene.addSurface(t);
+        tris.push_back(t);
+    }
+    std::vector<LightSource*> ls;
+    for (const eos_light& l : lights) {
+        ls.push_back(new LightSource(eos::vec4(l.position[0], l.position[1], l.position[2], l.position[3]),
+                                     eos::vec4(l.color[0], l.color[1], l.color[2], l.color[3])));
+        scene.addLightSource(ls.back());
+    }
+    scene.flush();
+
+    Camera camera;
+    camera.setPosition(eos::vec3(cam[0], cam[1], cam[2]));
+    camera.setViewingDirection(eos::vec3(cam[3], cam[4], cam[5]));
+    camera.setRightVector(eos::vec3(cam[6], cam[7], cam[8]));
+    camera.setUpVector(eos::vec3(cam[9], cam[10], cam[11]));
+    camera.setWidth(W);
+    camera.setHeight(H);
+
+    std::vector<unsigned char> image(4 * (size_t)W * H);
+    {
+        MultiDeviceRenderer renderer(&scene, &camera, W, H);
+        renderer.setTracedDepth(depth);
+        renderer.renderSceneToHostBuffer(image.data(), &camera);
+    }
+    FILE* o = std::fopen(argv[2], "wb");
+    std::fwrite(image.data(), 1, image.size(), o);
+    std::fclose(o);
+
+    cudaPreferL1Cache();
+    cudaScene_t* dscene = createCudaScene(&scene);
+    DeviceCameraHandler handler;
+    handler.setDeviceCamera(&camera);
+    DeviceRenderer dr(handler.getDeviceCamera(), W, H);
+    dr.setTracedDepth(depth);
+    std::vector<unsigned char> image2(4 * (size_t)W * H);
+    dr.renderCudaSceneToHostBufferMegaKernel(dscene, image2.data());
+    o = std::fopen(argv[3], "wb");
+    std::fwrite(image2.data(), 1, image2.size(), o);
+    std::fclose(o);
+    handler.freeDeviceCamera();
+    destroyCudaScene(dscene);
+    for (Triangle* t : tris) delete t;
+    for (LightSource* l : ls) delete l;
+    return 0;
+}

@@ -90,11 +90,18 @@ typedef struct eos_scene eos_scene;
 
 typedef struct eos_build_stats {
     double build_ms;          /* GPU time of the LBVH build (Morton, sort, Karras, refit, layout) */
-    int32_t num_internal;     /* internal nodes (N - 1) */
-    int32_t root_ref;         /* root child reference */
+    int32_t num_internal;     /* internal nodes of the binary tree (N - 1) */
+    int32_t root_ref;         /* root reference of the binary tree */
     int32_t leaf_max;         /* triangles per leaf after collapsing */
-    int32_t pad;
+    int32_t width;            /* tree the renderer walks: 2 (binary) or 4 (4-wide) */
+    int32_t num_nodes4;       /* 4-wide nodes */
+    int32_t root4;            /* root reference of the 4-wide tree */
 } eos_build_stats;
+
+typedef struct eos_build_opts {
+    int32_t leaf_max;         /* 1..8 triangles per leaf */
+    int32_t width;            /* 2: walk the binary LBVH; 4: collapse it two levels per node */
+} eos_build_opts;
 
 /* Render options.  Output pointers are optional (NULL = not wanted). */
 typedef struct eos_render_opts {
@@ -136,8 +143,10 @@ eos_status eos_destroy_scene(eos_scene* scene);
 /* Replaces Scene::flush's BVH build (src/Scene.cpp:204-209, src/BVH.cpp:97-111)
  * with a GPU Morton-code LBVH: radix sort + Karras hierarchy + bottom-up refit.
  * leaf_max (1..8) collapses subtrees of at most that many triangles into one
- * leaf.  Synchronous. */
+ * leaf; the renderer walks the 4-wide collapse of the tree.  Synchronous. */
 eos_status eos_build_lbvh(eos_scene* scene, int32_t leaf_max, eos_build_stats* stats);
+/* Same, choosing the tree width (2 = binary nodes, 4 = 4-wide nodes). */
+eos_status eos_build_lbvh_ex(eos_scene* scene, const eos_build_opts* opts, eos_build_stats* stats);
 
 /* Parameter updates without a rebuild (materials / lights only). */
 eos_status eos_update_materials(eos_scene* scene, const eos_material* materials, int32_t n);
@@ -219,7 +228,8 @@ enum {
     EOS_DBG_IBOX = 5,            /* float  [N-1][6] exact internal boxes */
     EOS_DBG_NODES = 6,           /* float  [N-1][16] 64-byte traversal nodes */
     EOS_DBG_TRIS = 7,            /* float  [N][12] sorted triangle records */
-    EOS_DBG_MORTON = 8           /* uint32 [N] unsorted codes */
+    EOS_DBG_MORTON = 8,          /* uint32 [N] unsorted codes */
+    EOS_DBG_NODES4 = 9           /* float  [num_nodes4][32] 128-byte 4-wide nodes */
 };
 eos_status eos_debug_read(const eos_scene* scene, int32_t what, void* host_dst, int64_t bytes);
 

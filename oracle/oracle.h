@@ -95,7 +95,11 @@ void orc_morton_codes(int32_t n, const float* verts, uint32_t* codes, float* cbo
  *   root_ref[1]. */
 void orc_lbvh_build(int32_t n, const float* verts, int32_t leaf_max,
                     uint32_t* sorted_codes, int32_t* perm, int32_t* kchild, int32_t* kparent,
-                    float* ibox, float* nodes, float* tris, int32_t* root_ref);
+                    float* ibox, float* nodes, float* tris, int32_t* root_ref,
+                    float* nodes4, int32_t* n_nodes4, int32_t* root4);
+/* nodes4: 128-byte 4-wide nodes [k][32]: lo.x[4] hi.x[4] lo.y[4] hi.y[4]
+ * lo.z[4] hi.z[4] ref[4] pad[4]; empty slots have ref 0x7FFFFFFE and boxes at
+ * +inf.  Pass nodes4 = NULL with n_nodes4 to get the count first. */
 /* Karras hierarchy on an already sorted code array. */
 void orc_karras(int32_t n, const uint32_t* sorted_codes, int32_t* kchild, int32_t* kparent,
                 int32_t* first, int32_t* last);
@@ -103,12 +107,13 @@ void orc_karras(int32_t n, const uint32_t* sorted_codes, int32_t* kchild, int32_
  * hit_orig[n_rays] (-1 miss), t/u/v may be NULL; visits/tests accumulate totals. */
 void orc_lbvh_trace(const float* nodes, int32_t root_ref, const float* tris, int32_t n_rays,
                     const float* orig, const float* dirs, int32_t any_hit, int32_t* hit_orig,
-                    float* hit_t, float* hit_u, float* hit_v, int64_t* visits, int64_t* tests);
+                    float* hit_t, float* hit_u, float* hit_v, int64_t* visits, int64_t* tests,
+                    int32_t width);
 /* Whitted render driven by the restated GPU traversal; fills rgba/rgb/hit_ids like
  * orc_render and counts node visits / triangle tests in counts. */
 int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, const float* tris,
                     const orc_camera* cam, int depth, int row_start, int row_step, uint8_t* rgba,
-                    float* rgb, int32_t* hit_ids, orc_counts* counts);
+                    float* rgb, int32_t* hit_ids, orc_counts* counts, int32_t width);
 
 /* ---- FP64 shading on fixed hit sets (gradient checks) ---- */
 /* Parameter vector layout: per material [kd r,g,b, ks r,g,b, shininess] (7*M), then

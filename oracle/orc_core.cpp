@@ -537,7 +537,10 @@ void orc_karras(int32_t n, const uint32_t* k, int32_t* kchild, int32_t* kparent,
 
 void orc_lbvh_build(int32_t n, const float* verts, int32_t leaf_max, uint32_t* sorted_codes,
                     int32_t* perm_out, int32_t* kchild_out, int32_t* kparent_out, float* ibox_out,
-                    float* nodes, float* tris, int32_t* root_ref) {
+                    float* nodes, float* tris, int32_t* root_ref, float* nodes4, int32_t* n_nodes4,
+                    int32_t* root4) {
+    if (n_nodes4) *n_nodes4 = 0;
+    if (root4) *root4 = n <= 0 ? 0x7FFFFFFF : (n <= std::max(1, std::min(leaf_max, 8)) ? leaf_ref(0, n) : 0);
     if (n <= 0) {
         if (root_ref) *root_ref = 0;
         return;
@@ -573,6 +576,7 @@ void orc_lbvh_build(int32_t n, const float* verts, int32_t leaf_max, uint32_t* s
     eps_scene *= 0x1p-22f;
     if (n == 1) {
         if (root_ref) *root_ref = leaf_ref(0, 1);
+        if (root4) *root4 = leaf_ref(0, 1);
         return;
     }
     std::vector<int32_t> kc(2 * (size_t)(n - 1)), kp(2 * (size_t)n - 1), fi(n - 1), la(n - 1);
@@ -637,6 +641,61 @@ void orc_lbvh_build(int32_t n, const float* verts, int32_t leaf_max, uint32_t* s
             nd[14] = 0.0f; nd[15] = 0.0f;
         }
     if (root_ref) *root_ref = (n <= leaf_max) ? leaf_ref(0, n) : 0;
+    if (!nodes4 && !n_nodes4) return;
+    // ---- 4-wide collapse: BVH4 nodes are the binary nodes that stay internal
+    // (range > leaf_max) at even depth, numbered in binary index order; a node's
+    // slots are its children, with an internal child replaced by its two children.
+    const int m = n - 1;
+    auto size_of = [&](int i) { return la[i] - fi[i] + 1; };
+    std::vector<int32_t> idx4(m, -1);
+    int cnt4 = 0;
+    for (int i = 0; i < m; ++i) {
+        if (size_of(i) <= leaf_max) continue;
+        int depth = 0;
+        for (int p = kp[i]; p >= 0; p = kp[p]) ++depth;
+        if ((depth & 1) == 0) idx4[i] = cnt4++;
+    }
+    if (n_nodes4) *n_nodes4 = cnt4;
+    if (!nodes4) return;
+    auto slot_ref = [&](int c) -> int32_t {
+        if (c < 0) return leaf_ref(~c, 1);
+        if (size_of(c) <= leaf_max) return leaf_ref(fi[c], size_of(c));
+        return idx4[c];
+    };
+    for (int i = 0; i < m; ++i) {
+        if (idx4[i] < 0) continue;
+        float* nd = nodes4 + 32 * (size_t)idx4[i];
+        int slots[4];
+        int ns = 0;
+        for (int c = 0; c < 2; ++c) {
+            int ch = kc[2 * i + c];
+            if (ch >= 0 && size_of(ch) > leaf_max) {
+                slots[ns++] = kc[2 * ch];
+                slots[ns++] = kc[2 * ch + 1];
+            } else {
+                slots[ns++] = ch;
+            }
+        }
+        for (int k = 0; k < 4; ++k) {
+            int32_t ref = 0x7FFFFFFE;   // empty slot
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
+            if (k < ns) {
+                int g = slots[k];
+                const float* b = g >= 0 ? &ib[6 * (size_t)g] : &lbox[6 * (size_t)~g];
+                for (int a = 0; a < 3; ++a) {
+                    float pad = inflate_pad(b[a], b[3 + a], eps_scene);
+                    lo[a] = b[a] - pad;
+                    hi[a] = b[3 + a] + pad;
+                }
+                ref = slot_ref(g);
+            }
+            nd[0 + k] = lo[0]; nd[4 + k] = hi[0];
+            nd[8 + k] = lo[1]; nd[12 + k] = hi[1];
+            nd[16 + k] = lo[2]; nd[20 + k] = hi[2];
+            std::memcpy(&nd[24 + k], &ref, 4);
+            nd[28 + k] = 0.0f;
+        }
+    }
 }
 
 }  // extern "C"
@@ -684,6 +743,7 @@ bool lbvh_trace_one(const float* nodes, int32_t root_ref, const float* tris, V3 
     int32_t stack[128];
     int sp = 0;
     int32_t cur = root_ref;
+    if (cur == 0x7FFFFFFF) return false;
     for (;;) {
         if (cur >= 0) {
             visits++;
@@ -730,21 +790,94 @@ bool lbvh_trace_one(const float* nodes, int32_t root_ref, const float* tris, V3 
     return h.orig >= 0;
 }
 
+// 4-wide: children hit sorted by (entry distance, slot); the nearest is taken,
+// the others pushed farthest first.
+template <bool ANY>
+bool lbvh4_trace_one(const float* nodes, int32_t root_ref, const float* tris, V3 o, V3 d, LHit& h,
+                     int64_t& visits, int64_t& tests) {
+    LRay r = make_lray(o, d);
+    int32_t stack[256];
+    int sp = 0;
+    int32_t cur = root_ref;
+    if (cur == 0x7FFFFFFF) return false;
+    for (;;) {
+        if (cur >= 0) {
+            visits++;
+            const float* nd = nodes + 32 * (size_t)cur;
+            float tmax = ANY ? INFINITY : h.t;
+            float tk[4];
+            int32_t rk[4];
+            int kk[4];
+            int nh = 0;
+            for (int k = 0; k < 4; ++k) {
+                int32_t ref;
+                std::memcpy(&ref, &nd[24 + k], 4);
+                float tm;
+                if (ref != 0x7FFFFFFE &&
+                    slab(r, nd[0 + k], nd[4 + k], nd[8 + k], nd[12 + k], nd[16 + k], nd[20 + k], tmax, tm)) {
+                    tk[nh] = tm; rk[nh] = ref; kk[nh] = k; ++nh;
+                }
+            }
+            // order by (t, slot)
+            for (int a = 1; a < nh; ++a)
+                for (int b = a; b > 0 && (tk[b] < tk[b - 1] || (tk[b] == tk[b - 1] && kk[b] < kk[b - 1])); --b) {
+                    std::swap(tk[b], tk[b - 1]); std::swap(rk[b], rk[b - 1]); std::swap(kk[b], kk[b - 1]);
+                }
+            if (nh == 0) {
+                if (sp == 0) break;
+                cur = stack[--sp];
+            } else {
+                for (int a = nh - 1; a >= 1; --a) stack[sp++] = rk[a];
+                cur = rk[0];
+            }
+        } else {
+            int32_t e = ~cur;
+            int32_t first = e & 0x0FFFFFFF, cnt = ((e >> 28) & 7) + 1;
+            bool stop = false;
+            for (int32_t s = first; s < first + cnt; ++s) {
+                tests++;
+                const float* tr = tris + 12 * (size_t)s;
+                V3 v1 = {tr[0], tr[1], tr[2]}, e1 = {tr[3], tr[4], tr[5]}, e2 = {tr[6], tr[7], tr[8]};
+                int32_t orig;
+                std::memcpy(&orig, &tr[9], 4);
+                float t, u, v;
+                if (mt_e(o, d, v1, e1, e2, t, u, v)) {
+                    if (ANY) { h.orig = orig; h.slot = s; h.t = t; h.u = u; h.v = v; stop = true; break; }
+                    if (t < h.t || (t == h.t && orig < h.orig)) {
+                        h.orig = orig; h.slot = s; h.t = t; h.u = u; h.v = v;
+                    }
+                }
+            }
+            if (stop) return true;
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+    }
+    return h.orig >= 0;
+}
+
+template <bool ANY>
+inline bool trace_w(int width, const float* nodes, int32_t root, const float* tris, V3 o, V3 d, LHit& h,
+                    int64_t& v, int64_t& t) {
+    return width == 4 ? lbvh4_trace_one<ANY>(nodes, root, tris, o, d, h, v, t)
+                      : lbvh_trace_one<ANY>(nodes, root, tris, o, d, h, v, t);
+}
+
 }  // namespace
 
 extern "C" {
 
 void orc_lbvh_trace(const float* nodes, int32_t root_ref, const float* tris, int32_t n_rays,
                     const float* orig, const float* dirs, int32_t any_hit, int32_t* hit_orig, float* hit_t,
-                    float* hit_u, float* hit_v, int64_t* visits, int64_t* tests) {
+                    float* hit_u, float* hit_v, int64_t* visits, int64_t* tests, int32_t width) {
     int64_t vs = 0, ts = 0;
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : vs, ts)
     for (int i = 0; i < n_rays; ++i) {
         LHit h;
         V3 o = ld3(orig + 3 * (size_t)i), d = ld3(dirs + 3 * (size_t)i);
         int64_t v = 0, t = 0;
-        if (any_hit) lbvh_trace_one<true>(nodes, root_ref, tris, o, d, h, v, t);
-        else lbvh_trace_one<false>(nodes, root_ref, tris, o, d, h, v, t);
+        if (any_hit) trace_w<true>(width, nodes, root_ref, tris, o, d, h, v, t);
+        else trace_w<false>(width, nodes, root_ref, tris, o, d, h, v, t);
         vs += v; ts += t;
         if (hit_orig) hit_orig[i] = h.orig;
         if (hit_t) hit_t[i] = h.t;
@@ -757,7 +890,7 @@ void orc_lbvh_trace(const float* nodes, int32_t root_ref, const float* tris, int
 
 int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, const float* tris,
                     const orc_camera* cam, int depth, int row_start, int row_step, uint8_t* rgba, float* rgb,
-                    int32_t* hit_ids, orc_counts* counts) {
+                    int32_t* hit_ids, orc_counts* counts, int32_t width) {
     const int W = cam->width, H = cam->height;
     if (row_step < 1) row_step = 1;
     int64_t prim = 0, shad = 0, sec = 0, hits = 0, vis = 0, tst = 0;
@@ -780,14 +913,14 @@ int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, co
                 [&](V3 o, V3 dd, Hit& h) {
                     if (empty) return;
                     LHit lh;
-                    if (lbvh_trace_one<false>(nodes, root_ref, tris, o, dd, lh, v, t)) {
+                    if (trace_w<false>(width, nodes, root_ref, tris, o, dd, lh, v, t)) {
                         h.tri = lh.orig; h.t = lh.t; h.u = lh.u; h.v = lh.v;
                     }
                 },
                 [&](V3 o, V3 dd) {
                     if (empty) return false;
                     LHit lh;
-                    return lbvh_trace_one<true>(nodes, root_ref, tris, o, dd, lh, v, t);
+                    return trace_w<true>(width, nodes, root_ref, tris, o, dd, lh, v, t);
                 },
                 col, hit_ids ? hit_ids + p : nullptr, nullptr, 0, c);
             if (rgb)

@@ -52,7 +52,7 @@ def lib():
         L.orc_accel_free.argtypes = [vp]
         L.orc_nearest.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp]
         L.orc_render.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
-        L.orc_render_lbvh.argtypes = [vp, vp, i32, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
+        L.orc_render_lbvh.argtypes = [vp, vp, i32, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, i32]
         L.orc_mt.argtypes = [vp, vp, vp, vp]
         L.orc_camera_ray.argtypes = [vp, C.c_int, C.c_int, vp]
         L.orc_quantize.restype = C.c_uint8
@@ -62,9 +62,9 @@ def lib():
         L.orc_shade_phong.restype = C.c_float
         L.orc_shade_phong.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
         L.orc_morton_codes.argtypes = [i32, vp, vp, vp]
-        L.orc_lbvh_build.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_lbvh_build.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_karras.argtypes = [i32, vp, vp, vp, vp, vp]
-        L.orc_lbvh_trace.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp]
+        L.orc_lbvh_trace.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, i32]
         L.orc_loss64.restype = C.c_double
         L.orc_loss64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, vp]
         L.orc_grad_fd64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, C.c_double, vp, i32, vp]
@@ -173,9 +173,16 @@ def lbvh_build(verts, leaf_max=1):
         "kchild": np.zeros(2 * m, np.int32), "kparent": np.zeros(2 * n - 1 if n else 1, np.int32),
         "ibox": np.zeros(6 * m, np.float32), "nodes": np.zeros(16 * m, np.float32),
         "tris": np.zeros(12 * max(n, 1), np.float32), "root": np.zeros(1, np.int32)}
+    n4 = np.zeros(1, np.int32)
+    r4 = np.zeros(1, np.int32)
+    lib().orc_lbvh_build(n, _p(v), leaf_max, None, None, None, None, None, None, None, None, None,
+                         _p(n4), None)
+    out["nodes4"] = np.zeros(32 * max(int(n4[0]), 1), np.float32)
     lib().orc_lbvh_build(n, _p(v), leaf_max, _p(out["codes"]), _p(out["perm"]), _p(out["kchild"]),
                          _p(out["kparent"]), _p(out["ibox"]), _p(out["nodes"]), _p(out["tris"]),
-                         _p(out["root"]))
+                         _p(out["root"]), _p(out["nodes4"]), _p(n4), _p(r4))
+    out["nodes4"] = out["nodes4"][:32 * int(n4[0])].reshape(-1, 32)
+    out["root4"] = int(r4[0])
     out["kchild"] = out["kchild"][:2 * (n - 1)].reshape(-1, 2)
     out["ibox"] = out["ibox"][:6 * (n - 1)].reshape(-1, 6)
     out["nodes"] = out["nodes"][:16 * (n - 1)].reshape(-1, 16)
@@ -184,7 +191,7 @@ def lbvh_build(verts, leaf_max=1):
     return out
 
 
-def lbvh_trace(b, origins, dirs, any_hit=False):
+def lbvh_trace(b, origins, dirs, any_hit=False, width=2):
     o = np.ascontiguousarray(origins, np.float32)
     d = np.ascontiguousarray(dirs, np.float32)
     n = o.shape[0]
@@ -194,14 +201,15 @@ def lbvh_trace(b, origins, dirs, any_hit=False):
     v = np.zeros(n, np.float32)
     vis = np.zeros(1, np.int64)
     tst = np.zeros(1, np.int64)
-    nodes = np.ascontiguousarray(b["nodes"], np.float32)
+    nodes = np.ascontiguousarray(b["nodes4"] if width == 4 else b["nodes"], np.float32)
     tris = np.ascontiguousarray(b["tris"], np.float32)
-    lib().orc_lbvh_trace(_p(nodes), b["root"], _p(tris), n, _p(o), _p(d), 1 if any_hit else 0,
-                         _p(hit), _p(t), _p(u), _p(v), _p(vis), _p(tst))
+    root = b["root4"] if width == 4 else b["root"]
+    lib().orc_lbvh_trace(_p(nodes), root, _p(tris), n, _p(o), _p(d), 1 if any_hit else 0,
+                         _p(hit), _p(t), _p(u), _p(v), _p(vis), _p(tst), width)
     return {"hit": hit, "t": t, "u": u, "v": v, "visits": int(vis[0]), "tests": int(tst[0])}
 
 
-def render_lbvh(scene, b, camera=None, depth=None, row_start=0, row_step=1):
+def render_lbvh(scene, b, camera=None, depth=None, row_start=0, row_step=1, width=2):
     o = Oracle.__new__(Oracle)
     verts = np.ascontiguousarray(scene.verts, np.float32)
     normals = np.ascontiguousarray(scene.normals, np.float32)
@@ -217,10 +225,11 @@ def render_lbvh(scene, b, camera=None, depth=None, row_start=0, row_step=1):
     rgb = np.zeros((H, W, 3), np.float32)
     hits = np.full((H, W), -1, np.int32)
     cnt = OrcCounts()
-    nodes = np.ascontiguousarray(b["nodes"], np.float32)
+    nodes = np.ascontiguousarray(b["nodes4"] if width == 4 else b["nodes"], np.float32)
     tris = np.ascontiguousarray(b["tris"], np.float32)
-    lib().orc_render_lbvh(C.byref(s), _p(nodes), b["root"], _p(tris), cam.ctypes.data, depth,
-                          row_start, row_step, _p(rgba), _p(rgb), _p(hits), C.byref(cnt))
+    root = b["root4"] if width == 4 else b["root"]
+    lib().orc_render_lbvh(C.byref(s), _p(nodes), root, _p(tris), cam.ctypes.data, depth,
+                          row_start, row_step, _p(rgba), _p(rgb), _p(hits), C.byref(cnt), width)
     del o
     return {"rgba": rgba, "rgb": rgb, "hit": hits, "counts": cnt.as_dict()}
 

@@ -19,7 +19,7 @@ def _declared():
 
 def test_every_declared_symbol_is_exported():
     names = _declared()
-    assert len(names) >= 14
+    assert len(names) >= 15
     assert sorted(E.EXPORTS) == names
     lib = E.lib()
     for n in names:
@@ -28,7 +28,8 @@ def test_every_declared_symbol_is_exported():
 
 def test_struct_layouts_match_header(tmp_path):
     """sizeof/offsetof from the C compiler equal the ctypes mirrors."""
-    structs = {"eos_scene_desc": E.SceneDesc, "eos_build_stats": E.BuildStats, "eos_render_opts": E.RenderOpts,
+    structs = {"eos_scene_desc": E.SceneDesc, "eos_build_stats": E.BuildStats, "eos_build_opts": E.BuildOpts,
+               "eos_render_opts": E.RenderOpts,
                "eos_render_stats": E.RenderStats, "eos_adjoint_opts": E.AdjointOpts,
                "eos_adjoint_stats": E.AdjointStats, "eos_batch_opts": E.BatchOpts,
                "eos_batch_stats": E.BatchStats, "eos_launch_config": E.LaunchConfig}
@@ -83,3 +84,11 @@ def test_scene_limits_checked_before_device_use():
     assert E.lib().eos_create_scene(C.byref(d), C.byref(h)) == E.EOS_ERR_LIMIT
     d = E.SceneDesc(0, None, None, None, 0, 0, None, 0, None)
     assert E.lib().eos_create_scene(C.byref(d), C.byref(h)) == E.EOS_ERR_INVALID
+
+
+def test_reference_api_symbols_exported():
+    """The extern "C" launchers and utilities of src/kernelWrapper.h / src/cudaWrapper.h."""
+    lib = E.lib()
+    for n in ["rayTrace_MegaKernel", "rayTrace_WarpShuffle_MegaKernel", "rayTrace_MultiMegaKernel",
+              "resetDevice", "setGLDevice", "gpuSynch", "cudaPreferL1Cache"]:
+        assert hasattr(lib, n), n

@@ -66,11 +66,22 @@ def test_adjoint_replay_equals_recompute_and_accumulates():
     gm1, gl1, st1 = g.adjoint(dl)
     gm2, gl2, st2 = g.adjoint(dl, recompute=True)
     assert st2["retrace_ms"] > 0.0
-    assert np.allclose(gm1, gm2, rtol=1e-6, atol=1e-9) and np.allclose(gl1, gl2, rtol=1e-6, atol=1e-9)
+    # same records; only the order of the FP32 workgroup sums (LDS atomics) differs
+    close = lambda a, b: np.allclose(a, b, rtol=1e-5, atol=1e-6 * np.abs(b).max())  # noqa: E731
+    assert close(gm1, gm2) and close(gl1, gl2)
     # records are dropped when parameters change: the next adjoint re-traces
     g.set_lights(s.lights)
     gm3, gl3, st3 = g.adjoint(dl)
-    assert st3["retrace_ms"] > 0.0 and np.allclose(gm1, gm3, rtol=1e-6, atol=1e-9)
+    assert st3["retrace_ms"] > 0.0 and close(gm1, gm3) and close(gl1, gl3)
+    # gradients accumulate into the caller's arrays (device path)
+    import torch
+    gm_d = torch.zeros((len(s.materials), 7), dtype=torch.float64, device="cuda:0")
+    gl_d = torch.zeros((len(s.lights), 6), dtype=torch.float64, device="cuda:0")
+    dl_d = torch.tensor(dl, device="cuda:0")
+    torch.cuda.synchronize()
+    for _ in range(2):
+        g.adjoint_device(s.camera, s.depth, dl_d.data_ptr(), gm_d.data_ptr(), gl_d.data_ptr())
+    assert close(gm_d.cpu().numpy(), 2 * gm1) and close(gl_d.cpu().numpy(), 2 * gl1)
 
 
 def test_autograd_function():

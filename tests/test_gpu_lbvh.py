@@ -50,6 +50,10 @@ def test_lbvh_bit_exact(name, leaf_max):
     assert np.array_equal(g.debug_read("perm"), ref["perm"])
     assert g.build_stats["root_ref"] == ref["root"]
     assert np.array_equal(_u32(g.debug_read("tris"))[:, :10], _u32(ref["tris"])[:, :10])
+    assert g.build_stats["num_nodes4"] == len(ref["nodes4"])
+    assert g.build_stats["root4"] == ref["root4"]
+    if len(ref["nodes4"]):
+        assert np.array_equal(_u32(g.debug_read("nodes4"))[:, :28], _u32(ref["nodes4"])[:, :28])
     if s.n_tri > 1:
         assert np.array_equal(g.debug_read("kchild"), ref["kchild"])
         assert np.array_equal(g.debug_read("kparent"), ref["kparent"])

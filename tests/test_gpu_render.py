@@ -46,13 +46,14 @@ RENDER_CASES = {
 }
 
 
+@pytest.mark.parametrize("width", [4, 2])
 @pytest.mark.parametrize("name", sorted(RENDER_CASES))
-def test_render_parity(name):
+def test_render_parity(name, width):
     from eos_amd import _lib as E
     s = RENDER_CASES[name]()
-    print(f"render case={name} n={s.n_tri} {s.width}x{s.height} depth={s.depth} lights={len(s.lights)}",
-          flush=True)
-    g = E.Scene(s, leaf_max=4)
+    print(f"render case={name} n={s.n_tri} {s.width}x{s.height} depth={s.depth} lights={len(s.lights)} "
+          f"width={width}", flush=True)
+    g = E.Scene(s, leaf_max=4, width=width)
     go = g.render(want_rgb=True, want_hits=True)
     oo = po.Oracle(s).render()
     agree, d = _compare(s, go, oo, name)
@@ -65,15 +66,16 @@ def test_render_parity(name):
     assert st["hits"] == oc["hits"]
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("leaf_max", [1, 4])
-def test_visit_counts_match_restatement(leaf_max):
+def test_visit_counts_match_restatement(leaf_max, width):
     from eos_amd import _lib as E
     s = scenes.config_c(96, 64, nu=96, nv=48)
-    print(f"visit counts leaf_max={leaf_max}", flush=True)
-    g = E.Scene(s, leaf_max=leaf_max)
+    print(f"visit counts leaf_max={leaf_max} width={width}", flush=True)
+    g = E.Scene(s, leaf_max=leaf_max, width=width)
     go = g.render(count_visits=True, want_hits=True)
     b = po.lbvh_build(s.verts, leaf_max)
-    oo = po.render_lbvh(s, b)
+    oo = po.render_lbvh(s, b, width=width)
     assert np.array_equal(go["hit"], oo["hit"])
     assert np.array_equal(go["rgba"], oo["rgba"]) or np.abs(go["rgba"].astype(int) - oo["rgba"]).max() <= 1
     assert go["stats"]["node_visits"] == oo["counts"]["node_visits"]

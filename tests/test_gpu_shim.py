@@ -1,0 +1,47 @@
+"""The reference-compatible C++ API (include/eos/eos.h) driven by a C++ program
+the way the reference's driver does (src/main.cpp:355-481)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from eos_amd import scenes
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DEMO = os.path.join(ROOT, "eos_amd", "examples", "render_scene")
+
+
+def _write_scene(path, s):
+    with open(path, "wb") as f:
+        np.array([s.n_tri, len(s.materials), len(s.lights), s.width, s.height, s.depth], np.int32).tofile(f)
+        np.ascontiguousarray(s.verts, np.float32).tofile(f)
+        np.ascontiguousarray(s.normals, np.float32).tofile(f)
+        np.ascontiguousarray(s.mat, np.int32).tofile(f)
+        np.ascontiguousarray(s.materials).tofile(f)
+        np.ascontiguousarray(s.lights).tofile(f)
+        np.concatenate([s.camera["position"], s.camera["view"], s.camera["right"],
+                        s.camera["up"]]).astype(np.float32).tofile(f)
+
+
+@pytest.mark.parametrize("name", ["config_a", "soup"])
+def test_cpp_api_renders_like_the_c_abi(tmp_path, name):
+    from eos_amd import _lib as E
+    s = scenes.config_a(80, 60) if name == "config_a" else scenes.config_b(72, 48, n=8000)
+    sc = tmp_path / "scene.bin"
+    _write_scene(sc, s)
+    o1, o2 = tmp_path / "multi.rgba", tmp_path / "single.rgba"
+    print(f"running {DEMO} on {name}", flush=True)
+    r = subprocess.run([DEMO, str(sc), str(o1), str(o2)], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, EOS_QUIET="1"))
+    assert r.returncode == 0, r.stderr
+    multi = np.fromfile(o1, np.uint8).reshape(s.height, s.width, 4)
+    single = np.fromfile(o2, np.uint8).reshape(s.height, s.width, 4)
+    ref = E.Scene(s, leaf_max=4).render()["rgba"]
+    assert np.array_equal(multi, ref)
+    assert np.array_equal(single, ref)
+    orc = po.Oracle(s).render()["rgba"]
+    assert (np.abs(multi.astype(int) - orc.astype(int)).max(-1) <= 1).mean() >= 0.9999

@@ -356,13 +356,21 @@ def test_lbvh_traversal_restatement_is_exact():
     d = rng.normal(size=(4000, 3))
     d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
     ref = bf.nearest(o, d)
+    occ = bf.nearest(o, d, any_hit=True)
     for leaf_max in (1, 4, 8):
         b = po.lbvh_build(s.verts, leaf_max)
-        r = po.lbvh_trace(b, o, d)
-        assert np.array_equal(r["hit"], ref["tri"])
-        a = po.lbvh_trace(b, o, d, any_hit=True)
-        assert np.array_equal(a["hit"] >= 0, bf.nearest(o, d, any_hit=True))
-        assert r["visits"] > 0 and r["tests"] > 0
+        visits = {}
+        for width in (2, 4):
+            r = po.lbvh_trace(b, o, d, width=width)
+            assert np.array_equal(r["hit"], ref["tri"])
+            a = po.lbvh_trace(b, o, d, any_hit=True, width=width)
+            assert np.array_equal(a["hit"] >= 0, occ)
+            assert r["visits"] > 0 and r["tests"] > 0
+            visits[width] = r["visits"]
+        assert visits[4] < visits[2]
+        # every 4-wide node's slots hold children or grandchildren of one binary node
+        n4 = b["nodes4"].view(np.int32)[:, 24:28]
+        assert ((n4 != 0x7FFFFFFE).sum(1) >= 2).all()
 
 
 def test_counts_definition():
