@@ -11,8 +11,8 @@ value = (primary + shadow + secondary rays per frame) * steps / wall time.
 
 Also reported:
   roofline      dominant kernel (the primary-ray closest-hit launch): algorithmic
-                bytes per launch = 64 B per node visit + 36 B per triangle test +
-                16 B hit record per ray, from the counting variant of the same
+                bytes per launch = 128 B per 4-wide node visit (64 B per binary
+                node visit) + 36 B per triangle test + 16 B hit record per ray, from the counting variant of the same
                 traversal (deterministic, so equal to the timed launches), divided
                 by the launch's mean duration from HIP events on its stream.
   cpu_baseline  the timing-faithful port of the reference's OpenMP CPU path on a
@@ -30,7 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, HIT_BYTES = 64, 36, 16
+NODE_BYTES = {2: 64, 4: 128}   # binary node / 4-wide node
+TRI_BYTES, HIT_BYTES = 36, 16
 
 
 def _traffic_from_profile(kernel: str, workload: str):
@@ -82,14 +83,139 @@ def _progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def bench_batched(E, torch, reps: int):
+    """64 soups (config D): sum of sequential eos_render frames vs one batched,
+    graph-replayed frame.  Both write RGBA8 to device memory; wall clock with the
+    device synchronised around each measurement."""
+    from eos_amd import autotune, scenes
+    t0 = time.perf_counter()
+    data = scenes.config_d()
+    sc = [E.Scene(d, leaf_max=4) for d in data]
+    _progress(f"batch: {len(sc)} scenes, {sum(d.n_tri for d in data)} triangles built in "
+              f"{time.perf_counter() - t0:.1f} s")
+    tuned = autotune.load()
+    source = "eos_amd/tuned_launch.json"
+    if tuned is None:
+        res = autotune.tune(sc, depth=2, reps=2, path=None, workload="config D",
+                            grid={"workgroup": (128, 256), "stack_lds": (8, 16), "waves_per_cu": (16, 32),
+                                  "refill": (16,)})
+        tuned, source = res["config"], "tuned in this run"
+    for s in sc:
+        s.set_launch_config(**tuned)
+    outs = [torch.empty(d.width * d.height * 4, dtype=torch.uint8, device="cuda:0") for d in data]
+    ptrs = [o.data_ptr() for o in outs]
+
+    def sequential():
+        rays = 0
+        for s, d, p in zip(sc, data, ptrs):
+            st = s.render_device(d.camera, 2, p)
+            rays += st["primary_rays"] + st["shadow_rays"] + st["secondary_rays"]
+        return rays
+
+    rays = sequential()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        sequential()
+    torch.cuda.synchronize()
+    t_seq = (time.perf_counter() - t) / reps
+    _, st = E.render_batch(sc, depth=2, use_graph=True, device_ptrs=ptrs)
+    assert st["rays"] == rays, (st["rays"], rays)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        E.render_batch(sc, depth=2, use_graph=True, device_ptrs=ptrs)
+    torch.cuda.synchronize()
+    t_bat = (time.perf_counter() - t) / reps
+    return {"scenes": len(sc), "rays_per_frame": int(rays), "sequential_ms": round(t_seq * 1e3, 3),
+            "batched_ms": round(t_bat * 1e3, 3), "sequential_mrays": round(rays / t_seq / 1e6, 2),
+            "batched_mrays": round(rays / t_bat / 1e6, 2), "speedup": round(t_seq / t_bat, 3),
+            "launch_config": tuned, "launch_config_source": source, "hipgraph": True}
+
+
+def bench_adjoint(E, torch, reps: int, check: bool):
+    """Config E: forward frame vs forward + hand-written adjoint (records replayed)."""
+    import numpy as np
+    from eos_amd import scenes
+    s = scenes.config_e()
+    g = E.Scene(s, leaf_max=4)
+    W, H, depth = s.width, s.height, s.depth
+    tm, tl = scenes.perturb_params(s, 1234)
+    g.set_materials(tm)
+    g.set_lights(tl)
+    tgt = torch.empty((H, W, 3), dtype=torch.float32, device="cuda:0")
+    g.render_device(s.camera, depth, None, rgb_dev_ptr=tgt.data_ptr())
+    g.set_materials(s.materials)
+    g.set_lights(s.lights)
+    rgb = torch.empty_like(tgt)
+    M, L = len(s.materials), len(s.lights)
+    gm = torch.zeros((M, 7), dtype=torch.float64, device="cuda:0")
+    gl = torch.zeros((L, 6), dtype=torch.float64, device="cuda:0")
+
+    def fwd():
+        return g.render_device(s.camera, depth, None, rgb_dev_ptr=rgb.data_ptr())
+
+    def fwd_adj():
+        st = g.render_device(s.camera, depth, None, rgb_dev_ptr=rgb.data_ptr(), keep_records=True)
+        dl = 2.0 * (rgb - tgt)
+        gm.zero_()
+        gl.zero_()
+        torch.cuda.synchronize()
+        ad = g.adjoint_device(s.camera, depth, dl.data_ptr(), gm.data_ptr(), gl.data_ptr())
+        return st, ad
+
+    fwd()
+    fwd_adj()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        st = fwd()
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t) / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        _, ad = fwd_adj()
+    torch.cuda.synchronize()
+    t_fa = (time.perf_counter() - t) / reps
+    out = {"triangles": s.n_tri, "pixels": W * H, "materials": M, "lights": L, "depth": depth,
+           "params": 7 * M + 6 * L, "forward_ms": round(t_f * 1e3, 3), "forward_adjoint_ms": round(t_fa * 1e3, 3),
+           "ratio": round(t_fa / t_f, 3), "adjoint_kernel_ms": round(ad["adjoint_ms"], 4),
+           "records": int(ad["records"])}
+    if check:
+        # gradient over every 16th row vs the FP64 forward-mode oracle on the same rows
+        from oracle import pyoracle as po
+        step = 16
+        mask = torch.zeros((H, 1, 1), device="cuda:0")
+        mask[::step] = 1.0
+        g.render_device(s.camera, depth, None, rgb_dev_ptr=rgb.data_ptr(), keep_records=True)
+        dl = 2.0 * (rgb - tgt) * mask
+        gm.zero_()
+        gl.zero_()
+        torch.cuda.synchronize()
+        g.adjoint_device(s.camera, depth, dl.data_ptr(), gm.data_ptr(), gl.data_ptr())
+        g_gpu = np.concatenate([gm.cpu().numpy().ravel(), gl.cpu().numpy().ravel()])
+        orc = po.Oracle(s)
+        rr = orc.render(row_start=0, row_step=step, want_recs=True)
+        ref = po.grad_fwd64(rr["recs"], s.camera["position"], M, L, po.params_vector(s.materials, s.lights),
+                            tgt.cpu().numpy())
+        scale = np.abs(ref).max()
+        err = np.abs(g_gpu - ref) / np.maximum(np.abs(ref), 1e-3 * scale)
+        out["grad_check"] = {"rows": f"every {step}th row", "reference": "FP64 forward mode (oracle)",
+                             "max_rel_err": float(err.max()), "median_rel_err": float(np.median(err))}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--leaf-max", type=int, default=4)
+    ap.add_argument("--width", type=int, default=4, choices=(2, 4), help="BVH width the renderer walks")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-stride", type=int, default=32)
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="before the frames, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for calibrating rocprofv3 byte counters)")
@@ -111,7 +237,7 @@ def main() -> int:
     _progress(f"scene generated in {gen_s:.1f} s ({scene.n_tri} triangles)")
     dev = torch.device("cuda:0")
     torch.cuda.init()
-    g = E.Scene(scene, leaf_max=args.leaf_max)
+    g = E.Scene(scene, leaf_max=args.leaf_max, width=args.width)
     build = g.build_stats
     W, H, depth = scene.width, scene.height, scene.depth
     out = torch.empty(W * H * 4, dtype=torch.uint8, device=dev)
@@ -146,8 +272,8 @@ def main() -> int:
     mean = lambda k: float(np.mean([p[k] for p in per]))  # noqa: E731
     t_first = mean("trace_first_ms")
     n_ids = ((W + 7) // 8) * ((H + 7) // 8) * 64
-    alg_bytes = (counted["closest_visits_first"] * NODE_BYTES + counted["closest_tests_first"] * TRI_BYTES
-                 + n_ids * HIT_BYTES)
+    alg_bytes = (counted["closest_visits_first"] * NODE_BYTES[build["width"]]
+                 + counted["closest_tests_first"] * TRI_BYTES + n_ids * HIT_BYTES)
     achieved = alg_bytes / (t_first * 1e-3) / 1e9
     kernel = "k_trace<MODE_PRIMARY>"
     traffic = _traffic_from_profile(kernel, "config_c")
@@ -167,7 +293,8 @@ def main() -> int:
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": "config C: displaced sphere 4,194,304 tris, 3840x2160, 4 lights, depth 3",
                    "triangles": scene.n_tri, "width": W, "height": H, "lights": int(len(scene.lights)),
-                   "depth": depth, "leaf_max": args.leaf_max, "parallelism": "single GPU full frame"},
+                   "depth": depth, "leaf_max": args.leaf_max, "bvh_width": build["width"],
+                   "parallelism": "single GPU full frame"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kernel, "launch_ms": round(t_first, 4),
@@ -186,6 +313,12 @@ def main() -> int:
         result["device"] = E.device_name()
     except E.EosError:
         pass
+    if not args.skip_batch:
+        result["batched"] = bench_batched(E, torch, reps=3)
+        _progress(f"batched: {result['batched']['speedup']}x over sequential")
+    if not args.skip_adjoint:
+        result["adjoint"] = bench_adjoint(E, torch, reps=3, check=not args.no_cpu)
+        _progress(f"adjoint: fwd+adj / fwd = {result['adjoint']['ratio']}")
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         _progress(f"CPU baseline (reference port, {threads} threads)")

@@ -95,6 +95,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python eos_amd/build.py` "
                                "(there is no CPU fallback)")
+        # torch ships its own libamdhip64 (same soname).  Loaded after this
+        # library, it would be a second HIP runtime in the process, and that one
+        # sees no GPU; loaded first, this library binds to it and the process has
+        # one runtime, so device pointers pass freely between the two.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, i32 = C.c_void_p, C.c_int32
         L.eos_create_scene.argtypes = [vp, C.POINTER(vp)]
@@ -249,12 +257,20 @@ class Scene:
         return out
 
 
-def render_batch(scenes, cameras=None, depth=2, use_graph=False):
+def render_batch(scenes, cameras=None, depth=2, use_graph=False, device_ptrs=None):
+    """Renders all scenes in one batched frame.  Outputs go to host arrays (returned),
+    or to the device buffers in ``device_ptrs`` (one RGBA8 image per scene)."""
     cams = np.ascontiguousarray(np.stack([s.data.camera for s in scenes]) if cameras is None else cameras)
-    outs = [np.zeros((int(c["height"]), int(c["width"]), 4), np.uint8) for c in cams]
     hs = (C.c_void_p * len(scenes))(*[s.h.value for s in scenes])
-    ptrs = (C.c_void_p * len(scenes))(*[o.ctypes.data for o in outs])
-    o = BatchOpts(depth, EOS_MEM_HOST, int(use_graph), 0)
+    if device_ptrs is None:
+        outs = [np.zeros((int(c["height"]), int(c["width"]), 4), np.uint8) for c in cams]
+        ptrs = (C.c_void_p * len(scenes))(*[o.ctypes.data for o in outs])
+        mem = EOS_MEM_HOST
+    else:
+        outs = None
+        ptrs = (C.c_void_p * len(scenes))(*device_ptrs)
+        mem = EOS_MEM_DEVICE
+    o = BatchOpts(depth, mem, int(use_graph), 0)
     st = BatchStats()
     check(lib().eos_render_batch(hs, cams.ctypes.data, len(scenes), C.byref(o), ptrs, C.byref(st)))
     return outs, {k: getattr(st, k) for k, _ in st._fields_}
