@@ -97,8 +97,8 @@ def bench_batched(E, torch, reps: int):
     source = "eos_amd/tuned_launch.json"
     if tuned is None:
         res = autotune.tune(sc, depth=2, reps=2, path=None, workload="config D",
-                            grid={"workgroup": (128, 256), "stack_lds": (8, 16), "waves_per_cu": (16, 32),
-                                  "refill": (16,)})
+                            grid={"workgroup": (256,), "stack_lds": (16,), "waves_per_cu": (16, 20, 32),
+                                  "refill": (16,), "batch": (64, 128), "primary_packets": (0, 1)})
         tuned, source = res["config"], "tuned in this run"
     for s in sc:
         s.set_launch_config(**tuned)
@@ -275,6 +275,8 @@ def main() -> int:
     alg_bytes = (counted["closest_visits_first"] * NODE_BYTES[build["width"]]
                  + counted["closest_tests_first"] * TRI_BYTES + n_ids * HIT_BYTES)
     achieved = alg_bytes / (t_first * 1e-3) / 1e9
+    frame_bytes = (counted["node_visits"] * NODE_BYTES[build["width"]] + counted["tri_tests"] * TRI_BYTES
+                   + st["hits"] * 72 + W * H * 4)
     kernel = "k_trace<MODE_PRIMARY>"
     traffic = _traffic_from_profile(kernel, "config_c")
 
@@ -308,6 +310,12 @@ def main() -> int:
         "lbvh_build_ms": round(build["build_ms"], 3),
         "scene_gen_s": round(gen_s, 2),
         "frame_totals": {"node_visits": counted["node_visits"], "tri_tests": counted["tri_tests"]},
+        # whole-frame algorithmic bytes: node visits and triangle tests of every
+        # ray, 72 B of shading data per hit, the RGBA8 frame
+        "frame_roofline": {"alg_bytes": int(frame_bytes), "render_ms": round(mean("render_ms"), 4),
+                           "achieved": round(frame_bytes / (mean("render_ms") * 1e-3) / 1e9, 2),
+                           "unit": "GB/s", "frac": round(frame_bytes / (mean("render_ms") * 1e-3) / 1e9
+                                                         / HBM_PEAK_GBS, 4)},
     }
     try:
         result["device"] = E.device_name()

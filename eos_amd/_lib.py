@@ -83,7 +83,7 @@ class BatchStats(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("workgroup", C.c_int32), ("stack_lds", C.c_int32), ("waves_per_cu", C.c_int32),
-                ("refill", C.c_int32)]
+                ("refill", C.c_int32), ("batch", C.c_int32), ("primary_packets", C.c_int32)]
 
 
 _LIB = None
@@ -191,8 +191,16 @@ class Scene:
         self._lights = np.ascontiguousarray(lights)
         check(lib().eos_update_lights(self.h, _ptr(self._lights), len(self._lights)))
 
-    def set_launch_config(self, workgroup=256, stack_lds=16, waves_per_cu=32, refill=16):
-        c = LaunchConfig(workgroup, stack_lds, waves_per_cu, refill)
+    DEFAULT_LAUNCH = {"workgroup": 256, "stack_lds": 16, "waves_per_cu": 20, "refill": 16, "batch": 128,
+                      "primary_packets": 0}
+
+    def set_launch_config(self, *args, **kw):
+        """Positional fields in eos_launch_config order, or keywords; fields not
+        given keep their current value.  No arguments: the defaults."""
+        cfg = dict(self.DEFAULT_LAUNCH) if not args and not kw else self.launch_config()
+        cfg.update(dict(zip([f for f, _ in LaunchConfig._fields_], args)))
+        cfg.update(kw)
+        c = LaunchConfig(**cfg)
         check(lib().eos_set_launch_config(self.h, C.byref(c)))
 
     def launch_config(self) -> dict:

@@ -1,9 +1,10 @@
 """Autotuner for the traversal launch configuration.
 
-Searches workgroup size, per-lane LDS stack depth, persistent waves per CU and
-the refill batch (how many idle lanes a wave refills at once) on a given
-workload, and persists the winner as JSON.  ``load`` returns a saved
-configuration for ``Scene.set_launch_config``.
+Searches workgroup size, per-lane LDS stack depth, persistent waves per CU,
+the refill threshold (how many idle lanes a wave refills at once) and the ray
+batch (how many items a wave claims per fetch) on a given workload, and
+persists the winner as JSON.  ``load`` returns a saved configuration for
+``Scene.set_launch_config``.
 """
 from __future__ import annotations
 
@@ -19,10 +20,12 @@ from . import _lib as E
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_launch.json")
 
 GRID = {
-    "workgroup": (64, 128, 256),
-    "stack_lds": (8, 16, 32),
-    "waves_per_cu": (16, 32),
-    "refill": (8, 32),
+    "workgroup": (128, 256),
+    "stack_lds": (8, 16),
+    "waves_per_cu": (16, 20, 32),
+    "refill": (16,),
+    "batch": (64, 128),
+    "primary_packets": (0, 1),
 }
 
 
@@ -40,12 +43,12 @@ def _time_single(scene, depth, reps):
     return float(np.median([scene.render(depth=depth, want_rgba=False)["stats"]["render_ms"] for _ in range(reps)]))
 
 
-def tune(scenes, depth: int, reps: int = 3, grid=None, batch: bool = True, use_graph: bool = True,
+def tune(scenes, depth: int, reps: int = 3, grid=None, batched: bool = True, use_graph: bool = True,
          path: str | None = DEFAULT_PATH, workload: str = "") -> dict:
     """Times every configuration of ``grid`` and returns (and saves) the fastest.
 
-    batch=True times eos_render_batch over ``scenes`` (the batch uses scenes[0]'s
-    configuration); batch=False times eos_render of scenes[0]."""
+    batched=True times eos_render_batch over ``scenes`` (the batch uses scenes[0]'s
+    configuration); batched=False times eos_render of scenes[0]."""
     grid = grid or GRID
     keys = list(grid)
     results = []
@@ -53,13 +56,13 @@ def tune(scenes, depth: int, reps: int = 3, grid=None, batch: bool = True, use_g
     for vals in itertools.product(*(grid[k] for k in keys)):
         cfg = dict(zip(keys, vals))
         scenes[0].set_launch_config(**cfg)
-        ms = _time_batch(scenes, depth, reps, use_graph) if batch else _time_single(scenes[0], depth, reps)
+        ms = _time_batch(scenes, depth, reps, use_graph) if batched else _time_single(scenes[0], depth, reps)
         results.append((ms, cfg))
     results.sort(key=lambda r: r[0])
     best_ms, best = results[0]
     scenes[0].set_launch_config(**best)
     out = {"config": best, "ms": best_ms, "worst_ms": results[-1][0], "tried": len(results),
-           "seconds": time.perf_counter() - t0, "workload": workload, "batch": batch,
+           "seconds": time.perf_counter() - t0, "workload": workload, "batched": batched,
            "device": E.device_name()}
     if path:
         with open(path, "w") as f:

@@ -193,8 +193,8 @@ eos_status render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint
         a.level = l;
         a.npix = (int)npix;
         a.count_ptr = ctr + CTR_HITS + l;
-        a.surf = fb.surf[l].as<float4>();
-        a.vis = fb.vis[l].as<uint8_t>();
+        a.surf = fb.surf_at(l);
+        a.vis = fb.vis_at(l);
         a.levcol = fb.levcol.as<float4>();
         a.dl = dl_ptr;
         a.gout = g_dev.as<double>();

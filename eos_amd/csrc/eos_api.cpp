@@ -69,12 +69,13 @@ static void free_scene(eos_scene* s) {
                       &s->fb.hit0, &s->fb.q[0], &s->fb.q[1],
                       &s->fb.hitq, &s->fb.levcol, &s->fb.levels, &s->fb.counters, &s->fb.gstack};
     for (DevBuf* b : bufs) b->release();
-    for (int i = 0; i <= EOS_MAX_DEPTH; ++i) {
-        s->fb.surf[i].release();
-        s->fb.vis[i].release();
-    }
+    s->fb.surf.release();
+    s->fb.vis.release();
     for (hipEvent_t& e : s->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t& e : s->sync)
+        if (e) (void)hipEventDestroy(e);
+    if (s->stream2) (void)hipStreamDestroy(s->stream2);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -141,12 +142,21 @@ eos_status eos_create_scene(const eos_scene_desc* d, eos_scene** out) {
     EOS_HIP(hipGetDeviceProperties(&p, s->device));
     s->num_cu = p.multiProcessorCount;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking);
     if (e != hipSuccess) {
+        free_scene(s);
         delete s;
         return fail(EOS_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     for (hipEvent_t& ev : s->ev) {
         if (hipEventCreate(&ev) != hipSuccess) {
+            free_scene(s);
+            delete s;
+            return fail(EOS_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    for (hipEvent_t& ev : s->sync) {
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
             free_scene(s);
             delete s;
             return fail(EOS_ERR_HIP, "hipEventCreate failed");
@@ -258,10 +268,15 @@ eos_status eos_set_launch_config(eos_scene* s, const eos_launch_config* c) {
         return fail(EOS_ERR_INVALID, "stack_lds must be 8, 16 or 32");
     if (c->waves_per_cu < 1 || c->waves_per_cu > 64) return fail(EOS_ERR_INVALID, "waves_per_cu must be in [1, 64]");
     if (c->refill < 1 || c->refill > 64) return fail(EOS_ERR_INVALID, "refill must be in [1, 64]");
+    if (c->batch < 1 || c->batch > 4096) return fail(EOS_ERR_INVALID, "batch must be in [1, 4096]");
+    if (c->primary_packets != 0 && c->primary_packets != 1)
+        return fail(EOS_ERR_INVALID, "primary_packets must be 0 or 1");
     s->cfg.workgroup = c->workgroup;
     s->cfg.stack_lds = c->stack_lds;
     s->cfg.waves_per_cu = c->waves_per_cu;
     s->cfg.refill = c->refill;
+    s->cfg.batch = c->batch;
+    s->cfg.packets = c->primary_packets;
     return EOS_OK;
 }
 
@@ -271,6 +286,8 @@ eos_status eos_get_launch_config(const eos_scene* s, eos_launch_config* c) {
     c->stack_lds = s->cfg.stack_lds;
     c->waves_per_cu = s->cfg.waves_per_cu;
     c->refill = s->cfg.refill;
+    c->batch = s->cfg.batch;
+    c->primary_packets = s->cfg.packets;
     return EOS_OK;
 }
 

@@ -104,8 +104,10 @@ struct DevBuf {
 struct LaunchCfg {
     int workgroup = 256;
     int stack_lds = 16;
-    int waves_per_cu = 32;
+    int waves_per_cu = 20;
     int refill = 16;
+    int batch = 128;       // most items a traversal wave claims per fetch
+    int packets = 0;       // primary rays as one 8x8 packet per wave (4-wide tree)
 };
 
 }  // namespace eos
@@ -116,8 +118,15 @@ struct eos_frame_buffers {
     eos::DevBuf hit0;       // float4 [n_ids]      level-0 hit records (t, slot, u, v)
     eos::DevBuf q[2];       // float4 [cap][2]     ray queues (o.xyz pixel | d.xyz 0)
     eos::DevBuf hitq;       // float4 [cap]        hit records of queued rays
-    eos::DevBuf surf[EOS_MAX_DEPTH + 1];  // per level: float4 [cap][2] (P.xyz pixel | N.xyz mat)
-    eos::DevBuf vis[EOS_MAX_DEPTH + 1];   // per level: uint8 [cap * L]
+    // Per level l, at fixed strides so one launch can address several levels:
+    eos::DevBuf surf;       // float4 [D+1][cap][2] hit surfaces (P.xyz pixel | N.xyz mat)
+    eos::DevBuf vis;        // uint8  [D+1][cap][L] light visible from the hit surface
+    size_t cap = 0;         // surfaces per level (pixels of the frame)
+    int lslots = 1;         // light slots per surface
+    size_t surf_stride() const { return 2 * cap; }                // float4 per level
+    size_t vis_stride() const { return cap * (size_t)lslots; }    // bytes per level
+    float4* surf_at(int l) { return surf.as<float4>() + (size_t)l * surf_stride(); }
+    uint8_t* vis_at(int l) { return vis.as<uint8_t>() + (size_t)l * vis_stride(); }
     eos::DevBuf levcol;     // float4 [(D+1) * npix]  level colour rgb + reflection weight
     eos::DevBuf levels;     // uint8  [npix]
     eos::DevBuf counters;   // int64  [64]
@@ -131,6 +140,11 @@ struct eos_frame_buffers {
 struct eos_scene {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Second stream of a frame: level l's shadow rays and shading run on it,
+    // overlapping level l+1's closest-hit trace on `stream`.  Both streams
+    // keep the default priority.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t sync[2] = {};  // fork to and join from stream2 (no timing)
     int n_tri = 0, n_mat = 0, n_light = 0;
     // original-order inputs (device)
     eos::DevBuf verts, normals, matidx;
@@ -149,7 +163,7 @@ struct eos_scene {
     eos::LaunchCfg cfg;
     int num_cu = 256;
     eos_frame_buffers fb;
-    hipEvent_t ev[64] = {};
+    hipEvent_t ev[96] = {};  // frame timing events
 };
 
 namespace eos {

@@ -240,7 +240,10 @@ eos_status launch_trace(const LaunchCfg& cfg, int width, int grid, hipStream_t s
     case WG * 100 + SL:                                                                                   \
         hipLaunchKernelGGL((k_trace<MODE, WG, SL, COUNT, BATCH, 4>), dim3(grid), dim3(WG), 0, s, a);      \
         break;
-    if (width == 2) {
+    if (MODE == MODE_PRIMARY && width == 4 && cfg.packets) {
+        hipLaunchKernelGGL((k_trace_packet<256, COUNT, BATCH>), dim3(std::max(1, grid * cfg.workgroup / 256)),
+                           dim3(256), 0, s, a);
+    } else if (width == 2) {
         hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 2>), dim3(grid), dim3(256), 0, s, a);
     } else if (COUNT) {
         hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 4>), dim3(grid), dim3(256), 0, s, a);
@@ -297,9 +300,31 @@ struct Frame {
     int32_t* hits = nullptr;
 };
 
+// Overflow stack entries of one persistent traversal grid.
+size_t gstack_ints(const Frame& f, int trace_grid) {
+    const int maxs = f.width == 4 ? kMaxStack4 : kMaxStack;
+    return (size_t)trace_grid * f.cfg.workgroup * (maxs - f.cfg.stack_lds);
+}
+
 struct FrameEvents {
     int start, end, resolve0;
-    int c[2 * EOS_MAX_DEPTH + 2], s[2 * EOS_MAX_DEPTH + 2], p[2 * EOS_MAX_DEPTH + 2];
+    // per level l: [2l] before, [2l+1] after the launch
+    int c[2 * EOS_MAX_DEPTH + 2];  // closest-hit trace (main stream)
+    int u[2 * EOS_MAX_DEPTH + 2];  // hit setup (main stream)
+    int s[4];                      // shadow trace: level 0, levels 1..depth
+    int h[2 * EOS_MAX_DEPTH + 2];  // Phong (second stream)
+};
+
+// The streams a frame runs on.  Level 0's shadow rays (most of a frame's
+// any-hit work) and its Phong run on S2 after a fork event recorded behind
+// level 0's setup, overlapping the reflection levels' closest-hit traces on S;
+// the shadow rays of levels 1..depth follow on S in one launch after the last
+// setup; S joins S2 before the resolve.  The two traversals that can be in
+// flight together use disjoint halves of the overflow stack area.
+struct FrameStreams {
+    hipStream_t S = nullptr;
+    hipStream_t S2 = nullptr;       // null: everything in order on S
+    hipEvent_t* sync = nullptr;     // [2]: fork, join
 };
 
 eos_status ensure_buffers(eos_frame_buffers& fb, const Frame& f, int trace_grid) {
@@ -308,15 +333,14 @@ eos_status ensure_buffers(eos_frame_buffers& fb, const Frame& f, int trace_grid)
     EOS_TRY(fb.q[0].ensure(32 * cap));
     EOS_TRY(fb.q[1].ensure(32 * cap));
     EOS_TRY(fb.hitq.ensure(16 * cap));
-    for (int l = 0; l <= f.depth; ++l) {
-        EOS_TRY(fb.surf[l].ensure(32 * cap));
-        EOS_TRY(fb.vis[l].ensure(cap * (size_t)std::max(f.lstride, 1)));
-    }
+    fb.cap = cap;
+    fb.lslots = std::max(f.lstride, 1);
+    EOS_TRY(fb.surf.ensure(16 * fb.surf_stride() * (size_t)(f.depth + 1)));
+    EOS_TRY(fb.vis.ensure(fb.vis_stride() * (size_t)(f.depth + 1)));
     EOS_TRY(fb.levcol.ensure(16 * f.npix * (size_t)(f.depth + 1)));
     EOS_TRY(fb.levels.ensure(f.npix));
     EOS_TRY(fb.counters.ensure(4 * CTR_COUNT + 8 * kStats));
-    const int maxs = f.width == 4 ? kMaxStack4 : kMaxStack;
-    EOS_TRY(fb.gstack.ensure((size_t)trace_grid * f.cfg.workgroup * (maxs - f.cfg.stack_lds) * 4));
+    EOS_TRY(fb.gstack.ensure(2 * gstack_ints(f, trace_grid) * 4));
     fb.npix = f.npix;
     fb.n_ids = f.n_ids;
     fb.depth = f.depth;
@@ -324,22 +348,71 @@ eos_status ensure_buffers(eos_frame_buffers& fb, const Frame& f, int trace_grid)
     return EOS_OK;
 }
 
-// Enqueues every launch of one frame on stream S.  Events are recorded when
-// E is non-null (not while a graph is being captured).
-eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hipEvent_t* E, FrameEvents* fe) {
+// Enqueues every launch of one frame.  Timing events are recorded when E is
+// non-null (not while a graph is being captured).
+eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStreams& fs, hipEvent_t* E,
+                         FrameEvents* fe) {
     const int trace_grid = std::max(1, f.num_cu * f.cfg.waves_per_cu / (f.cfg.workgroup / 64));
     int* ctr = fb.counters.as<int>();
     unsigned long long* stats = reinterpret_cast<unsigned long long*>(fb.counters.as<char>() + 4 * CTR_COUNT);
     const unsigned shade_grid = (unsigned)f.num_cu * 8;
+    const hipStream_t S = fs.S;
+    const bool two = fs.S2 != nullptr;
+    const hipStream_t S2 = two ? fs.S2 : fs.S;
     int ev = 0;
-    auto rec = [&]() -> int {
-        if (E) (void)hipEventRecord(E[ev], S);
+    auto rec = [&](hipStream_t st) -> int {
+        if (E) (void)hipEventRecord(E[ev], st);
         return ev++;
     };
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, S, ctr, CTR_COUNT + 2 * kStats);
     FrameEvents local;
     FrameEvents& F = fe ? *fe : local;
-    F.start = rec();
+    F.start = rec(S);
+    // Any-hit trace of levels [l0, l1] in one launch, then Phong of each level.
+    auto shadow_and_phong = [&](int l0, int l1, hipStream_t st, size_t gstack_off, int slot) -> eos_status {
+        F.s[2 * slot] = rec(st);
+        if (f.lstride > 0) {
+            TraceArgs sh{};
+            sh.sc = f.sc;
+            sh.cam = f.cam;
+            sh.tiles_x = f.tiles_x;
+            sh.table = f.table;
+            sh.n_scenes = f.n_scenes;
+            sh.gstack = fb.gstack.as<int32_t>() + gstack_off;
+            sh.err = ctr + CTR_ERR;
+            sh.refill = f.cfg.refill;
+            sh.batch = f.cfg.batch;
+            sh.cursor = ctr + CTR_CUR_S + l0;
+            sh.stats = stats + 2 * (16 + l0);
+            sh.count_ptr = ctr + CTR_HITS + l0;
+            sh.count_mul = f.lstride;
+            sh.nlev = l1 - l0 + 1;
+            sh.surf = fb.surf_at(l0);
+            sh.vis_out = fb.vis_at(l0);
+            sh.surf_stride = (long long)fb.surf_stride();
+            sh.vis_stride = (long long)fb.vis_stride();
+            EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.width, f.cfg, trace_grid, st, sh));
+        }
+        F.s[2 * slot + 1] = rec(st);
+        for (int l = l0; l <= l1; ++l) {
+            PhongArgs pa{};
+            pa.sc = f.sc;
+            pa.cam = f.cam;
+            pa.table = f.table;
+            pa.level = l;
+            pa.npix = (int)f.npix;
+            pa.lstride = f.lstride;
+            pa.count_ptr = ctr + CTR_HITS + l;
+            pa.surf = fb.surf_at(l);
+            pa.vis = fb.vis_at(l);
+            pa.levcol = fb.levcol.as<float4>();
+            F.h[2 * l] = rec(st);
+            if (f.batch) hipLaunchKernelGGL(k_phong<true>, dim3(shade_grid), dim3(256), 0, st, pa);
+            else hipLaunchKernelGGL(k_phong<false>, dim3(shade_grid), dim3(256), 0, st, pa);
+            F.h[2 * l + 1] = rec(st);
+        }
+        return EOS_OK;
+    };
     for (int l = 0; l <= f.depth; ++l) {
         float4* qin = fb.q[l & 1].as<float4>();
         float4* qout = fb.q[(l + 1) & 1].as<float4>();
@@ -352,6 +425,7 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, h
         ta.gstack = fb.gstack.as<int32_t>();
         ta.err = ctr + CTR_ERR;
         ta.refill = f.cfg.refill;
+        ta.batch = f.cfg.batch;
         ta.cursor = ctr + CTR_CUR_C + l;
         ta.stats = stats + 2 * l;
         if (l == 0) {
@@ -365,10 +439,10 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, h
             ta.q = qin;
             ta.hit_out = fb.hitq.as<float4>();
         }
-        F.c[2 * l] = rec();
+        F.c[2 * l] = rec(S);
         if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(f.count, f.batch, f.width, f.cfg, trace_grid, S, ta));
         else EOS_TRY(launch_trace_any<MODE_QUEUE>(f.count, f.batch, f.width, f.cfg, trace_grid, S, ta));
-        F.c[2 * l + 1] = rec();
+        F.c[2 * l + 1] = rec(S);
         SetupArgs sa{};
         sa.sc = f.sc;
         sa.cam = f.cam;
@@ -381,14 +455,14 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, h
         sa.count_ptr = ctr + CTR_Q + l;
         sa.q_in = qin;
         sa.hit_in = l == 0 ? fb.hit0.as<float4>() : fb.hitq.as<float4>();
-        sa.surf_out = fb.surf[l].as<float4>();
+        sa.surf_out = fb.surf_at(l);
         sa.hits_ctr = ctr + CTR_HITS + l;
         sa.q_out = qout;
         sa.q_out_ctr = ctr + CTR_Q + l + 1;
         sa.shadow_ctr = ctr + CTR_SHADOW;
         sa.levels = fb.levels.as<uint8_t>();
         sa.hit_ids = l == 0 ? f.hits : nullptr;
-        F.p[2 * l] = rec();
+        F.u[2 * l] = rec(S);
         if (l == 0) {
             if (f.batch)
                 hipLaunchKernelGGL((k_setup<true, true>), dim3(blocks_for(f.n_ids, 256, shade_grid)), dim3(256), 0, S, sa);
@@ -398,37 +472,26 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, h
             if (f.batch) hipLaunchKernelGGL((k_setup<false, true>), dim3(shade_grid), dim3(256), 0, S, sa);
             else hipLaunchKernelGGL((k_setup<false, false>), dim3(shade_grid), dim3(256), 0, S, sa);
         }
-        F.s[2 * l] = rec();
-        if (f.lstride > 0) {
-            TraceArgs sh = ta;
-            sh.cursor = ctr + CTR_CUR_S + l;
-            sh.stats = stats + 2 * (16 + l);
-            sh.count_ptr = ctr + CTR_HITS + l;
-            sh.count_mul = f.lstride;
-            sh.surf = fb.surf[l].as<float4>();
-            sh.vis_out = fb.vis[l].as<uint8_t>();
-            EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.width, f.cfg, trace_grid, S, sh));
+        F.u[2 * l + 1] = rec(S);
+        if (l == 0) {
+            // level 0's shadow rays and shading overlap the reflection chain
+            if (two) {
+                EOS_HIP(hipEventRecord(fs.sync[0], S));
+                EOS_HIP(hipStreamWaitEvent(S2, fs.sync[0], 0));
+            }
+            EOS_TRY(shadow_and_phong(0, 0, S2, two ? gstack_ints(f, trace_grid) : 0, 0));
         }
-        F.s[2 * l + 1] = rec();
-        PhongArgs pa{};
-        pa.sc = f.sc;
-        pa.cam = f.cam;
-        pa.table = f.table;
-        pa.level = l;
-        pa.npix = (int)f.npix;
-        pa.lstride = f.lstride;
-        pa.count_ptr = ctr + CTR_HITS + l;
-        pa.surf = fb.surf[l].as<float4>();
-        pa.vis = fb.vis[l].as<uint8_t>();
-        pa.levcol = fb.levcol.as<float4>();
-        if (f.batch) hipLaunchKernelGGL(k_phong<true>, dim3(shade_grid), dim3(256), 0, S, pa);
-        else hipLaunchKernelGGL(k_phong<false>, dim3(shade_grid), dim3(256), 0, S, pa);
-        F.p[2 * l + 1] = rec();
+    }
+    // deeper levels' shadow rays in one launch once the last level is set up
+    if (f.depth >= 1) EOS_TRY(shadow_and_phong(1, f.depth, S, 0, 1));
+    if (two) {
+        EOS_HIP(hipEventRecord(fs.sync[1], S2));
+        EOS_HIP(hipStreamWaitEvent(S, fs.sync[1], 0));
     }
     ResolveArgs ra{(int)f.npix, fb.levels.as<uint8_t>(), fb.levcol.as<float4>(), f.rgba, f.rgb};
-    F.resolve0 = rec();
+    F.resolve0 = rec(S);
     hipLaunchKernelGGL(k_resolve, dim3(blocks_for(f.npix, 256, shade_grid)), dim3(256), 0, S, ra);
-    F.end = rec();
+    F.end = rec(S);
     EOS_HIP(hipGetLastError());
     return EOS_OK;
 }
@@ -458,22 +521,25 @@ eos_status finish_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hi
         (void)hipEventElapsedTime(&ms, E[F->start], E[F->end]);
         st->render_ms = ms;
         for (int l = 0; l <= f.depth; ++l) {
-            float a = 0.f, b = 0.f, c = 0.f, d2 = 0.f;
+            float a = 0.f, c = 0.f, d2 = 0.f;
             (void)hipEventElapsedTime(&a, E[F->c[2 * l]], E[F->c[2 * l + 1]]);
-            (void)hipEventElapsedTime(&b, E[F->s[2 * l]], E[F->s[2 * l + 1]]);
-            (void)hipEventElapsedTime(&c, E[F->p[2 * l]], E[F->s[2 * l]]);
-            (void)hipEventElapsedTime(&d2, E[F->s[2 * l + 1]], E[F->p[2 * l + 1]]);
+            (void)hipEventElapsedTime(&c, E[F->u[2 * l]], E[F->u[2 * l + 1]]);
+            (void)hipEventElapsedTime(&d2, E[F->h[2 * l]], E[F->h[2 * l + 1]]);
             st->trace_closest_ms += a;
-            st->trace_shadow_ms += b;
             st->shade_ms += c + d2;
             if (l == 0) st->trace_first_ms = a;
+        }
+        for (int k = 0; k < (f.depth >= 1 ? 2 : 1); ++k) {
+            float b = 0.f;
+            (void)hipEventElapsedTime(&b, E[F->s[2 * k]], E[F->s[2 * k + 1]]);
+            st->trace_shadow_ms += b;
         }
         float r = 0.f;
         (void)hipEventElapsedTime(&r, E[F->resolve0], E[F->end]);
         st->shade_ms += r;
     }
     st->trace_closest_launches = f.depth + 1;
-    st->trace_shadow_launches = f.lstride > 0 ? f.depth + 1 : 0;
+    st->trace_shadow_launches = f.lstride > 0 ? (f.depth >= 1 ? 2 : 1) : 0;
     if (f.count) {
         for (int l = 0; l <= f.depth; ++l) {
             st->node_visits += (int64_t)(hstats[2 * l] + hstats[2 * (16 + l)]);
@@ -513,7 +579,8 @@ eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count
     const int trace_grid = std::max(1, f.num_cu * f.cfg.waves_per_cu / (f.cfg.workgroup / 64));
     EOS_TRY(ensure_buffers(s->fb, f, trace_grid));
     FrameEvents fe;
-    EOS_TRY(enqueue_frame(s->fb, f, s->stream, s->ev, &fe));
+    FrameStreams fs{s->stream, s->stream2, s->sync};
+    EOS_TRY(enqueue_frame(s->fb, f, fs, s->ev, &fe));
     return finish_frame(s->fb, f, s->stream, s->ev, &fe, st);
 }
 
@@ -577,6 +644,7 @@ struct BatchState {
     LaunchCfg cfg;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t sync[2] = {};
     Frame frame;
 };
 BatchState g_batch;
@@ -585,7 +653,7 @@ bool same_batch(eos_scene* const* scenes, const eos_camera* cams, int n, int dep
     const BatchState& b = g_batch;
     if ((int)b.scenes.size() != n || b.depth != depth) return false;
     if (b.cfg.workgroup != cfg.workgroup || b.cfg.stack_lds != cfg.stack_lds || b.cfg.waves_per_cu != cfg.waves_per_cu ||
-        b.cfg.refill != cfg.refill)
+        b.cfg.refill != cfg.refill || b.cfg.batch != cfg.batch || b.cfg.packets != cfg.packets)
         return false;
     for (int i = 0; i < n; ++i) {
         if (b.scenes[i] != scenes[i]) return false;
@@ -607,7 +675,9 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
     if (!b.ev[0]) {
         EOS_HIP(hipEventCreate(&b.ev[0]));
         EOS_HIP(hipEventCreate(&b.ev[1]));
+        for (hipEvent_t& e : b.sync) EOS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
+    const FrameStreams fs{S, s0->stream2, b.sync};
     const bool same = same_batch(scenes, cams, n, o->depth, cfg);
     if (!same) {
         if (b.exec) {
@@ -666,7 +736,7 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
         if (!b.exec) {
             hipGraph_t graph = nullptr;
             EOS_HIP(hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
-            eos_status es = enqueue_frame(b.fb, f, S, nullptr, nullptr);
+            eos_status es = enqueue_frame(b.fb, f, fs, nullptr, nullptr);
             hipError_t ce = hipStreamEndCapture(S, &graph);
             if (es != EOS_OK) return es;
             if (ce != hipSuccess) return fail(EOS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
@@ -679,7 +749,7 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
         EOS_HIP(hipEventRecord(b.ev[1], S));
     } else {
         EOS_HIP(hipEventRecord(b.ev[0], S));
-        EOS_TRY(enqueue_frame(b.fb, f, S, nullptr, nullptr));
+        EOS_TRY(enqueue_frame(b.fb, f, fs, nullptr, nullptr));
         EOS_HIP(hipEventRecord(b.ev[1], S));
     }
     eos_render_stats rs;

@@ -61,21 +61,24 @@ __device__ __forceinline__ bool tile_pixel(int r, int tiles_x, int W, int H, int
 }
 
 // src/Triangle.cpp:57-91 with e1 = v2 - v1, e2 = v3 - v1 precomputed (the same
-// float subtractions, done once at build time).
+// float subtractions, done once at build time).  Branch-free: every quantity is
+// computed with the reference's operations and the reference's early-outs
+// become one predicate with the same comparisons (so NaN behaves the same).
+// Keeping the arithmetic unconditional stops the compiler from sinking the
+// load of v1 behind the determinant test, which would cost a second memory
+// round trip per triangle.
 __device__ __forceinline__ bool moller_trumbore(F3 o, F3 d, F3 v1, F3 e1, F3 e2, float& t, float& u, float& v) {
     const F3 P = cross3(d, e2);
     const float det = dot3(e1, P);
-    if (det > -0.00001f && det < 0.00001f) return false;
     const float inv = 1.0f / det;
     const F3 T = sub(o, v1);
     const F3 Q = cross3(T, e1);
     t = dot3(e2, Q) * inv;
-    if (t < 0.0f) return false;
     u = dot3(T, P) * inv;
-    if (u < 0.0f || u > 1.0f) return false;
     v = dot3(d, Q) * inv;
-    if (v < 0.0f || u + v > 1.0f) return false;
-    return true;
+    const bool rej = (det > -0.00001f && det < 0.00001f) | (t < 0.0f) | (u < 0.0f || u > 1.0f) |
+                     (v < 0.0f || u + v > 1.0f);
+    return !rej;
 }
 
 __device__ __forceinline__ float safe_inv(float d) {
@@ -175,18 +178,26 @@ struct TraceArgs {
     int* cursor;             // persistent fetch cursor
     const float4* q;         // MODE_QUEUE: ray queue
     float4* hit_out;         // closest: (t, slot, u, v)
-    const float4* surf;      // MODE_SHADOW: hit surfaces (P.xyz pixel | N.xyz mat)
-    uint8_t* vis_out;        // MODE_SHADOW: 1 = light visible
+    const float4* surf;      // MODE_SHADOW: hit surfaces (P.xyz pixel | N.xyz mat) of the first level
+    uint8_t* vis_out;        // MODE_SHADOW: 1 = light visible, first level
+    int nlev;                // MODE_SHADOW: consecutive levels in this launch (count_ptr[0..nlev))
+    long long surf_stride;   // MODE_SHADOW: float4 between levels' surfaces
+    long long vis_stride;    // MODE_SHADOW: bytes between levels' visibility
     int32_t* gstack;         // overflow stacks
     int* err;
     unsigned long long* stats;
     int refill;
+    int batch;               // most items a wave claims per fetch
 };
 
 // Persistent traversal.  Each wave keeps up to 64 independent rays in flight;
-// whenever at least `refill` lanes are idle it claims that many new items from
-// the fetch cursor with one atomic (ballot + mbcnt ranks the idle lanes), so
-// lanes never wait for the slowest ray of a fixed batch.  Per lane: ordered
+// whenever at least `refill` lanes are idle it hands them the next items of its
+// private chunk (ballot + mbcnt ranks the idle lanes), so lanes never wait for
+// the slowest ray of a fixed batch.  A wave claims a new chunk from the fetch
+// cursor with one atomic; chunks shrink as the cursor nears the end (guided
+// self-scheduling: remaining / (2 * waves), between the idle count and the
+// launch's ray batch), which keeps the single device-scope counter far from
+// saturation on cheap rays and still balances the tail.  Per lane: ordered
 // near-first descent, far child pushed onto a short stack in LDS (STACK_LDS
 // entries, column per lane) that overflows into a private global area, pruning
 // against the best hit so far.  Per-ray visit order is deterministic and equal
@@ -213,13 +224,22 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     const int tid = threadIdx.x;
     const int lane = __lane_id();
     int32_t* gst = a.gstack + (size_t)(blockIdx.x * WG + tid) * (MAXS - STACK_LDS);
-    const int n = a.count_ptr ? (*a.count_ptr) * a.count_mul : a.n_static;
+    int n = a.n_static;
+    if (a.count_ptr) {
+        n = 0;
+        const int nlev = MODE == MODE_SHADOW ? a.nlev : 1;
+        for (int k = 0; k < nlev; ++k) n += a.count_ptr[k] * a.count_mul;
+    }
     const Node* __restrict__ nodes = a.sc.nodes;
     const Node4* __restrict__ nodes4 = a.sc.nodes4;
     const float4* __restrict__ geo = a.sc.geo;
 
     int item = -1;
-    bool drained = false;
+    size_t out_idx = 0;          // MODE_SHADOW: visibility byte of the item
+    bool drained = false;        // the fetch cursor has passed n
+    int chunk_next = 0, chunk_end = 0;  // wave-uniform: unissued items of this wave's chunk
+    int seen = 0;                       // wave-uniform: cursor value after this wave's last claim
+    const int waves_total = (int)gridDim.x * (WG / 64);
     F3 o = {0, 0, 0}, d = {0, 0, 0};
     float ix = 0, iy = 0, iz = 0, px = 0, py = 0, pz = 0;
     float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
@@ -234,15 +254,38 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         // ---- refill idle lanes
         const uint64_t idle = __ballot(item < 0);
         const int nidle = __popcll(idle);
-        if (!drained && nidle >= thresh) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(a.cursor, nidle);
-            base = __shfl(base, leader);
-            if (base + nidle >= n) drained = true;
+        if (nidle >= thresh && (!drained || chunk_next < chunk_end)) {
+            // Wave-uniform bookkeeping: idle lanes take the rest of the chunk
+            // first, then the head of a newly claimed one.
+            const int avail = chunk_end - chunk_next;
+            const int old_next = chunk_next;
+            int base = 0, take = 0;
+            if (avail < nidle && !drained) {
+                const int want = nidle - avail;
+                int size = (n - seen) / (2 * waves_total);
+                size = size > a.batch ? a.batch : size;
+                size = size < want ? want : size;
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                if (lane == leader) base = atomicAdd(a.cursor, size);
+                base = __shfl(base, leader);
+                seen = base + size;
+                if (base + size >= n) drained = true;
+                const int end = base + size < n ? base + size : n;
+                take = end > base ? end - base : 0;
+            }
+            const int rank = __popcll(idle & lanemask_lt());
+            int my = -1;
+            if (rank < avail) my = old_next + rank;
+            else if (rank - avail < take) my = base + (rank - avail);
+            if (avail >= nidle) {
+                chunk_next = old_next + nidle;
+            } else {
+                const int used = nidle - avail < take ? nidle - avail : take;
+                chunk_next = take > used ? base + used : 0;
+                chunk_end = take > used ? base + take : 0;
+            }
             if (item < 0) {
-                const int my = base + __popcll(idle & lanemask_lt());
-                if (my < n) {
+                if (my >= 0) {
                     item = my;
                     bool valid = true;
                     int root = WIDTH == 4 ? a.sc.root4 : a.sc.root;
@@ -276,13 +319,23 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                             max_visits = B.sc.n_tri + 64;
                         }
                     } else {
-                        const int j = my / a.count_mul, li = my - j * a.count_mul;
-                        const float4 s0 = a.surf[2 * (size_t)j];
+                        // item -> (level, surface j, light slot li)
+                        int rem = my, lev = 0;
+                        for (int k = 0; k + 1 < a.nlev; ++k) {
+                            const int c = a.count_ptr[k] * a.count_mul;
+                            if (rem < c) break;
+                            rem -= c;
+                            lev = k + 1;
+                        }
+                        out_idx = (size_t)lev * (size_t)a.vis_stride + (size_t)rem;
+                        const float4* surf = a.surf + (size_t)lev * (size_t)a.surf_stride;
+                        const int j = rem / a.count_mul, li = rem - j * a.count_mul;
+                        const float4 s0 = surf[2 * (size_t)j];
                         const F3 P = f3(s0.x, s0.y, s0.z);
                         const DevLight* lights = a.sc.lights;
                         int nl = a.sc.n_light;
                         if (BATCH) {
-                            const BatchScene& B = a.table[unpack_scene(__float_as_int(a.surf[2 * (size_t)j + 1].w))];
+                            const BatchScene& B = a.table[unpack_scene(__float_as_int(surf[2 * (size_t)j + 1].w))];
                             nodes = B.sc.nodes;
                             nodes4 = B.sc.nodes4;
                             geo = B.sc.geo;
@@ -318,7 +371,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
             }
         }
         if (__ballot(item >= 0) == 0) {
-            if (drained) break;
+            if (drained && chunk_next >= chunk_end) break;
             continue;
         }
         // ---- traverse
@@ -335,10 +388,11 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                         const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
                         const int4 rf = nd->ref;
                         float t0, t1, t2, t3;
-                        const bool h0 = rf.x != kEmpty && slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ix, iy, iz, px, py, pz, tmax, t0);
-                        const bool h1 = rf.y != kEmpty && slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ix, iy, iz, px, py, pz, tmax, t1);
-                        const bool h2 = rf.z != kEmpty && slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ix, iy, iz, px, py, pz, tmax, t2);
-                        const bool h3 = rf.w != kEmpty && slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ix, iy, iz, px, py, pz, tmax, t3);
+                        // all four tests unconditionally (no load sinking), empty slots masked after
+                        const bool h0 = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ix, iy, iz, px, py, pz, tmax, t0) & (rf.x != kEmpty);
+                        const bool h1 = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ix, iy, iz, px, py, pz, tmax, t1) & (rf.y != kEmpty);
+                        const bool h2 = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ix, iy, iz, px, py, pz, tmax, t2) & (rf.z != kEmpty);
+                        const bool h3 = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ix, iy, iz, px, py, pz, tmax, t3) & (rf.w != kEmpty);
                         const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
                         if (nh == 0) {
                             cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
@@ -404,23 +458,41 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     const int first = e & ((1 << kLeafShift) - 1);
                     const int cnt = ((e >> kLeafShift) & 7) + 1;
                     bool occluded = false;
-                    for (int s = first; s < first + cnt; ++s) {
-                        if (COUNT) ++tests;
-                        const float4 g0 = geo[3 * (size_t)s], g1 = geo[3 * (size_t)s + 1], g2 = geo[3 * (size_t)s + 2];
-                        float t, u, v;
-                        if (moller_trumbore(o, d, f3(g0.x, g0.y, g0.z), f3(g0.w, g1.x, g1.y), f3(g1.z, g1.w, g2.x), t,
-                                            u, v)) {
-                            if (ANY) {
+                    // Two triangles per iteration, both fetched before either is
+                    // tested; an odd leaf tests its last triangle twice (same
+                    // result, counted once).
+                    const int last = first + cnt - 1;
+                    for (int s0 = first; s0 <= last; s0 += 2) {
+                        const int s1 = s0 + 1 <= last ? s0 + 1 : s0;
+                        const float4 a0 = geo[3 * (size_t)s0], a1 = geo[3 * (size_t)s0 + 1], a2 = geo[3 * (size_t)s0 + 2];
+                        const float4 b0 = geo[3 * (size_t)s1], b1 = geo[3 * (size_t)s1 + 1], b2 = geo[3 * (size_t)s1 + 2];
+                        float ta, ua, va, tb, ub, vb;
+                        const bool ha = moller_trumbore(o, d, f3(a0.x, a0.y, a0.z), f3(a0.w, a1.x, a1.y),
+                                                        f3(a1.z, a1.w, a2.x), ta, ua, va);
+                        const bool hb = moller_trumbore(o, d, f3(b0.x, b0.y, b0.z), f3(b0.w, b1.x, b1.y),
+                                                        f3(b1.z, b1.w, b2.x), tb, ub, vb);
+                        // counted as the one-at-a-time walk would: an any-hit ray stops at its first hit
+                        if (COUNT) tests += (ANY && ha) ? 1 : (s1 != s0 ? 2 : 1);
+                        if (ANY) {
+                            if (ha | hb) {
                                 occluded = true;
                                 break;
                             }
-                            const int orig = __float_as_int(g2.y);
-                            if (t < best_t || (t == best_t && orig < best_orig)) {
-                                best_t = t;
-                                best_orig = orig;
-                                best_slot = s;
-                                bu = u;
-                                bv = v;
+                        } else {
+                            const int oa = __float_as_int(a2.y), ob = __float_as_int(b2.y);
+                            if (ha && (ta < best_t || (ta == best_t && oa < best_orig))) {
+                                best_t = ta;
+                                best_orig = oa;
+                                best_slot = s0;
+                                bu = ua;
+                                bv = va;
+                            }
+                            if (hb && (tb < best_t || (tb == best_t && ob < best_orig))) {
+                                best_t = tb;
+                                best_orig = ob;
+                                best_slot = s1;
+                                bu = ub;
+                                bv = vb;
                             }
                         }
                     }
@@ -433,7 +505,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                 }
                 if (cur == kDone) {
                     if (ANY) {
-                        a.vis_out[item] = best_slot >= 0 ? 0 : 1;
+                        a.vis_out[out_idx] = best_slot >= 0 ? 0 : 1;
                     } else {
                         a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
                     }
@@ -453,6 +525,163 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
             atomicAdd(&a.stats[0], visits);
             atomicAdd(&a.stats[1], tests);
         }
+    }
+}
+
+// Packet traversal of primary rays (4-wide tree).  One wave owns one 8x8 pixel
+// tile at a time: the node or leaf being visited is the same for the whole
+// wave (a wave-uniform reference, so node and triangle records arrive through
+// scalar loads, once per wave), every lane tests its own ray against the four
+// child boxes with its own best t, and the wave descends into every child that
+// some lane hits.  Children are ordered by the entry distance of the first lane
+// (lowest lane index) that hits them, then by slot; the nearest is taken and
+// the others are pushed farthest first onto a per-wave stack in LDS.  A lane's
+// nearest triangle lies in boxes that lane hits with entry distance <= its
+// best t, so the wave visits them: each lane's result is the exact nearest hit
+// under the (t, original index) order, as in the per-ray walk.  Visit and test
+// counts are per wave (COUNT).
+// Read-only views in the constant address space: a wave-uniform load through
+// them is a scalar load (scalar cache, SGPRs).  Nodes and triangle records are
+// never written while a frame traces.  (The host pass only type-checks kernel
+// bodies and has no such address space.)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) Node4* ConstNode4Ptr;
+typedef const __attribute__((address_space(4))) float4* ConstF4Ptr;
+#else
+typedef const Node4* ConstNode4Ptr;
+typedef const float4* ConstF4Ptr;
+#endif
+
+template <int WG, bool COUNT, bool BATCH>
+__global__ __launch_bounds__(WG) void k_trace_packet(TraceArgs a) {
+    constexpr int MAXS = kMaxStack4;
+    __shared__ int32_t stk[WG / 64][MAXS];
+    const int lane = __lane_id();
+    int32_t* st = stk[threadIdx.x >> 6];
+    const int n_tiles = a.n_static >> 6;
+    unsigned long long visits = 0, tests = 0;
+    for (;;) {
+        int tile = 0;
+        if (lane == 0) tile = atomicAdd(a.cursor, 1);
+        tile = __builtin_amdgcn_readfirstlane(__shfl(tile, 0));
+        if (tile >= n_tiles) break;
+        const int my = tile * 64 + lane;
+        ConstNode4Ptr nodes4 = (ConstNode4Ptr)a.sc.nodes4;
+        ConstF4Ptr geo = (ConstF4Ptr)a.sc.geo;
+        int root = a.sc.root4, max_visits = a.sc.n_tri + 64;
+        int x, y;
+        bool valid;
+        F3 o, d;
+        if (BATCH) {
+            const BatchScene& B = a.table[scene_of_id(a.table, a.n_scenes, tile * 64)];
+            nodes4 = (ConstNode4Ptr)B.sc.nodes4;
+            geo = (ConstF4Ptr)B.sc.geo;
+            root = B.sc.root4;
+            max_visits = B.sc.n_tri + 64;
+            valid = tile_pixel(my - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
+            o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
+            d = camera_dir(B.cam, x, y);
+        } else {
+            valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
+            o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
+            d = camera_dir(a.cam, x, y);
+        }
+        const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
+        const float px = o.x * ix, py = o.y * iy, pz = o.z * iz;
+        float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
+        int best_slot = -1, best_orig = 0x7FFFFFFF;
+        int cur = __builtin_amdgcn_readfirstlane(root);
+        int sp = 0, budget = max_visits;
+        while (cur != kDone) {
+            if (cur >= 0) {
+                if (--budget < 0) {
+                    if (lane == 0) atomicOr(a.err, 2);
+                    break;
+                }
+                if (COUNT) ++visits;
+                ConstNode4Ptr nd = nodes4 + cur;
+                const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+                const int4 rf = nd->ref;
+                float t0, t1, t2, t3;
+                const bool h0 = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ix, iy, iz, px, py, pz, best_t, t0) & valid;
+                const bool h1 = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ix, iy, iz, px, py, pz, best_t, t1) & valid;
+                const bool h2 = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ix, iy, iz, px, py, pz, best_t, t2) & valid;
+                const bool h3 = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ix, iy, iz, px, py, pz, best_t, t3) & valid;
+                const uint64_t m0 = rf.x != kEmpty ? __ballot(h0) : 0, m1 = rf.y != kEmpty ? __ballot(h1) : 0,
+                               m2 = rf.z != kEmpty ? __ballot(h2) : 0, m3 = rf.w != kEmpty ? __ballot(h3) : 0;
+                // entry distance of the first lane hitting each child (wave-uniform)
+                float e0, e1, e2, e3;
+                e0 = m0 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t0), __ffsll((unsigned long long)m0) - 1)) : INFINITY;
+                e1 = m1 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t1), __ffsll((unsigned long long)m1) - 1)) : INFINITY;
+                e2 = m2 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t2), __ffsll((unsigned long long)m2) - 1)) : INFINITY;
+                e3 = m3 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t3), __ffsll((unsigned long long)m3) - 1)) : INFINITY;
+                const int nh = (m0 != 0) + (m1 != 0) + (m2 != 0) + (m3 != 0);
+                if (nh == 0) {
+                    cur = sp == 0 ? kDone : __builtin_amdgcn_readfirstlane(st[--sp]);
+                } else if (sp + nh - 1 > MAXS) {
+                    if (lane == 0) atomicOr(a.err, 1);
+                    break;
+                } else {
+                    int k0 = 0, k1 = 1, k2 = 2, k3 = 3;
+                    int r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+                    cmpswap(e0, k0, r0, e1, k1, r1);
+                    cmpswap(e2, k2, r2, e3, k3, r3);
+                    cmpswap(e0, k0, r0, e2, k2, r2);
+                    cmpswap(e1, k1, r1, e3, k3, r3);
+                    cmpswap(e1, k1, r1, e2, k2, r2);
+                    if (nh > 3) {
+                        if (lane == 0) st[sp] = r3;
+                        ++sp;
+                    }
+                    if (nh > 2) {
+                        if (lane == 0) st[sp] = r2;
+                        ++sp;
+                    }
+                    if (nh > 1) {
+                        if (lane == 0) st[sp] = r1;
+                        ++sp;
+                    }
+                    cur = __builtin_amdgcn_readfirstlane(r0);
+                }
+            } else {
+                const int e = ~cur;
+                const int first = e & ((1 << kLeafShift) - 1);
+                const int last = first + ((e >> kLeafShift) & 7);
+                for (int s0 = first; s0 <= last; s0 += 2) {
+                    const int s1 = s0 + 1 <= last ? s0 + 1 : s0;
+                    const float4 a0 = geo[3 * (size_t)s0], a1 = geo[3 * (size_t)s0 + 1], a2 = geo[3 * (size_t)s0 + 2];
+                    const float4 b0 = geo[3 * (size_t)s1], b1 = geo[3 * (size_t)s1 + 1], b2 = geo[3 * (size_t)s1 + 2];
+                    if (COUNT) tests += s1 != s0 ? 2 : 1;
+                    float ta, ua, va, tb, ub, vb;
+                    const bool ha = moller_trumbore(o, d, f3(a0.x, a0.y, a0.z), f3(a0.w, a1.x, a1.y),
+                                                    f3(a1.z, a1.w, a2.x), ta, ua, va);
+                    const bool hb = moller_trumbore(o, d, f3(b0.x, b0.y, b0.z), f3(b0.w, b1.x, b1.y),
+                                                    f3(b1.z, b1.w, b2.x), tb, ub, vb);
+                    const int oa = __float_as_int(a2.y), ob = __float_as_int(b2.y);
+                    if (ha && (ta < best_t || (ta == best_t && oa < best_orig))) {
+                        best_t = ta;
+                        best_orig = oa;
+                        best_slot = s0;
+                        bu = ua;
+                        bv = va;
+                    }
+                    if (hb && (tb < best_t || (tb == best_t && ob < best_orig))) {
+                        best_t = tb;
+                        best_orig = ob;
+                        best_slot = s1;
+                        bu = ub;
+                        bv = vb;
+                    }
+                }
+                cur = sp == 0 ? kDone : __builtin_amdgcn_readfirstlane(st[--sp]);
+            }
+        }
+        a.hit_out[my] = valid ? make_float4(best_t, __int_as_float(best_slot), bu, bv)
+                              : make_float4(INFINITY, __int_as_float(-1), 0.0f, 0.0f);
+    }
+    if (COUNT && lane == 0) {
+        atomicAdd(&a.stats[0], visits);
+        atomicAdd(&a.stats[1], tests);
     }
 }
 

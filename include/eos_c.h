@@ -214,6 +214,9 @@ typedef struct eos_launch_config {
     int32_t stack_lds;        /* per-lane LDS stack entries: 8, 16 or 32 */
     int32_t waves_per_cu;     /* persistent waves per CU launched */
     int32_t refill;           /* idle lanes that trigger a refill, 1..64 */
+    int32_t batch;            /* ray batch: most items a wave claims per fetch, 1..4096 */
+    int32_t primary_packets;  /* 1: primary rays walk the 4-wide tree as 8x8-pixel packets, one per wave
+                                 (pays off when triangles cover many pixels; 0, the default, walks per ray) */
 } eos_launch_config;
 
 eos_status eos_set_launch_config(eos_scene* scene, const eos_launch_config* cfg);

@@ -109,6 +109,14 @@ void orc_lbvh_trace(const float* nodes, int32_t root_ref, const float* tris, int
                     const float* orig, const float* dirs, int32_t any_hit, int32_t* hit_orig,
                     float* hit_t, float* hit_u, float* hit_v, int64_t* visits, int64_t* tests,
                     int32_t width);
+/* Primary rays of a frame walked as 8x8-pixel packets over the 4-wide tree, as
+ * the GPU's packet kernel does: one shared walk per tile, every child some
+ * valid lane hits is visited, children ordered by (entry distance of the
+ * lowest-index lane hitting them, slot).  Per pixel [H][W]: hit_orig (-1 miss),
+ * t/u/v (may be NULL); visits/tests accumulate per-tile totals. */
+void orc_lbvh4_packet_primary(const float* nodes4, int32_t root4, const float* tris, const orc_camera* cam,
+                              int32_t* hit_orig, float* hit_t, float* hit_u, float* hit_v, int64_t* visits,
+                              int64_t* tests);
 /* Whitted render driven by the restated GPU traversal; fills rgba/rgb/hit_ids like
  * orc_render and counts node visits / triangle tests in counts. */
 int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, const float* tris,

@@ -888,6 +888,101 @@ void orc_lbvh_trace(const float* nodes, int32_t root_ref, const float* tris, int
     if (tests) *tests += ts;
 }
 
+void orc_lbvh4_packet_primary(const float* nodes4, int32_t root4, const float* tris, const orc_camera* cam,
+                              int32_t* hit_orig, float* hit_t, float* hit_u, float* hit_v, int64_t* visits,
+                              int64_t* tests) {
+    const int W = cam->width, H = cam->height;
+    const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
+    const int n_tiles = tiles_x * tiles_y;
+    const V3 o = ld3(cam->position);
+    int64_t vs = 0, ts = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : vs, ts)
+    for (int tile = 0; tile < n_tiles; ++tile) {
+        const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+        LRay r[64];
+        V3 dir[64];
+        bool valid[64];
+        LHit h[64];
+        for (int i = 0; i < 64; ++i) {
+            const int x = tx * 8 + (i & 7), y = ty * 8 + (i >> 3);
+            valid[i] = x < W && y < H;
+            dir[i] = camera_dir(*cam, x, y);
+            r[i] = make_lray(o, dir[i]);
+        }
+        int32_t stack[96];
+        int sp = 0;
+        int32_t cur = root4;
+        while (cur != 0x7FFFFFFF) {
+            if (cur >= 0) {
+                vs++;
+                const float* nd = nodes4 + 32 * (size_t)cur;
+                float e[4];
+                int kk[4], rk[4];
+                int nh = 0;
+                for (int k = 0; k < 4; ++k) {
+                    int32_t ref;
+                    std::memcpy(&ref, &nd[24 + k], 4);
+                    e[k] = INFINITY;
+                    kk[k] = k;
+                    rk[k] = ref;
+                    if (ref == 0x7FFFFFFE) continue;
+                    for (int i = 0; i < 64; ++i) {
+                        float tm;
+                        if (valid[i] && slab(r[i], nd[0 + k], nd[4 + k], nd[8 + k], nd[12 + k], nd[16 + k],
+                                             nd[20 + k], h[i].t, tm)) {
+                            e[k] = tm;   // the lowest-index lane that hits
+                            ++nh;
+                            break;
+                        }
+                    }
+                }
+                for (int a = 1; a < 4; ++a)
+                    for (int b = a; b > 0 && (e[b] < e[b - 1] || (e[b] == e[b - 1] && kk[b] < kk[b - 1])); --b) {
+                        std::swap(e[b], e[b - 1]); std::swap(kk[b], kk[b - 1]); std::swap(rk[b], rk[b - 1]);
+                    }
+                if (nh == 0) {
+                    cur = sp == 0 ? 0x7FFFFFFF : stack[--sp];
+                } else {
+                    for (int a = nh - 1; a >= 1; --a) stack[sp++] = rk[a];
+                    cur = rk[0];
+                }
+            } else {
+                const int32_t ee = ~cur;
+                const int32_t first = ee & 0x0FFFFFFF, last = first + ((ee >> 28) & 7);
+                for (int32_t s0 = first; s0 <= last; s0 += 2) {
+                    const int32_t s1 = s0 + 1 <= last ? s0 + 1 : s0;
+                    ts += s1 != s0 ? 2 : 1;
+                    for (int32_t s : {s0, s1}) {
+                        const float* tr = tris + 12 * (size_t)s;
+                        const V3 v1 = {tr[0], tr[1], tr[2]}, e1 = {tr[3], tr[4], tr[5]}, e2 = {tr[6], tr[7], tr[8]};
+                        int32_t orig;
+                        std::memcpy(&orig, &tr[9], 4);
+                        for (int i = 0; i < 64; ++i) {
+                            float t, u, v;
+                            if (mt_e(o, dir[i], v1, e1, e2, t, u, v) &&
+                                (t < h[i].t || (t == h[i].t && orig < h[i].orig))) {
+                                h[i].orig = orig; h[i].slot = s; h[i].t = t; h[i].u = u; h[i].v = v;
+                            }
+                        }
+                    }
+                }
+                cur = sp == 0 ? 0x7FFFFFFF : stack[--sp];
+            }
+        }
+        for (int i = 0; i < 64; ++i) {
+            if (!valid[i]) continue;
+            const int x = tx * 8 + (i & 7), y = ty * 8 + (i >> 3);
+            const size_t p = (size_t)y * W + x;
+            if (hit_orig) hit_orig[p] = h[i].orig;
+            if (hit_t) hit_t[p] = h[i].t;
+            if (hit_u) hit_u[p] = h[i].u;
+            if (hit_v) hit_v[p] = h[i].v;
+        }
+    }
+    if (visits) *visits += vs;
+    if (tests) *tests += ts;
+}
+
 int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, const float* tris,
                     const orc_camera* cam, int depth, int row_start, int row_step, uint8_t* rgba, float* rgb,
                     int32_t* hit_ids, orc_counts* counts, int32_t width) {

@@ -65,6 +65,7 @@ def lib():
         L.orc_lbvh_build.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_karras.argtypes = [i32, vp, vp, vp, vp, vp]
         L.orc_lbvh_trace.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, i32]
+        L.orc_lbvh4_packet_primary.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_loss64.restype = C.c_double
         L.orc_loss64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, vp]
         L.orc_grad_fd64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, C.c_double, vp, i32, vp]
@@ -207,6 +208,35 @@ def lbvh_trace(b, origins, dirs, any_hit=False, width=2):
     lib().orc_lbvh_trace(_p(nodes), root, _p(tris), n, _p(o), _p(d), 1 if any_hit else 0,
                          _p(hit), _p(t), _p(u), _p(v), _p(vis), _p(tst), width)
     return {"hit": hit, "t": t, "u": u, "v": v, "visits": int(vis[0]), "tests": int(tst[0])}
+
+
+def packet_primary(scene, b, camera=None):
+    """Primary rays of a frame walked as 8x8 packets over the 4-wide tree (the
+    GPU packet kernel's order): per-pixel hits and per-tile visit/test totals."""
+    cam = np.array(scene.camera if camera is None else camera, copy=True)
+    W, H = int(cam["width"]), int(cam["height"])
+    hit = np.full((H, W), -1, np.int32)
+    t = np.zeros((H, W), np.float32)
+    u = np.zeros((H, W), np.float32)
+    v = np.zeros((H, W), np.float32)
+    vis = np.zeros(1, np.int64)
+    tst = np.zeros(1, np.int64)
+    nodes = np.ascontiguousarray(b["nodes4"], np.float32)
+    tris = np.ascontiguousarray(b["tris"], np.float32)
+    lib().orc_lbvh4_packet_primary(_p(nodes), b["root4"], _p(tris), cam.ctypes.data, _p(hit), _p(t), _p(u), _p(v),
+                                   _p(vis), _p(tst))
+    return {"hit": hit, "t": t, "u": u, "v": v, "visits": int(vis[0]), "tests": int(tst[0])}
+
+
+def camera_rays(scene, camera=None):
+    """Camera ray directions [H][W][3] (the reference formula)."""
+    cam = np.array(scene.camera if camera is None else camera, copy=True)
+    W, H = int(cam["width"]), int(cam["height"])
+    out = np.zeros((H, W, 3), np.float32)
+    for y in range(H):
+        for x in range(W):
+            lib().orc_camera_ray(cam.ctypes.data, x, y, out[y, x].ctypes.data)
+    return out
 
 
 def render_lbvh(scene, b, camera=None, depth=None, row_start=0, row_step=1, width=2):

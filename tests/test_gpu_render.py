@@ -66,20 +66,58 @@ def test_render_parity(name, width):
     assert st["hits"] == oc["hits"]
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width,packets", [(2, 0), (4, 0), (4, 1)])
 @pytest.mark.parametrize("leaf_max", [1, 4])
-def test_visit_counts_match_restatement(leaf_max, width):
+def test_visit_counts_match_restatement(leaf_max, width, packets):
+    """Node visits and triangle tests of the counting frame equal the CPU
+    restatement exactly: per ray for every ray, or, with primary packets, per
+    8x8 tile for the primary rays and per ray for the others."""
     from eos_amd import _lib as E
     s = scenes.config_c(96, 64, nu=96, nv=48)
-    print(f"visit counts leaf_max={leaf_max} width={width}", flush=True)
+    print(f"visit counts leaf_max={leaf_max} width={width} packets={packets}", flush=True)
     g = E.Scene(s, leaf_max=leaf_max, width=width)
+    g.set_launch_config(primary_packets=packets)
     go = g.render(count_visits=True, want_hits=True)
     b = po.lbvh_build(s.verts, leaf_max)
     oo = po.render_lbvh(s, b, width=width)
     assert np.array_equal(go["hit"], oo["hit"])
     assert np.array_equal(go["rgba"], oo["rgba"]) or np.abs(go["rgba"].astype(int) - oo["rgba"]).max() <= 1
-    assert go["stats"]["node_visits"] == oo["counts"]["node_visits"]
-    assert go["stats"]["tri_tests"] == oo["counts"]["tri_tests"]
+    want_v, want_t = oo["counts"]["node_visits"], oo["counts"]["tri_tests"]
+    if packets:
+        dirs = po.camera_rays(s).reshape(-1, 3)
+        orig = np.tile(np.asarray(s.camera["position"], np.float32).reshape(1, 3), (len(dirs), 1))
+        per_ray = po.lbvh_trace(b, orig, dirs, width=4)
+        pk = po.packet_primary(s, b)
+        assert np.array_equal(pk["hit"], oo["hit"])
+        assert go["stats"]["closest_visits_first"] == pk["visits"]
+        assert go["stats"]["closest_tests_first"] == pk["tests"]
+        want_v += pk["visits"] - per_ray["visits"]
+        want_t += pk["tests"] - per_ray["tests"]
+    assert go["stats"]["node_visits"] == want_v
+    assert go["stats"]["tri_tests"] == want_t
+
+
+PACKET_CASES = {
+    "config_a_odd": lambda: scenes.config_a(37, 23),
+    "soup_20k": lambda: scenes.config_b(100, 70, n=20000),
+    "sphere_e_like": lambda: scenes.config_e(72, 40, nu=100, nv=50, n_materials=16, n_lights=8),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PACKET_CASES))
+def test_primary_packets_equal_per_ray_walk(name):
+    """Primary packets and the per-ray walk give bit-identical frames."""
+    from eos_amd import _lib as E
+    s = PACKET_CASES[name]()
+    print(f"packets vs per-ray case={name}", flush=True)
+    g = E.Scene(s, leaf_max=4)
+    g.set_launch_config(primary_packets=1)
+    a = g.render(want_rgb=True, want_hits=True)
+    g.set_launch_config(primary_packets=0)
+    b = g.render(want_rgb=True, want_hits=True)
+    assert np.array_equal(a["hit"], b["hit"])
+    assert np.array_equal(a["rgba"], b["rgba"])
+    assert np.array_equal(a["rgb"].view(np.uint32), b["rgb"].view(np.uint32))
 
 
 @pytest.mark.parametrize("wg,stack", [(64, 8), (64, 32), (128, 16), (256, 8), (256, 16), (256, 32)])
