@@ -24,6 +24,7 @@ constexpr int kSortThreads = 256;
 constexpr int kSortChunks = 16;                      // keys per block = 4096
 constexpr int kSortTile = kSortThreads * kSortChunks;
 constexpr int kWaves = kSortThreads / 64;
+constexpr int kRefitPasses = 64;                     // >= height of a Karras tree over 30-bit codes + index
 
 __device__ __forceinline__ uint32_t part1by2(uint32_t x) {
     x &= 0x000003ffu;
@@ -120,28 +121,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of `total` counters in place (digit-major, then block), one block.
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* __restrict__ hist, int total) {
-    __shared__ uint32_t s[1024];
-    const int per = (total + 1023) / 1024;
-    const int b = threadIdx.x * per, e = min(total, b + per);
-    uint32_t sum = 0;
-    for (int i = b; i < e; ++i) sum += hist[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = s[threadIdx.x] - sum;  // exclusive prefix of this thread's range
-    for (int i = b; i < e; ++i) {
-        uint32_t v = hist[i];
-        hist[i] = run;
-        run += v;
-    }
-}
+// The digit-major histogram is scanned with the multi-block scan below.
 
 // Stable scatter: keys are taken in order, 256 per step; lanes holding the same
 // digit are matched with eight ballots, ranked by lane, and waves are ordered
@@ -238,16 +218,14 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int n, int32_t v) {
     if (i < n) p[i] = v;
 }
 
-__device__ __forceinline__ float ld_agent(const float* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__global__ void k_copy_i32(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
-// One thread per leaf: write the leaf box, then climb; the second child to
-// arrive at a node computes its box (min/max are exact, so arrival order does
-// not change the result).
-__global__ __launch_bounds__(256) void k_refit(const float* __restrict__ verts, const int32_t* __restrict__ perm, int n,
-                                               const int2* __restrict__ kchild, const int32_t* __restrict__ kparent,
-                                               float* lbox, float* ibox, int32_t* flags) {
+// Leaf boxes: exact min/max of the three vertices, one thread per sorted slot.
+__global__ __launch_bounds__(256) void k_leaf_boxes(const float* __restrict__ verts, const int32_t* __restrict__ perm,
+                                                    int n, float* __restrict__ lbox) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const float* t = verts + 9 * (size_t)perm[s];
@@ -256,22 +234,35 @@ __global__ __launch_bounds__(256) void k_refit(const float* __restrict__ verts, 
         lbox[6 * (size_t)s + k] = fminf(fminf(t[k], t[3 + k]), t[6 + k]);
         lbox[6 * (size_t)s + 3 + k] = fmaxf(fmaxf(t[k], t[3 + k]), t[6 + k]);
     }
-    int p = kparent[n - 1 + s];
-    while (p >= 0) {
-        int old = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;
-        const int2 c = kchild[p];
-        const float* b0 = c.x >= 0 ? ibox + 6 * (size_t)c.x : lbox + 6 * (size_t)~c.x;
-        const float* b1 = c.y >= 0 ? ibox + 6 * (size_t)c.y : lbox + 6 * (size_t)~c.y;
-        float r[6];
+}
+
+// One level-synchronous refit pass over the internal nodes.  done[i] is the
+// pass that boxed node i (0: not yet).  A node is boxed once both children are
+// leaves or were boxed in an EARLIER pass, so every box it reads was written by
+// an earlier kernel: no cross-workgroup ordering inside a launch is needed
+// (device-scope release/acquire per climbing thread costs L2 writebacks on a
+// multi-die part).  Passes = tree height; min/max are exact, so the result
+// does not depend on the schedule.  Once the root (node 0) is boxed every node
+// is, and the remaining passes return at once.
+__global__ __launch_bounds__(256) void k_refit_pass(const int2* __restrict__ kchild, int m, int pass,
+                                                    const float* __restrict__ lbox, float* __restrict__ ibox,
+                                                    int32_t* __restrict__ done) {
+    if (done[0] != 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m && done[i] == 0) {
+        const int2 c = kchild[i];
+        const int d0 = c.x >= 0 ? done[c.x] : -1, d1 = c.y >= 0 ? done[c.y] : -1;
+        const bool r0 = c.x < 0 || (d0 > 0 && d0 < pass), r1 = c.y < 0 || (d1 > 0 && d1 < pass);
+        if (r0 && r1) {
+            const float* b0 = c.x >= 0 ? ibox + 6 * (size_t)c.x : lbox + 6 * (size_t)~c.x;
+            const float* b1 = c.y >= 0 ? ibox + 6 * (size_t)c.y : lbox + 6 * (size_t)~c.y;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            r[k] = fminf(ld_agent(b0 + k), ld_agent(b1 + k));
-            r[3 + k] = fmaxf(ld_agent(b0 + 3 + k), ld_agent(b1 + 3 + k));
+            for (int k = 0; k < 3; ++k) {
+                ibox[6 * (size_t)i + k] = fminf(b0[k], b1[k]);
+                ibox[6 * (size_t)i + 3 + k] = fmaxf(b0[3 + k], b1[3 + k]);
+            }
+            done[i] = pass;
         }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) __hip_atomic_store(ibox + 6 * (size_t)p + k, r[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        p = kparent[p];
     }
 }
 
@@ -508,7 +499,8 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st
     EOS_TRY(s->perm.ensure(4 * un));
     EOS_TRY(s->sort_tmp_k.ensure(4 * un));
     EOS_TRY(s->sort_tmp_v.ensure(4 * un));
-    EOS_TRY(s->sort_hist.ensure(4 * 256 * (size_t)nblocks));
+    EOS_TRY(s->sort_hist.ensure(2 * 4 * 256 * (size_t)nblocks));
+    EOS_TRY(s->scan_tmp.ensure(4 * blocks(256 * (size_t)nblocks, kScanBlock) + 4 * blocks(un, kScanBlock) + 8));
     EOS_TRY(s->kchild.ensure(8 * m));
     EOS_TRY(s->kparent.ensure(4 * (2 * un)));
     EOS_TRY(s->first_last.ensure(8 * m));
@@ -522,7 +514,8 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st
     EOS_TRY(s->shade.ensure(48 * un));
     EOS_HIP(hipEventRecord(s->ev[0], S));
     int32_t* err = s->cbounds.as<int32_t>() + 8;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S, err, 1, 0);
+    int32_t* root_pass = err + 1;   // refit pass that boxed the root (0: none)
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S, err, 2, 0);
     if (n > 0) {
         const float* V = s->verts.as<float>();
         float* cb = s->cbounds.as<float>();
@@ -534,6 +527,9 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st
         int32_t* vB = s->sort_tmp_v.as<int32_t>();
         hipLaunchKernelGGL(k_morton, dim3(blocks(un, 256)), dim3(256), 0, S, V, n, cb, s->codes.as<uint32_t>(), kA, vA);
         uint32_t* H = s->sort_hist.as<uint32_t>();
+        uint32_t* H2 = H + 256 * (size_t)nblocks;
+        const int htot = 256 * nblocks, hb = (int)blocks((size_t)htot, kScanBlock);
+        int32_t* hsums = s->scan_tmp.as<int32_t>();
         for (int pass = 0; pass < 4; ++pass) {
             const int shift = 8 * pass;
             const uint32_t* ki = (pass & 1) ? kB : kA;
@@ -541,8 +537,13 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st
             uint32_t* ko = (pass & 1) ? kA : kB;
             int32_t* vo = (pass & 1) ? vA : vB;
             hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, S, ki, n, shift, H, nblocks);
-            hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, S, H, 256 * nblocks);
-            hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, S, ki, vi, ko, vo, n, shift, H,
+            // exclusive scan of the digit-major histogram into H2 (counts < 2^31)
+            hipLaunchKernelGGL(k_scan_blocks, dim3(hb), dim3(256), 0, S, reinterpret_cast<const int32_t*>(H),
+                               reinterpret_cast<int32_t*>(H2), htot, hsums);
+            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, S, hsums, hb);
+            hipLaunchKernelGGL(k_scan_add, dim3(blocks(htot, 256)), dim3(256), 0, S, reinterpret_cast<int32_t*>(H2),
+                               htot, hsums);
+            hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, S, ki, vi, ko, vo, n, shift, H2,
                                nblocks);
         }
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks(2 * un, 256)), dim3(256), 0, S, s->kparent.as<int32_t>(),
@@ -552,9 +553,15 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st
                                s->kparent.as<int32_t>(), s->first_last.as<int2>());
             hipLaunchKernelGGL(k_fill_i32, dim3(blocks(m, 256)), dim3(256), 0, S, s->flags.as<int32_t>(), (int)m, 0);
         }
-        hipLaunchKernelGGL(k_refit, dim3(blocks(un, 256)), dim3(256), 0, S, V, vA, n, s->kchild.as<const int2>(),
-                           s->kparent.as<const int32_t>(), s->lbox.as<float>(), s->ibox.as<float>(),
-                           s->flags.as<int32_t>());
+        hipLaunchKernelGGL(k_leaf_boxes, dim3(blocks(un, 256)), dim3(256), 0, S, V, vA, n, s->lbox.as<float>());
+        if (n > 1) {
+            for (int pass = 1; pass <= kRefitPasses; ++pass)
+                hipLaunchKernelGGL(k_refit_pass, dim3(blocks(m, 256)), dim3(256), 0, S, s->kchild.as<const int2>(),
+                                   (int)m, pass, s->lbox.as<const float>(), s->ibox.as<float>(),
+                                   s->flags.as<int32_t>());
+            // the root's pass number, 0 if the passes did not reach it
+            hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(64), 0, S, s->flags.as<const int32_t>(), root_pass, 1);
+        }
         if (n > 1)
             hipLaunchKernelGGL(k_nodes, dim3(blocks(m, 256)), dim3(256), 0, S, s->kchild.as<const int2>(),
                                s->first_last.as<const int2>(), s->ibox.as<const float>(), s->lbox.as<const float>(),
@@ -588,11 +595,12 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st
     }
     EOS_HIP(hipGetLastError());
     EOS_HIP(hipEventRecord(s->ev[1], S));
-    int32_t herr = 0;
-    EOS_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, S));
+    int32_t herr[2] = {0, 0};
+    EOS_HIP(hipMemcpyAsync(herr, err, 8, hipMemcpyDeviceToHost, S));
     EOS_HIP(hipStreamSynchronize(S));
-    if (herr != 0)
-        return fail(EOS_ERR_INVALID, "eos_build_lbvh: " + std::to_string(herr) + " material indices out of range");
+    if (herr[0] != 0)
+        return fail(EOS_ERR_INVALID, "eos_build_lbvh: " + std::to_string(herr[0]) + " material indices out of range");
+    if (n > 1 && herr[1] == 0) return fail(EOS_ERR_STATE, "eos_build_lbvh: refit did not reach the root");
     float ms = 0.0f;
     EOS_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
     if (n == 0) s->root = kDone;

@@ -217,7 +217,7 @@ template <int WIDTH>
 constexpr int max_stack() { return WIDTH == 4 ? kMaxStack4 : kMaxStack; }
 
 template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH, int WIDTH>
-__global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6))) void k_trace(TraceArgs a) {
     constexpr bool ANY = MODE == MODE_SHADOW;
     constexpr int MAXS = max_stack<WIDTH>();
     __shared__ int32_t lds[STACK_LDS * WG];
@@ -374,9 +374,12 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
             if (drained && chunk_next >= chunk_end) break;
             continue;
         }
-        // ---- traverse
-        if (item >= 0) {
-            for (int step = 0; step < 32; ++step) {
+        // ---- traverse until `thresh` lanes are idle (all of them once no work is
+        // left to claim), so finished lanes get new items without waiting for the
+        // longest ray in the wave
+        const int need = (drained && chunk_next >= chunk_end) ? 64 : thresh;
+        for (;;) {
+            if (item >= 0) {
                 if (cur >= 0 && cur != kDone && --budget < 0) {
                     atomicOr(a.err, 2);
                     cur = kDone;
@@ -510,9 +513,9 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                         a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
                     }
                     item = -1;
-                    break;
                 }
             }
+            if (__popcll(__ballot(item < 0)) >= need) break;
         }
     }
     if (COUNT) {
