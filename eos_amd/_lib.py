@@ -21,7 +21,7 @@ DBG = {"morton_sorted": 1, "perm": 2, "kchild": 3, "kparent": 4, "ibox": 5, "nod
 EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_build_lbvh_ex", "eos_update_materials",
            "eos_update_lights", "eos_render", "eos_render_adjoint", "eos_render_batch",
            "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
-           "eos_abi_version", "eos_device_name"]
+           "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free"]
 
 
 class EosError(RuntimeError):
@@ -81,9 +81,14 @@ class BatchStats(C.Structure):
     _fields_ = [("render_ms", C.c_double), ("rays", C.c_int64)]
 
 
+class Mesh(C.Structure):
+    _fields_ = [("num_triangles", C.c_int32), ("vertices", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float))]
+
+
 class LaunchConfig(C.Structure):
     _fields_ = [("workgroup", C.c_int32), ("stack_lds", C.c_int32), ("waves_per_cu", C.c_int32),
-                ("refill", C.c_int32), ("batch", C.c_int32), ("primary_packets", C.c_int32)]
+                ("refill", C.c_int32), ("batch", C.c_int32), ("primary_packets", C.c_int32), ("coop", C.c_int32)]
 
 
 _LIB = None
@@ -120,6 +125,8 @@ def lib():
         L.eos_last_error.restype = C.c_char_p
         L.eos_abi_version.restype = i32
         L.eos_device_name.argtypes = [C.c_char_p, i32]
+        L.eos_mesh_load_obj.argtypes = [C.c_char_p, vp]
+        L.eos_mesh_free.argtypes = [vp]
         for name in EXPORTS:
             if name != "eos_last_error" and name != "eos_abi_version":
                 getattr(L, name).restype = C.c_int
@@ -136,6 +143,20 @@ def device_name() -> str:
     buf = C.create_string_buffer(256)
     check(lib().eos_device_name(buf, 256))
     return buf.value.decode()
+
+
+def load_obj(path: str):
+    """Triangles of a Wavefront OBJ file (eos_mesh_load_obj): vertices and
+    per-corner normals, float32 [N][3][3] each."""
+    m = Mesh()
+    check(lib().eos_mesh_load_obj(os.fsencode(path), C.byref(m)))
+    try:
+        n = m.num_triangles
+        v = np.ctypeslib.as_array(m.vertices, shape=(n * 9,)).reshape(n, 3, 3).copy() if n else np.zeros((0, 3, 3), np.float32)
+        nn = np.ctypeslib.as_array(m.normals, shape=(n * 9,)).reshape(n, 3, 3).copy() if n else np.zeros((0, 3, 3), np.float32)
+    finally:
+        lib().eos_mesh_free(C.byref(m))
+    return v, nn
 
 
 def _ptr(a):
@@ -192,7 +213,7 @@ class Scene:
         check(lib().eos_update_lights(self.h, _ptr(self._lights), len(self._lights)))
 
     DEFAULT_LAUNCH = {"workgroup": 256, "stack_lds": 16, "waves_per_cu": 20, "refill": 16, "batch": 128,
-                      "primary_packets": 0}
+                      "primary_packets": 0, "coop": 3}
 
     def set_launch_config(self, *args, **kw):
         """Positional fields in eos_launch_config order, or keywords; fields not

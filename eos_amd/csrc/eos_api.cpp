@@ -271,12 +271,14 @@ eos_status eos_set_launch_config(eos_scene* s, const eos_launch_config* c) {
     if (c->batch < 1 || c->batch > 4096) return fail(EOS_ERR_INVALID, "batch must be in [1, 4096]");
     if (c->primary_packets != 0 && c->primary_packets != 1)
         return fail(EOS_ERR_INVALID, "primary_packets must be 0 or 1");
+    if (c->coop < 0 || c->coop > 4) return fail(EOS_ERR_INVALID, "coop must be in [0, 4]");
     s->cfg.workgroup = c->workgroup;
     s->cfg.stack_lds = c->stack_lds;
     s->cfg.waves_per_cu = c->waves_per_cu;
     s->cfg.refill = c->refill;
     s->cfg.batch = c->batch;
     s->cfg.packets = c->primary_packets;
+    s->cfg.coop = c->coop;
     return EOS_OK;
 }
 
@@ -288,6 +290,7 @@ eos_status eos_get_launch_config(const eos_scene* s, eos_launch_config* c) {
     c->refill = s->cfg.refill;
     c->batch = s->cfg.batch;
     c->primary_packets = s->cfg.packets;
+    c->coop = s->cfg.coop;
     return EOS_OK;
 }
 

@@ -69,6 +69,43 @@ bool Scene::addSurface(Surface* s) {
     return true;
 }
 
+bool Scene::addTriangleMesh(TriangleMesh* mesh) {
+    if (!mesh) return false;
+    const int start = getNumSurfaces();
+    for (unsigned int i = 0; i < mesh->getNumTriangles(); ++i) {
+        Triangle* t = mesh->getTriangle((int)i);
+        if (t) m_Surfaces.push_back(t);
+    }
+    mesh->setSceneStartIndex(start);
+    mesh->setSceneEndIndex(getNumSurfaces() - start);
+    return true;
+}
+
+// ---------------------------------------------------------------- TriangleMesh
+TriangleMesh::~TriangleMesh() {
+    for (Triangle* t : m_Triangles) delete t;
+}
+
+bool TriangleMesh::loadFromFile(const char* filename) {
+    eos_mesh m;
+    if (eos_mesh_load_obj(filename, &m) != EOS_OK) return false;
+    for (Triangle* t : m_Triangles) delete t;
+    m_Triangles.clear();
+    m_Triangles.reserve((size_t)m.num_triangles);
+    for (int i = 0; i < m.num_triangles; ++i) {
+        const float* v = m.vertices + 9 * (size_t)i;
+        const float* n = m.normals + 9 * (size_t)i;
+        Triangle* t = new Triangle(eos::vec3(v[0], v[1], v[2]), eos::vec3(v[3], v[4], v[5]), eos::vec3(v[6], v[7], v[8]),
+                                   eos::vec3(n[0], n[1], n[2]), eos::vec3(n[3], n[4], n[5]), eos::vec3(n[6], n[7], n[8]));
+        t->setMaterial(m_Material);
+        t->setMaterialIndex(m_MaterialIndex);
+        t->setTransformationIndex(m_TransformationIndex);
+        m_Triangles.push_back(t);
+    }
+    eos_mesh_free(&m);
+    return true;
+}
+
 bool Scene::addLightSource(LightSource* l) {
     if (!l) return false;
     m_Lights.push_back(l);

@@ -339,7 +339,7 @@ eos_status ensure_buffers(eos_frame_buffers& fb, const Frame& f, int trace_grid)
     EOS_TRY(fb.vis.ensure(fb.vis_stride() * (size_t)(f.depth + 1)));
     EOS_TRY(fb.levcol.ensure(16 * f.npix * (size_t)(f.depth + 1)));
     EOS_TRY(fb.levels.ensure(f.npix));
-    EOS_TRY(fb.counters.ensure(4 * CTR_COUNT + 8 * kStats));
+    EOS_TRY(fb.counters.ensure(4 * (size_t)kCounterInts));
     EOS_TRY(fb.gstack.ensure(2 * gstack_ints(f, trace_grid) * 4));
     fb.npix = f.npix;
     fb.n_ids = f.n_ids;
@@ -364,7 +364,9 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         if (E) (void)hipEventRecord(E[ev], st);
         return ev++;
     };
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, S, ctr, CTR_COUNT + 2 * kStats);
+    hipLaunchKernelGGL(k_zero_i32, dim3(16), dim3(256), 0, S, ctr, kCounterInts);
+    // fetch cursors of launch slot k (closest level l: k = l; any hit from level l: k = 16 + l)
+    auto cursors = [&](int k) { return ctr + kCursorBase + k * kRanges * kCursorStride; };
     FrameEvents local;
     FrameEvents& F = fe ? *fe : local;
     F.start = rec(S);
@@ -382,7 +384,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             sh.err = ctr + CTR_ERR;
             sh.refill = f.cfg.refill;
             sh.batch = f.cfg.batch;
-            sh.cursor = ctr + CTR_CUR_S + l0;
+            sh.coop = f.cfg.coop;
+            sh.cursor = cursors(16 + l0);
             sh.stats = stats + 2 * (16 + l0);
             sh.count_ptr = ctr + CTR_HITS + l0;
             sh.count_mul = f.lstride;
@@ -426,7 +429,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         ta.err = ctr + CTR_ERR;
         ta.refill = f.cfg.refill;
         ta.batch = f.cfg.batch;
-        ta.cursor = ctr + CTR_CUR_C + l;
+        ta.coop = f.cfg.coop;
+        ta.cursor = cursors(l);
         ta.stats = stats + 2 * l;
         if (l == 0) {
             ta.n_static = (int)f.n_ids;
@@ -653,7 +657,8 @@ bool same_batch(eos_scene* const* scenes, const eos_camera* cams, int n, int dep
     const BatchState& b = g_batch;
     if ((int)b.scenes.size() != n || b.depth != depth) return false;
     if (b.cfg.workgroup != cfg.workgroup || b.cfg.stack_lds != cfg.stack_lds || b.cfg.waves_per_cu != cfg.waves_per_cu ||
-        b.cfg.refill != cfg.refill || b.cfg.batch != cfg.batch || b.cfg.packets != cfg.packets)
+        b.cfg.refill != cfg.refill || b.cfg.batch != cfg.batch || b.cfg.packets != cfg.packets ||
+        b.cfg.coop != cfg.coop)
         return false;
     for (int i = 0; i < n; ++i) {
         if (b.scenes[i] != scenes[i]) return false;

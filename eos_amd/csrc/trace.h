@@ -127,8 +127,6 @@ __device__ __forceinline__ void wave_sum_add(int* ctr, int v) {
 enum : int {
     CTR_Q = 0,         // + level: rays queued for level (level >= 1)
     CTR_HITS = 16,     // + level: hits at level
-    CTR_CUR_C = 32,    // + level: fetch cursor of the closest-hit launch
-    CTR_CUR_S = 48,    // + level: fetch cursor of the any-hit launch
     CTR_ERR = 64,      // stack overflow / invalid state
     CTR_SHADOW = 65,   // shadow rays (sum over hits of the hit scene's light count)
     CTR_COUNT = 128
@@ -136,6 +134,17 @@ enum : int {
 // int64 statistics (count_visits): [2*launch] visits, [2*launch+1] tests;
 // launch = level (closest) or 16 + level (any hit).
 constexpr int kStats = 64;
+// Fetch cursors, after the statistics: per launch (same numbering) one cursor
+// per item range, each on its own 128-byte line.  A launch's items are split
+// into kRanges contiguous ranges, range r served first to the waves of
+// workgroups with blockIdx % kRanges == r: workgroups are dispatched to the
+// eight XCDs round robin, so each XCD walks one band of the item space (its own
+// L2 holds that band's nodes) and the chunk claims spread over eight counters.
+constexpr int kRanges = 8;
+constexpr int kCursorStride = 32;                         // ints per cursor (128 B)
+constexpr int kCursorBase = CTR_COUNT + 2 * kStats;       // int offset of the cursors
+constexpr int kCursorInts = 32 * kRanges * kCursorStride; // 32 launches
+constexpr int kCounterInts = kCursorBase + kCursorInts;
 
 enum TraceMode : int { MODE_PRIMARY = 0, MODE_QUEUE = 1, MODE_SHADOW = 2 };
 
@@ -175,7 +184,7 @@ struct TraceArgs {
     int n_static;            // item count when count_ptr is null
     const int* count_ptr;    // device item count (queued rays / hits)
     int count_mul;           // items per counted entry (light slots for shadow rays)
-    int* cursor;             // persistent fetch cursor
+    int* cursor;             // fetch cursors of the launch: range r at cursor[r * kCursorStride]
     const float4* q;         // MODE_QUEUE: ray queue
     float4* hit_out;         // closest: (t, slot, u, v)
     const float4* surf;      // MODE_SHADOW: hit surfaces (P.xyz pixel | N.xyz mat) of the first level
@@ -188,21 +197,9 @@ struct TraceArgs {
     unsigned long long* stats;
     int refill;
     int batch;               // most items a wave claims per fetch
+    int coop;                // quad-cooperative walk: 0 never, 1..3 at most lanes/4, /2, /1 items, 4 always
 };
 
-// Persistent traversal.  Each wave keeps up to 64 independent rays in flight;
-// whenever at least `refill` lanes are idle it hands them the next items of its
-// private chunk (ballot + mbcnt ranks the idle lanes), so lanes never wait for
-// the slowest ray of a fixed batch.  A wave claims a new chunk from the fetch
-// cursor with one atomic; chunks shrink as the cursor nears the end (guided
-// self-scheduling: remaining / (2 * waves), between the idle count and the
-// launch's ray batch), which keeps the single device-scope counter far from
-// saturation on cheap rays and still balances the tail.  Per lane: ordered
-// near-first descent, far child pushed onto a short stack in LDS (STACK_LDS
-// entries, column per lane) that overflows into a private global area, pruning
-// against the best hit so far.  Per-ray visit order is deterministic and equal
-// to the CPU restatement in the oracle.  BATCH: every ray looks up its own
-// scene (BVH, triangles, lights, camera) in the batch table.
 // Compare-exchange on (entry distance, slot): a total order, so the result does
 // not depend on the network's shape.
 __device__ __forceinline__ void cmpswap(float& ta, int& ka, int& ra, float& tb, int& kb, int& rb) {
@@ -216,144 +213,404 @@ __device__ __forceinline__ void cmpswap(float& ta, int& ka, int& ra, float& tb, 
 template <int WIDTH>
 constexpr int max_stack() { return WIDTH == 4 ? kMaxStack4 : kMaxStack; }
 
+// Wave-uniform state of the two-level work fetch: a wave claims a chunk of
+// items from a range cursor with one atomic and hands them out to idle lanes
+// (or lane groups) locally.  It starts on its XCD's range and moves on to the
+// next range when that one is used up; after every range it is drained.
+struct Fetch {
+    bool drained = false;   // every range is used up
+    int next = 0, end = 0;  // unissued items of this wave's chunk
+    int seen = 0;           // cursor value of the current range after this wave's last claim
+    int range = 0;          // current range
+    int tried = 0;          // ranges found used up
+};
+
+__device__ __forceinline__ Fetch fetch_start() {
+    Fetch f;
+    f.range = (int)(blockIdx.x % kRanges);
+    return f;
+}
+
+// Hands out items to `nidle` idle units (lanes or lane groups) of the wave;
+// returns the item of the unit ranked `rank` among them (-1: none).  Every lane
+// of the wave calls it with the same fetch state, idle mask and count.  Chunks
+// follow guided self-scheduling within the range: remaining / (2 * waves per
+// range), between the idle count and the launch's ray batch.
+__device__ __forceinline__ int fetch_items(Fetch& f, const TraceArgs& a, int n, int waves_total, uint64_t idle,
+                                           int nidle, int rank) {
+    const int avail = f.end - f.next;
+    const int old_next = f.next;
+    int base = 0, take = 0;
+    if (avail < nidle && !f.drained) {
+        const int want = nidle - avail;
+        const int leader = __ffsll((unsigned long long)idle) - 1;
+        const int wpr = waves_total / kRanges > 0 ? waves_total / kRanges : 1;
+        for (int tries = 0; tries < kRanges; ++tries) {  // wave-uniform
+            const int r = f.range;
+            const int lo = (int)((long long)n * r / kRanges), hi = (int)((long long)n * (r + 1) / kRanges);
+            const int len = hi - lo;
+            int size = (len - f.seen) / (2 * wpr);
+            size = size > a.batch ? a.batch : size;
+            size = size < want ? want : size;
+            int got = 0;
+            if (__lane_id() == leader) got = atomicAdd(a.cursor + r * kCursorStride, size);
+            got = __shfl(got, leader);
+            if (got < len) {
+                f.seen = got + size;
+                base = lo + got;
+                take = (got + size < len ? size : len - got);
+                break;
+            }
+            f.range = (r + 1) % kRanges;
+            f.seen = 0;
+            if (++f.tried == kRanges) {
+                f.drained = true;
+                break;
+            }
+        }
+    }
+    int my = -1;
+    if (rank < avail) my = old_next + rank;
+    else if (rank - avail < take) my = base + (rank - avail);
+    if (avail >= nidle) {
+        f.next = old_next + nidle;
+    } else {
+        const int used = nidle - avail < take ? nidle - avail : take;
+        f.next = take > used ? base + used : 0;
+        f.end = take > used ? base + take : 0;
+    }
+    return my;
+}
+
+// One traversal item, set up: the ray and the scene it runs in.
+struct RayItem {
+    F3 o, d;
+    bool valid;              // false: no ray (pixel outside the frame, light slot beyond the scene's lights)
+    int root, max_visits;
+    const Node* nodes;
+    const Node4* nodes4;
+    const float4* geo;
+    size_t out_idx;          // MODE_SHADOW: visibility byte of the item
+};
+
+template <int MODE, bool BATCH, int WIDTH>
+__device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& r) {
+    r.valid = true;
+    r.nodes = a.sc.nodes;
+    r.nodes4 = a.sc.nodes4;
+    r.geo = a.sc.geo;
+    r.root = WIDTH == 4 ? a.sc.root4 : a.sc.root;
+    r.max_visits = a.sc.n_tri + 64;
+    r.out_idx = 0;
+    auto use_scene = [&](const BatchScene& B) {
+        r.nodes = B.sc.nodes;
+        r.nodes4 = B.sc.nodes4;
+        r.geo = B.sc.geo;
+        r.root = WIDTH == 4 ? B.sc.root4 : B.sc.root;
+        r.max_visits = B.sc.n_tri + 64;
+    };
+    if (MODE == MODE_PRIMARY) {
+        int x, y;
+        if (BATCH) {
+            const BatchScene& B = a.table[scene_of_id(a.table, a.n_scenes, my)];
+            use_scene(B);
+            r.valid = tile_pixel(my - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
+            r.o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
+            r.d = camera_dir(B.cam, x, y);
+        } else {
+            r.valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
+            r.o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
+            r.d = camera_dir(a.cam, x, y);
+        }
+    } else if (MODE == MODE_QUEUE) {
+        const float4 q0 = a.q[2 * (size_t)my], q1 = a.q[2 * (size_t)my + 1];
+        r.o = f3(q0.x, q0.y, q0.z);
+        r.d = f3(q1.x, q1.y, q1.z);
+        if (BATCH) use_scene(a.table[__float_as_int(q1.w)]);
+    } else {
+        // item -> (level, surface j, light slot li)
+        int rem = my, lev = 0;
+        for (int k = 0; k + 1 < a.nlev; ++k) {
+            const int c = a.count_ptr[k] * a.count_mul;
+            if (rem < c) break;
+            rem -= c;
+            lev = k + 1;
+        }
+        r.out_idx = (size_t)lev * (size_t)a.vis_stride + (size_t)rem;
+        const float4* surf = a.surf + (size_t)lev * (size_t)a.surf_stride;
+        const int j = rem / a.count_mul, li = rem - j * a.count_mul;
+        const float4 s0 = surf[2 * (size_t)j];
+        const F3 P = f3(s0.x, s0.y, s0.z);
+        const DevLight* lights = a.sc.lights;
+        int nl = a.sc.n_light;
+        if (BATCH) {
+            const BatchScene& B = a.table[unpack_scene(__float_as_int(surf[2 * (size_t)j + 1].w))];
+            use_scene(B);
+            lights = B.sc.lights;
+            nl = B.sc.n_light;
+        }
+        if (li < nl) {
+            const DevLight& L = lights[li];
+            r.d = normalize3(sub(f3(L.position[0], L.position[1], L.position[2]), P));
+            r.o = add(P, scale(1e-3f, r.d));
+        } else {
+            r.valid = false;
+            r.d = f3(0.0f, 0.0f, 1.0f);
+            r.o = P;
+        }
+    }
+}
+
+// Items per launch at or below which the cooperative walk is used, as a
+// multiple of the launch's lanes / 4: a launch with at most one ray per lane
+// is one round, and its time is the longest ray's chain of steps.
+__device__ __forceinline__ bool use_coop(int mode, long long n, long long lanes) {
+    switch (mode) {
+        case 1: return 4 * n <= lanes;
+        case 2: return 2 * n <= lanes;
+        case 3: return n <= lanes;
+        case 4: return true;
+        default: return false;
+    }
+}
+
+// Cooperative walk of the 4-wide tree: four lanes (a quad) share one ray.  At a
+// node lane k tests child k; the quad ranks the hit children on (entry
+// distance, slot) with quad shuffles, pushes ranks 1.. farthest first onto the
+// quad's stack in LDS and continues with rank 0.  At a leaf lane k tests
+// triangles k and k + 4 of the leaf, and the quad takes the least (t, original
+// index) hit (closest) or the first hit in slot order (any hit).  Every ray
+// visits the same nodes in the same order as the per-lane walk, ends with the
+// same hit and is counted the same way; what changes is the length of the chain
+// of dependent instructions per step, which is what bounds a launch that is one
+// round of rays.
+template <int MODE, int WG, bool COUNT, bool BATCH>
+__device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* wave_stack, unsigned long long& visits,
+                                           unsigned long long& tests) {
+    constexpr bool ANY = MODE == MODE_SHADOW;
+    const int lane = __lane_id();
+    const int sub = lane & 3, grp = lane >> 2;
+    const int gbit = 4 * grp;
+    int32_t* stk = wave_stack + grp * kMaxStack4;
+    const int waves_total = (int)gridDim.x * (WG / 64);
+    int gthresh = (a.refill + 3) / 4;
+    gthresh = gthresh < 1 ? 1 : (gthresh > 16 ? 16 : gthresh);
+    Fetch f = fetch_start();
+    int item = -1;
+    size_t out_idx = 0;
+    F3 o = {0, 0, 0}, d = {0, 0, 0};
+    float ix = 0, iy = 0, iz = 0, px = 0, py = 0, pz = 0;
+    float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
+    int best_slot = -1, best_orig = 0x7FFFFFFF;
+    int cur = kDone, sp = 0, budget = 0;
+    const Node4* nodes4 = a.sc.nodes4;
+    const float4* geo = a.sc.geo;
+    for (;;) {
+        // one bit per idle quad, at its lane 0
+        const uint64_t idle = __ballot(sub == 0 && item < 0);
+        const int nidle = __popcll(idle);
+        if (nidle >= gthresh && (!f.drained || f.next < f.end)) {
+            const int rank = __popcll(idle & ((1ull << gbit) - 1));
+            const int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
+            if (item < 0 && my >= 0) {
+                RayItem r;
+                setup_item<MODE, BATCH, 4>(a, my, r);
+                item = my;
+                out_idx = r.out_idx;
+                o = r.o;
+                d = r.d;
+                nodes4 = r.nodes4;
+                geo = r.geo;
+                ix = safe_inv(d.x);
+                iy = safe_inv(d.y);
+                iz = safe_inv(d.z);
+                px = o.x * ix;
+                py = o.y * iy;
+                pz = o.z * iz;
+                best_t = INFINITY;
+                best_slot = -1;
+                best_orig = 0x7FFFFFFF;
+                bu = bv = 0.0f;
+                sp = 0;
+                budget = r.max_visits;
+                cur = r.valid ? r.root : kDone;
+            }
+        }
+        if (__ballot(item >= 0) == 0) {
+            if (f.drained && f.next >= f.end) break;
+            continue;
+        }
+        const int need = (f.drained && f.next >= f.end) ? 16 : gthresh;
+        for (;;) {
+            if (item >= 0) {  // uniform within a quad from here on
+                if (cur >= 0 && cur != kDone && --budget < 0) {
+                    if (sub == 0) atomicOr(a.err, 2);
+                    cur = kDone;
+                } else if (cur >= 0 && cur != kDone) {
+                    if (COUNT && sub == 0) ++visits;
+                    const float* nd = reinterpret_cast<const float*>(nodes4 + cur);
+                    const float lx = nd[sub], hx = nd[4 + sub], ly = nd[8 + sub], hy = nd[12 + sub],
+                                lz = nd[16 + sub], hz = nd[20 + sub];
+                    const int rf = __float_as_int(nd[24 + sub]);
+                    const float tmax = ANY ? INFINITY : best_t;
+                    float t;
+                    const bool h = slab(lx, hx, ly, hy, lz, hz, ix, iy, iz, px, py, pz, tmax, t) & (rf != kEmpty);
+                    const float key = h ? t : INFINITY;
+                    const float k1 = __shfl_xor(key, 1), k2 = __shfl_xor(key, 2), k3 = __shfl_xor(key, 3);
+                    // rank on (entry distance, slot); misses rank after every hit
+                    const int rank = (k1 < key || (k1 == key && (sub ^ 1) < sub)) +
+                                     (k2 < key || (k2 == key && (sub ^ 2) < sub)) +
+                                     (k3 < key || (k3 == key && (sub ^ 3) < sub));
+                    const int hm = (int)((__ballot(h) >> gbit) & 0xFull);
+                    const int nh = __popc(hm);
+                    if (nh == 0) {
+                        cur = sp == 0 ? kDone : stk[--sp];
+                    } else if (sp + nh - 1 > kMaxStack4) {
+                        if (sub == 0) atomicOr(a.err, 1);
+                        cur = kDone;
+                    } else {
+                        if (h && rank >= 1) stk[sp + (nh - 1 - rank)] = rf;
+                        const int src = __ffs((int)((__ballot(h && rank == 0) >> gbit) & 0xFull)) - 1;
+                        cur = __shfl(rf, (grp << 2) + src);
+                        sp += nh - 1;
+                    }
+                } else if (cur != kDone) {
+                    const int e = ~cur;
+                    const int first = e & ((1 << kLeafShift) - 1);
+                    const int cnt = ((e >> kLeafShift) & 7) + 1;
+                    float lt = INFINITY, lu = 0.0f, lv = 0.0f;
+                    int lo_ = 0x7FFFFFFF, ls = -1, lfirst = 8;
+                    for (int b = 0; b < cnt; b += 4) {
+                        const int k = b + sub;
+                        const bool act = k < cnt;
+                        const int sl = first + (act ? k : 0);
+                        const float4 g0 = geo[3 * (size_t)sl], g1 = geo[3 * (size_t)sl + 1], g2 = geo[3 * (size_t)sl + 2];
+                        float tt, uu, vv;
+                        const bool hit = moller_trumbore(o, d, f3(g0.x, g0.y, g0.z), f3(g0.w, g1.x, g1.y),
+                                                         f3(g1.z, g1.w, g2.x), tt, uu, vv) & act;
+                        if (ANY) {
+                            if (hit && k < lfirst) lfirst = k;
+                        } else {
+                            const int og = __float_as_int(g2.y);
+                            if (hit && (tt < lt || (tt == lt && og < lo_))) {
+                                lt = tt;
+                                lo_ = og;
+                                ls = sl;
+                                lu = uu;
+                                lv = vv;
+                            }
+                        }
+                    }
+                    if (ANY) {
+                        int fh = lfirst;
+                        fh = min(fh, __shfl_xor(fh, 1));
+                        fh = min(fh, __shfl_xor(fh, 2));
+                        // counted as the one-at-a-time walk would: it stops at its first hit
+                        if (COUNT && sub == 0) tests += fh < 8 ? fh + 1 : cnt;
+                        if (fh < 8) {
+                            best_slot = 0;  // marks "hit" for the any-hit result
+                            cur = kDone;
+                        } else {
+                            cur = sp == 0 ? kDone : stk[--sp];
+                        }
+                    } else {
+                        if (COUNT && sub == 0) tests += cnt;
+                        // least (t, original index) over the quad, then against the ray's best
+                        for (int off = 1; off <= 2; off <<= 1) {
+                            const float ot = __shfl_xor(lt, off), ou = __shfl_xor(lu, off), ov = __shfl_xor(lv, off);
+                            const int oo = __shfl_xor(lo_, off), os = __shfl_xor(ls, off);
+                            if (ot < lt || (ot == lt && oo < lo_)) {
+                                lt = ot;
+                                lo_ = oo;
+                                ls = os;
+                                lu = ou;
+                                lv = ov;
+                            }
+                        }
+                        if (ls >= 0 && (lt < best_t || (lt == best_t && lo_ < best_orig))) {
+                            best_t = lt;
+                            best_orig = lo_;
+                            best_slot = ls;
+                            bu = lu;
+                            bv = lv;
+                        }
+                        cur = sp == 0 ? kDone : stk[--sp];
+                    }
+                }
+                if (cur == kDone) {
+                    if (sub == 0) {
+                        if (ANY) a.vis_out[out_idx] = best_slot >= 0 ? 0 : 1;
+                        else a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
+                    }
+                    item = -1;
+                }
+            }
+            if (__popcll(__ballot(sub == 0 && item < 0)) >= need) break;
+        }
+    }
+}
+
+// Persistent traversal.  Each wave keeps up to 64 independent rays in flight
+// (one per lane) and refills idle lanes from its chunk of items whenever at
+// least `refill` lanes are idle (ballot + mbcnt ranks them), so lanes never
+// wait for the slowest ray of a fixed batch; the traversal loop returns to the
+// refill as soon as that many lanes are idle.  Per lane: ordered near-first
+// descent, far children pushed onto a short stack in LDS (STACK_LDS entries,
+// column per lane) that overflows into a private global area, pruning against
+// the best hit so far.  Per-ray visit order is deterministic and equal to the
+// CPU restatement in the oracle.  A launch with few items for its lanes (4-wide
+// tree, a.coop) takes the quad-cooperative walk instead.  BATCH: every ray looks
+// up its own scene (BVH, triangles, lights, camera) in the batch table.
 template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH, int WIDTH>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6))) void k_trace(TraceArgs a) {
+__global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     constexpr bool ANY = MODE == MODE_SHADOW;
     constexpr int MAXS = max_stack<WIDTH>();
-    __shared__ int32_t lds[STACK_LDS * WG];
+    constexpr int LDS_LANE = STACK_LDS * WG, LDS_COOP = WIDTH == 4 ? (WG / 4) * kMaxStack4 : 0;
+    __shared__ int32_t lds[LDS_LANE > LDS_COOP ? LDS_LANE : LDS_COOP];
     const int tid = threadIdx.x;
     const int lane = __lane_id();
-    int32_t* gst = a.gstack + (size_t)(blockIdx.x * WG + tid) * (MAXS - STACK_LDS);
     int n = a.n_static;
     if (a.count_ptr) {
         n = 0;
         const int nlev = MODE == MODE_SHADOW ? a.nlev : 1;
         for (int k = 0; k < nlev; ++k) n += a.count_ptr[k] * a.count_mul;
     }
-    const Node* __restrict__ nodes = a.sc.nodes;
-    const Node4* __restrict__ nodes4 = a.sc.nodes4;
-    const float4* __restrict__ geo = a.sc.geo;
-
-    int item = -1;
-    size_t out_idx = 0;          // MODE_SHADOW: visibility byte of the item
-    bool drained = false;        // the fetch cursor has passed n
-    int chunk_next = 0, chunk_end = 0;  // wave-uniform: unissued items of this wave's chunk
-    int seen = 0;                       // wave-uniform: cursor value after this wave's last claim
-    const int waves_total = (int)gridDim.x * (WG / 64);
-    F3 o = {0, 0, 0}, d = {0, 0, 0};
-    float ix = 0, iy = 0, iz = 0, px = 0, py = 0, pz = 0;
-    float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
-    int best_slot = -1, best_orig = 0x7FFFFFFF;
-    int cur = kDone, sp = 0;
-    int budget = 0;  // node visits left for this ray: a tree visits each internal node at most once
-    int max_visits = a.sc.n_tri + 64;
     unsigned long long visits = 0, tests = 0;
-    const int thresh = a.refill < 1 ? 1 : (a.refill > 64 ? 64 : a.refill);
+    if (WIDTH == 4 && use_coop(a.coop, n, (long long)gridDim.x * WG)) {
+        trace_coop<MODE, WG, COUNT, BATCH>(a, n, lds + (tid >> 6) * 16 * kMaxStack4, visits, tests);
+    } else {
+        int32_t* gst = a.gstack + (size_t)(blockIdx.x * WG + tid) * (MAXS - STACK_LDS);
+        const Node* __restrict__ nodes = a.sc.nodes;
+        const Node4* __restrict__ nodes4 = a.sc.nodes4;
+        const float4* __restrict__ geo = a.sc.geo;
+        int item = -1;
+        size_t out_idx = 0;
+        Fetch f = fetch_start();
+        const int waves_total = (int)gridDim.x * (WG / 64);
+        F3 o = {0, 0, 0}, d = {0, 0, 0};
+        float ix = 0, iy = 0, iz = 0, px = 0, py = 0, pz = 0;
+        float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
+        int best_slot = -1, best_orig = 0x7FFFFFFF;
+        int cur = kDone, sp = 0;
+        int budget = 0;  // node visits left for this ray: a tree visits each internal node at most once
+        const int thresh = a.refill < 1 ? 1 : (a.refill > 64 ? 64 : a.refill);
 
-    for (;;) {
-        // ---- refill idle lanes
-        const uint64_t idle = __ballot(item < 0);
-        const int nidle = __popcll(idle);
-        if (nidle >= thresh && (!drained || chunk_next < chunk_end)) {
-            // Wave-uniform bookkeeping: idle lanes take the rest of the chunk
-            // first, then the head of a newly claimed one.
-            const int avail = chunk_end - chunk_next;
-            const int old_next = chunk_next;
-            int base = 0, take = 0;
-            if (avail < nidle && !drained) {
-                const int want = nidle - avail;
-                int size = (n - seen) / (2 * waves_total);
-                size = size > a.batch ? a.batch : size;
-                size = size < want ? want : size;
-                const int leader = __ffsll((unsigned long long)idle) - 1;
-                if (lane == leader) base = atomicAdd(a.cursor, size);
-                base = __shfl(base, leader);
-                seen = base + size;
-                if (base + size >= n) drained = true;
-                const int end = base + size < n ? base + size : n;
-                take = end > base ? end - base : 0;
-            }
-            const int rank = __popcll(idle & lanemask_lt());
-            int my = -1;
-            if (rank < avail) my = old_next + rank;
-            else if (rank - avail < take) my = base + (rank - avail);
-            if (avail >= nidle) {
-                chunk_next = old_next + nidle;
-            } else {
-                const int used = nidle - avail < take ? nidle - avail : take;
-                chunk_next = take > used ? base + used : 0;
-                chunk_end = take > used ? base + take : 0;
-            }
-            if (item < 0) {
-                if (my >= 0) {
+        for (;;) {
+            // ---- refill idle lanes
+            const uint64_t idle = __ballot(item < 0);
+            const int nidle = __popcll(idle);
+            if (nidle >= thresh && (!f.drained || f.next < f.end)) {
+                const int my = fetch_items(f, a, n, waves_total, idle, nidle, __popcll(idle & lanemask_lt()));
+                if (item < 0 && my >= 0) {
+                    RayItem r;
+                    setup_item<MODE, BATCH, WIDTH>(a, my, r);
                     item = my;
-                    bool valid = true;
-                    int root = WIDTH == 4 ? a.sc.root4 : a.sc.root;
-                    if (MODE == MODE_PRIMARY) {
-                        int x, y;
-                        if (BATCH) {
-                            const BatchScene& B = a.table[scene_of_id(a.table, a.n_scenes, my)];
-                            nodes = B.sc.nodes;
-                            nodes4 = B.sc.nodes4;
-                            geo = B.sc.geo;
-                            root = WIDTH == 4 ? B.sc.root4 : B.sc.root;
-                            max_visits = B.sc.n_tri + 64;
-                            valid = tile_pixel(my - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
-                            o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
-                            d = camera_dir(B.cam, x, y);
-                        } else {
-                            valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
-                            o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-                            d = camera_dir(a.cam, x, y);
-                        }
-                    } else if (MODE == MODE_QUEUE) {
-                        const float4 q0 = a.q[2 * (size_t)my], q1 = a.q[2 * (size_t)my + 1];
-                        o = f3(q0.x, q0.y, q0.z);
-                        d = f3(q1.x, q1.y, q1.z);
-                        if (BATCH) {
-                            const BatchScene& B = a.table[__float_as_int(q1.w)];
-                            nodes = B.sc.nodes;
-                            nodes4 = B.sc.nodes4;
-                            geo = B.sc.geo;
-                            root = WIDTH == 4 ? B.sc.root4 : B.sc.root;
-                            max_visits = B.sc.n_tri + 64;
-                        }
-                    } else {
-                        // item -> (level, surface j, light slot li)
-                        int rem = my, lev = 0;
-                        for (int k = 0; k + 1 < a.nlev; ++k) {
-                            const int c = a.count_ptr[k] * a.count_mul;
-                            if (rem < c) break;
-                            rem -= c;
-                            lev = k + 1;
-                        }
-                        out_idx = (size_t)lev * (size_t)a.vis_stride + (size_t)rem;
-                        const float4* surf = a.surf + (size_t)lev * (size_t)a.surf_stride;
-                        const int j = rem / a.count_mul, li = rem - j * a.count_mul;
-                        const float4 s0 = surf[2 * (size_t)j];
-                        const F3 P = f3(s0.x, s0.y, s0.z);
-                        const DevLight* lights = a.sc.lights;
-                        int nl = a.sc.n_light;
-                        if (BATCH) {
-                            const BatchScene& B = a.table[unpack_scene(__float_as_int(surf[2 * (size_t)j + 1].w))];
-                            nodes = B.sc.nodes;
-                            nodes4 = B.sc.nodes4;
-                            geo = B.sc.geo;
-                            root = WIDTH == 4 ? B.sc.root4 : B.sc.root;
-                            max_visits = B.sc.n_tri + 64;
-                            lights = B.sc.lights;
-                            nl = B.sc.n_light;
-                        }
-                        if (li < nl) {
-                            const DevLight& L = lights[li];
-                            d = normalize3(sub(f3(L.position[0], L.position[1], L.position[2]), P));
-                            o = add(P, scale(1e-3f, d));
-                        } else {
-                            valid = false;  // light slot beyond this scene's lights
-                            d = f3(0.0f, 0.0f, 1.0f);
-                            o = P;
-                        }
-                    }
+                    out_idx = r.out_idx;
+                    o = r.o;
+                    d = r.d;
+                    nodes = r.nodes;
+                    nodes4 = r.nodes4;
+                    geo = r.geo;
                     ix = safe_inv(d.x);
                     iy = safe_inv(d.y);
                     iz = safe_inv(d.z);
@@ -365,157 +622,157 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6))) void k_
                     best_orig = 0x7FFFFFFF;
                     bu = bv = 0.0f;
                     sp = 0;
-                    budget = max_visits;
-                    cur = valid ? root : kDone;
+                    budget = r.max_visits;
+                    cur = r.valid ? r.root : kDone;
                 }
             }
-        }
-        if (__ballot(item >= 0) == 0) {
-            if (drained && chunk_next >= chunk_end) break;
-            continue;
-        }
-        // ---- traverse until `thresh` lanes are idle (all of them once no work is
-        // left to claim), so finished lanes get new items without waiting for the
-        // longest ray in the wave
-        const int need = (drained && chunk_next >= chunk_end) ? 64 : thresh;
-        for (;;) {
-            if (item >= 0) {
-                if (cur >= 0 && cur != kDone && --budget < 0) {
-                    atomicOr(a.err, 2);
-                    cur = kDone;
-                } else if (cur >= 0 && cur != kDone) {
-                    if (COUNT) ++visits;
-                    const float tmax = ANY ? INFINITY : best_t;
-                    if (WIDTH == 4) {
-                        const Node4* nd = nodes4 + cur;
-                        const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
-                        const int4 rf = nd->ref;
-                        float t0, t1, t2, t3;
-                        // all four tests unconditionally (no load sinking), empty slots masked after
-                        const bool h0 = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ix, iy, iz, px, py, pz, tmax, t0) & (rf.x != kEmpty);
-                        const bool h1 = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ix, iy, iz, px, py, pz, tmax, t1) & (rf.y != kEmpty);
-                        const bool h2 = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ix, iy, iz, px, py, pz, tmax, t2) & (rf.z != kEmpty);
-                        const bool h3 = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ix, iy, iz, px, py, pz, tmax, t3) & (rf.w != kEmpty);
-                        const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-                        if (nh == 0) {
-                            cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
-                        } else if (sp + nh - 1 > MAXS) {
-                            atomicOr(a.err, 1);
-                            cur = kDone;
-                        } else {
-                            float e0 = h0 ? t0 : INFINITY, e1 = h1 ? t1 : INFINITY, e2 = h2 ? t2 : INFINITY,
-                                  e3 = h3 ? t3 : INFINITY;
-                            int k0 = 0, k1 = 1, k2 = 2, k3 = 3;
-                            int r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
-                            cmpswap(e0, k0, r0, e1, k1, r1);
-                            cmpswap(e2, k2, r2, e3, k3, r3);
-                            cmpswap(e0, k0, r0, e2, k2, r2);
-                            cmpswap(e1, k1, r1, e3, k3, r3);
-                            cmpswap(e1, k1, r1, e2, k2, r2);
-                            if (nh > 3) {
-                                if (sp < STACK_LDS) lds[sp * WG + tid] = r3;
-                                else gst[sp - STACK_LDS] = r3;
-                                ++sp;
-                            }
-                            if (nh > 2) {
-                                if (sp < STACK_LDS) lds[sp * WG + tid] = r2;
-                                else gst[sp - STACK_LDS] = r2;
-                                ++sp;
-                            }
-                            if (nh > 1) {
-                                if (sp < STACK_LDS) lds[sp * WG + tid] = r1;
-                                else gst[sp - STACK_LDS] = r1;
-                                ++sp;
-                            }
-                            cur = r0;
-                        }
-                    } else {
-                        const Node* nd = nodes + cur;
-                        const float4 b0 = nd->b0, b1 = nd->b1, b2 = nd->b2;
-                        const int4 c = nd->c;
-                        float t0, t1;
-                        const bool h0 = slab(b0.x, b0.y, b0.z, b0.w, b2.x, b2.y, ix, iy, iz, px, py, pz, tmax, t0);
-                        const bool h1 = slab(b1.x, b1.y, b1.z, b1.w, b2.z, b2.w, ix, iy, iz, px, py, pz, tmax, t1);
-                        if (h0 && h1) {
-                            const bool swap = t1 < t0;
-                            const int nearc = swap ? c.y : c.x, farc = swap ? c.x : c.y;
-                            if (sp >= MAXS) {
+            if (__ballot(item >= 0) == 0) {
+                if (f.drained && f.next >= f.end) break;
+                continue;
+            }
+            // ---- traverse until `thresh` lanes are idle (all of them once no work is
+            // left to claim), so finished lanes get new items without waiting for the
+            // longest ray in the wave
+            const int need = (f.drained && f.next >= f.end) ? 64 : thresh;
+            for (;;) {
+                if (item >= 0) {
+                    if (cur >= 0 && cur != kDone && --budget < 0) {
+                        atomicOr(a.err, 2);
+                        cur = kDone;
+                    } else if (cur >= 0 && cur != kDone) {
+                        if (COUNT) ++visits;
+                        const float tmax = ANY ? INFINITY : best_t;
+                        if (WIDTH == 4) {
+                            const Node4* nd = nodes4 + cur;
+                            const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+                            const int4 rf = nd->ref;
+                            float t0, t1, t2, t3;
+                            // all four tests unconditionally (no load sinking), empty slots masked after
+                            const bool h0 = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ix, iy, iz, px, py, pz, tmax, t0) & (rf.x != kEmpty);
+                            const bool h1 = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ix, iy, iz, px, py, pz, tmax, t1) & (rf.y != kEmpty);
+                            const bool h2 = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ix, iy, iz, px, py, pz, tmax, t2) & (rf.z != kEmpty);
+                            const bool h3 = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ix, iy, iz, px, py, pz, tmax, t3) & (rf.w != kEmpty);
+                            const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+                            if (nh == 0) {
+                                cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
+                            } else if (sp + nh - 1 > MAXS) {
                                 atomicOr(a.err, 1);
                                 cur = kDone;
                             } else {
-                                if (sp < STACK_LDS) lds[sp * WG + tid] = farc;
-                                else gst[sp - STACK_LDS] = farc;
-                                ++sp;
-                                cur = nearc;
+                                float e0 = h0 ? t0 : INFINITY, e1 = h1 ? t1 : INFINITY, e2 = h2 ? t2 : INFINITY,
+                                      e3 = h3 ? t3 : INFINITY;
+                                int k0 = 0, k1 = 1, k2 = 2, k3 = 3;
+                                int r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+                                cmpswap(e0, k0, r0, e1, k1, r1);
+                                cmpswap(e2, k2, r2, e3, k3, r3);
+                                cmpswap(e0, k0, r0, e2, k2, r2);
+                                cmpswap(e1, k1, r1, e3, k3, r3);
+                                cmpswap(e1, k1, r1, e2, k2, r2);
+                                if (nh > 3) {
+                                    if (sp < STACK_LDS) lds[sp * WG + tid] = r3;
+                                    else gst[sp - STACK_LDS] = r3;
+                                    ++sp;
+                                }
+                                if (nh > 2) {
+                                    if (sp < STACK_LDS) lds[sp * WG + tid] = r2;
+                                    else gst[sp - STACK_LDS] = r2;
+                                    ++sp;
+                                }
+                                if (nh > 1) {
+                                    if (sp < STACK_LDS) lds[sp * WG + tid] = r1;
+                                    else gst[sp - STACK_LDS] = r1;
+                                    ++sp;
+                                }
+                                cur = r0;
                             }
-                        } else if (h0) {
-                            cur = c.x;
-                        } else if (h1) {
-                            cur = c.y;
+                        } else {
+                            const Node* nd = nodes + cur;
+                            const float4 b0 = nd->b0, b1 = nd->b1, b2 = nd->b2;
+                            const int4 c = nd->c;
+                            float t0, t1;
+                            const bool h0 = slab(b0.x, b0.y, b0.z, b0.w, b2.x, b2.y, ix, iy, iz, px, py, pz, tmax, t0);
+                            const bool h1 = slab(b1.x, b1.y, b1.z, b1.w, b2.z, b2.w, ix, iy, iz, px, py, pz, tmax, t1);
+                            if (h0 && h1) {
+                                const bool swap = t1 < t0;
+                                const int nearc = swap ? c.y : c.x, farc = swap ? c.x : c.y;
+                                if (sp >= MAXS) {
+                                    atomicOr(a.err, 1);
+                                    cur = kDone;
+                                } else {
+                                    if (sp < STACK_LDS) lds[sp * WG + tid] = farc;
+                                    else gst[sp - STACK_LDS] = farc;
+                                    ++sp;
+                                    cur = nearc;
+                                }
+                            } else if (h0) {
+                                cur = c.x;
+                            } else if (h1) {
+                                cur = c.y;
+                            } else {
+                                cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
+                            }
+                        }
+                    } else if (cur != kDone) {
+                        const int e = ~cur;
+                        const int first = e & ((1 << kLeafShift) - 1);
+                        const int cnt = ((e >> kLeafShift) & 7) + 1;
+                        bool occluded = false;
+                        // Two triangles per iteration, both fetched before either is
+                        // tested; an odd leaf tests its last triangle twice (same
+                        // result, counted once).
+                        const int last = first + cnt - 1;
+                        for (int s0 = first; s0 <= last; s0 += 2) {
+                            const int s1 = s0 + 1 <= last ? s0 + 1 : s0;
+                            const float4 a0 = geo[3 * (size_t)s0], a1 = geo[3 * (size_t)s0 + 1], a2 = geo[3 * (size_t)s0 + 2];
+                            const float4 b0 = geo[3 * (size_t)s1], b1 = geo[3 * (size_t)s1 + 1], b2 = geo[3 * (size_t)s1 + 2];
+                            float ta, ua, va, tb, ub, vb;
+                            const bool ha = moller_trumbore(o, d, f3(a0.x, a0.y, a0.z), f3(a0.w, a1.x, a1.y),
+                                                            f3(a1.z, a1.w, a2.x), ta, ua, va);
+                            const bool hb = moller_trumbore(o, d, f3(b0.x, b0.y, b0.z), f3(b0.w, b1.x, b1.y),
+                                                            f3(b1.z, b1.w, b2.x), tb, ub, vb);
+                            // counted as the one-at-a-time walk would: an any-hit ray stops at its first hit
+                            if (COUNT) tests += (ANY && ha) ? 1 : (s1 != s0 ? 2 : 1);
+                            if (ANY) {
+                                if (ha | hb) {
+                                    occluded = true;
+                                    break;
+                                }
+                            } else {
+                                const int oa = __float_as_int(a2.y), ob = __float_as_int(b2.y);
+                                if (ha && (ta < best_t || (ta == best_t && oa < best_orig))) {
+                                    best_t = ta;
+                                    best_orig = oa;
+                                    best_slot = s0;
+                                    bu = ua;
+                                    bv = va;
+                                }
+                                if (hb && (tb < best_t || (tb == best_t && ob < best_orig))) {
+                                    best_t = tb;
+                                    best_orig = ob;
+                                    best_slot = s1;
+                                    bu = ub;
+                                    bv = vb;
+                                }
+                            }
+                        }
+                        if (ANY && occluded) {
+                            best_slot = 0;  // marks "hit" for the any-hit result
+                            cur = kDone;
                         } else {
                             cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
                         }
                     }
-                } else if (cur != kDone) {
-                    const int e = ~cur;
-                    const int first = e & ((1 << kLeafShift) - 1);
-                    const int cnt = ((e >> kLeafShift) & 7) + 1;
-                    bool occluded = false;
-                    // Two triangles per iteration, both fetched before either is
-                    // tested; an odd leaf tests its last triangle twice (same
-                    // result, counted once).
-                    const int last = first + cnt - 1;
-                    for (int s0 = first; s0 <= last; s0 += 2) {
-                        const int s1 = s0 + 1 <= last ? s0 + 1 : s0;
-                        const float4 a0 = geo[3 * (size_t)s0], a1 = geo[3 * (size_t)s0 + 1], a2 = geo[3 * (size_t)s0 + 2];
-                        const float4 b0 = geo[3 * (size_t)s1], b1 = geo[3 * (size_t)s1 + 1], b2 = geo[3 * (size_t)s1 + 2];
-                        float ta, ua, va, tb, ub, vb;
-                        const bool ha = moller_trumbore(o, d, f3(a0.x, a0.y, a0.z), f3(a0.w, a1.x, a1.y),
-                                                        f3(a1.z, a1.w, a2.x), ta, ua, va);
-                        const bool hb = moller_trumbore(o, d, f3(b0.x, b0.y, b0.z), f3(b0.w, b1.x, b1.y),
-                                                        f3(b1.z, b1.w, b2.x), tb, ub, vb);
-                        // counted as the one-at-a-time walk would: an any-hit ray stops at its first hit
-                        if (COUNT) tests += (ANY && ha) ? 1 : (s1 != s0 ? 2 : 1);
+                    if (cur == kDone) {
                         if (ANY) {
-                            if (ha | hb) {
-                                occluded = true;
-                                break;
-                            }
+                            a.vis_out[out_idx] = best_slot >= 0 ? 0 : 1;
                         } else {
-                            const int oa = __float_as_int(a2.y), ob = __float_as_int(b2.y);
-                            if (ha && (ta < best_t || (ta == best_t && oa < best_orig))) {
-                                best_t = ta;
-                                best_orig = oa;
-                                best_slot = s0;
-                                bu = ua;
-                                bv = va;
-                            }
-                            if (hb && (tb < best_t || (tb == best_t && ob < best_orig))) {
-                                best_t = tb;
-                                best_orig = ob;
-                                best_slot = s1;
-                                bu = ub;
-                                bv = vb;
-                            }
+                            a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
                         }
-                    }
-                    if (ANY && occluded) {
-                        best_slot = 0;  // marks "hit" for the any-hit result
-                        cur = kDone;
-                    } else {
-                        cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
+                        item = -1;
                     }
                 }
-                if (cur == kDone) {
-                    if (ANY) {
-                        a.vis_out[out_idx] = best_slot >= 0 ? 0 : 1;
-                    } else {
-                        a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
-                    }
-                    item = -1;
-                }
+                if (__popcll(__ballot(item < 0)) >= need) break;
             }
-            if (__popcll(__ballot(item < 0)) >= need) break;
         }
     }
     if (COUNT) {

@@ -4,7 +4,11 @@
 // MultiDeviceRenderer and render once to a host buffer; then the single-GPU
 // route (createCudaScene + DeviceCameraHandler + DeviceRenderer).
 //
-// usage: render_scene <scene.bin> <out_multi.rgba> <out_single.rgba>
+// usage: render_scene <scene.bin> <out_multi.rgba> <out_single.rgba> [mesh.obj mesh_material]
+// With a mesh, a TriangleMesh read from the OBJ file is added after the
+// scene's own triangles, the way the reference's driver adds its OBJ model
+// (src/main.cpp:386-403): material and transformation index set before
+// loadFromFile, then Scene::addTriangleMesh.
 // scene.bin: int32 n_tri, n_mat, n_light, width, height, depth; float32
 // verts[n][9], normals[n][9]; int32 mat[n]; 48-byte materials; 32-byte lights;
 // float32 camera position, view, right, up (12 floats).
@@ -23,8 +27,8 @@ static void rd(FILE* f, T* p, size_t n) {
 }
 
 int main(int argc, char** argv) {
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: %s scene.bin out_multi.rgba out_single.rgba\n", argv[0]);
+    if (argc != 4 && argc != 6) {
+        std::fprintf(stderr, "usage: %s scene.bin out_multi.rgba out_single.rgba [mesh.obj mesh_material]\n", argv[0]);
         return 2;
     }
     FILE* f = std::fopen(argv[1], "rb");
@@ -68,6 +72,17 @@ int main(int argc, char** argv) {
         t->setTransformationIndex(ident);
         scene.addSurface(t);
         tris.push_back(t);
+    }
+    TriangleMesh mesh;
+    if (argc == 6) {
+        mesh.setTransformationIndex(ident);
+        mesh.setMaterialIndex(matIndex[std::atoi(argv[5])]);
+        if (!mesh.loadFromFile(argv[4])) {
+            std::fprintf(stderr, "cannot read %s: %s\n", argv[4], eos_last_error());
+            return 2;
+        }
+        scene.addTriangleMesh(&mesh);
+        std::printf("Triangle Mesh (%s) has %u Triangles.\n", argv[4], mesh.getNumTriangles());
     }
     std::vector<LightSource*> ls;
     for (const eos_light& l : lights) {

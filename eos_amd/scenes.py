@@ -336,6 +336,22 @@ def displaced_sphere(nu: int, nv: int, seed: int, amplitude: float = 0.15, octav
     return verts.astype(np.float32), norms.astype(np.float32), rt
 
 
+def write_obj(path: str, verts: np.ndarray, normals: np.ndarray) -> None:
+    """Writes triangles as a Wavefront OBJ file: one `v` and one `vn` per corner
+    (9 significant digits, so float32 values read back exactly) and faces
+    `f a//a b//b c//c`."""
+    v = np.asarray(verts, np.float32).reshape(-1, 3)
+    n = np.asarray(normals, np.float32).reshape(-1, 3)
+    with open(path, "w") as f:
+        for x, y, z in v:
+            f.write(f"v {x:.9g} {y:.9g} {z:.9g}\n")
+        for x, y, z in n:
+            f.write(f"vn {x:.9g} {y:.9g} {z:.9g}\n")
+        for t in range(len(v) // 3):
+            a, b, c = 3 * t + 1, 3 * t + 2, 3 * t + 3
+            f.write(f"f {a}//{a} {b}//{b} {c}//{c}\n")
+
+
 def _band(values: np.ndarray, bands: int) -> np.ndarray:
     lo, hi = float(values.min()), float(values.max())
     b = np.floor((values - lo) / max(hi - lo, 1e-30) * bands)

@@ -7,6 +7,7 @@
  *   Material             src/Material.h:43-78
  *   LightSource          src/LightSource.h:27-39
  *   Surface, Triangle    src/surface.h:31-47, src/Triangle.h:32-99
+ *   TriangleMesh         src/triangleMesh.h:30-80 (native OBJ reader instead of assimp)
  *   Scene                src/Scene.h:42-118
  *   cudaScene_t, createCudaScene            src/cudaStructures.h:87-98
  *   DeviceCameraHandler  src/DeviceCameraHandler.h:29-40
@@ -164,9 +165,44 @@ private:
     int m_TransformationIndex = -1;
 };
 
+/* A mesh read from a Wavefront OBJ file (eos_mesh_load_obj; the reference
+ * reads through assimp, src/triangleMesh.cpp:79-160).  Like the reference, the
+ * material index and transformation index set before loadFromFile are given to
+ * every triangle it creates; the mesh owns its triangles. */
+class TriangleMesh {
+public:
+    TriangleMesh() = default;
+    ~TriangleMesh();
+    TriangleMesh(const TriangleMesh&) = delete;
+    TriangleMesh& operator=(const TriangleMesh&) = delete;
+    bool loadFromFile(const char* filename);
+    const eos::mat4& transformation() { return m_LocalToWorldTransformation; }
+    void setTransformation(eos::mat4& transformation) { m_LocalToWorldTransformation = transformation; }
+    Material& getMaterial() { return m_Material; }
+    void setMaterial(const Material& material) { m_Material = material; }
+    unsigned int getNumTriangles() { return (unsigned int)m_Triangles.size(); }
+    Triangle* getTriangle(int index) { return index >= 0 && index < (int)m_Triangles.size() ? m_Triangles[index] : nullptr; }
+    void setSceneStartIndex(int start) { m_SceneIndexStart = start; }
+    void setSceneEndIndex(int end) { m_SceneIndexEnd = end; }
+    void setMaterialIndex(int index) { m_MaterialIndex = index; }
+    int getMaterialIndex(int) const { return m_MaterialIndex; }
+    void setTransformationIndex(int index) { m_TransformationIndex = index; }
+    int getTransformationIndex() const { return m_TransformationIndex; }
+
+private:
+    std::vector<Triangle*> m_Triangles;
+    eos::mat4 m_LocalToWorldTransformation;
+    Material m_Material;
+    int m_MaterialIndex = 0;
+    int m_TransformationIndex = -1;
+    int m_SceneIndexStart = 0, m_SceneIndexEnd = 0;
+};
+
 class Scene {
 public:
     bool addSurface(Surface* surface);
+    /* src/Scene.cpp:55-74: appends the mesh's triangles (the mesh keeps owning them). */
+    bool addTriangleMesh(TriangleMesh* mesh);
     bool addLightSource(LightSource* light);
     int addMaterial(Material material);
     int addTransformation(const eos::mat4& trans);

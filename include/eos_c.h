@@ -206,6 +206,22 @@ typedef struct eos_batch_stats {
 eos_status eos_render_batch(eos_scene* const* scenes, const eos_camera* cameras, int32_t n,
                             const eos_batch_opts* opts, uint8_t* const* rgba_out, eos_batch_stats* stats);
 
+/* ---- mesh files ------------------------------------------------------------ */
+
+/* Triangles read from a Wavefront OBJ file, in the layout of eos_scene_desc
+ * (replaces TriangleMesh::loadFromFile, src/triangleMesh.cpp:79-160, which
+ * reads through assimp): positions and per-corner normals, polygons as fans.
+ * Corners without a `vn` get the normalized sum of the unit normals of the
+ * faces sharing their position.  Arrays are malloc'ed; free with eos_mesh_free. */
+typedef struct eos_mesh {
+    int32_t num_triangles;
+    float* vertices;          /* [N][3][3] */
+    float* normals;           /* [N][3][3] */
+} eos_mesh;
+
+eos_status eos_mesh_load_obj(const char* path, eos_mesh* out);
+eos_status eos_mesh_free(eos_mesh* mesh);
+
 /* ---- tuning, diagnostics ------------------------------------------------ */
 
 /* Traversal launch configuration (the autotuner's knobs). */
@@ -217,6 +233,8 @@ typedef struct eos_launch_config {
     int32_t batch;            /* ray batch: most items a wave claims per fetch, 1..4096 */
     int32_t primary_packets;  /* 1: primary rays walk the 4-wide tree as 8x8-pixel packets, one per wave
                                  (pays off when triangles cover many pixels; 0, the default, walks per ray) */
+    int32_t coop;             /* launches walked by lane quads (four lanes per ray, 4-wide tree): 0 never;
+                                 1, 2, 3: launches with at most lanes/4, lanes/2, lanes items; 4 always */
 } eos_launch_config;
 
 eos_status eos_set_launch_config(eos_scene* scene, const eos_launch_config* cfg);

@@ -54,8 +54,8 @@ def test_batch_with_launch_configs(batch):
     from eos_amd import _lib as E
     data, sc = batch
     ref, _ = E.render_batch(sc, depth=2)
-    for wg, stack, batch_items, packets in ((64, 8, 64, 1), (128, 32, 1024, 1), (256, 16, 16, 0)):
-        sc[0].set_launch_config(wg, stack, 16, 8, batch_items, packets)
+    for wg, stack, batch_items, packets, coop in ((64, 8, 64, 1, 4), (128, 32, 1024, 1, 0), (256, 16, 16, 0, 1)):
+        sc[0].set_launch_config(wg, stack, 16, 8, batch_items, packets, coop)
         outs, _ = E.render_batch(sc, depth=2, use_graph=True)
         for a, b in zip(outs, ref):
             assert np.array_equal(a, b)

@@ -66,17 +66,18 @@ def test_render_parity(name, width):
     assert st["hits"] == oc["hits"]
 
 
-@pytest.mark.parametrize("width,packets", [(2, 0), (4, 0), (4, 1)])
-@pytest.mark.parametrize("leaf_max", [1, 4])
-def test_visit_counts_match_restatement(leaf_max, width, packets):
+@pytest.mark.parametrize("width,packets,coop", [(2, 0, 0), (4, 0, 0), (4, 0, 4), (4, 1, 0), (4, 1, 4)])
+@pytest.mark.parametrize("leaf_max", [1, 4, 8])
+def test_visit_counts_match_restatement(leaf_max, width, packets, coop):
     """Node visits and triangle tests of the counting frame equal the CPU
-    restatement exactly: per ray for every ray, or, with primary packets, per
-    8x8 tile for the primary rays and per ray for the others."""
+    restatement exactly: per ray for every ray (per-lane or quad-cooperative
+    walk), or, with primary packets, per 8x8 tile for the primary rays and per
+    ray for the others."""
     from eos_amd import _lib as E
     s = scenes.config_c(96, 64, nu=96, nv=48)
-    print(f"visit counts leaf_max={leaf_max} width={width} packets={packets}", flush=True)
+    print(f"visit counts leaf_max={leaf_max} width={width} packets={packets} coop={coop}", flush=True)
     g = E.Scene(s, leaf_max=leaf_max, width=width)
-    g.set_launch_config(primary_packets=packets)
+    g.set_launch_config(primary_packets=packets, coop=coop)
     go = g.render(count_visits=True, want_hits=True)
     b = po.lbvh_build(s.verts, leaf_max)
     oo = po.render_lbvh(s, b, width=width)
@@ -102,6 +103,25 @@ PACKET_CASES = {
     "soup_20k": lambda: scenes.config_b(100, 70, n=20000),
     "sphere_e_like": lambda: scenes.config_e(72, 40, nu=100, nv=50, n_materials=16, n_lights=8),
 }
+
+
+@pytest.mark.parametrize("leaf_max", [1, 4, 8])
+@pytest.mark.parametrize("name", sorted(PACKET_CASES))
+def test_quad_walk_equals_per_lane_walk(name, leaf_max):
+    """Quad-cooperative and per-lane walks give bit-identical frames (closest
+    and any-hit launches, every leaf size)."""
+    from eos_amd import _lib as E
+    s = PACKET_CASES[name]()
+    print(f"quad vs per-lane case={name} leaf_max={leaf_max}", flush=True)
+    g = E.Scene(s, leaf_max=leaf_max)
+    g.set_launch_config(coop=4)
+    a = g.render(want_rgb=True, want_hits=True)
+    g.set_launch_config(coop=0)
+    b = g.render(want_rgb=True, want_hits=True)
+    assert np.array_equal(a["hit"], b["hit"])
+    assert np.array_equal(a["rgba"], b["rgba"])
+    assert np.array_equal(a["rgb"].view(np.uint32), b["rgb"].view(np.uint32))
+    assert a["stats"]["shadow_rays"] == b["stats"]["shadow_rays"]
 
 
 @pytest.mark.parametrize("name", sorted(PACKET_CASES))

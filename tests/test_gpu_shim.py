@@ -45,3 +45,31 @@ def test_cpp_api_renders_like_the_c_abi(tmp_path, name):
     assert np.array_equal(single, ref)
     orc = po.Oracle(s).render()["rgba"]
     assert (np.abs(multi.astype(int) - orc.astype(int)).max(-1) <= 1).mean() >= 0.9999
+
+
+def test_cpp_api_triangle_mesh_from_obj(tmp_path):
+    """Config A with its sphere read from an OBJ file through TriangleMesh and
+    Scene::addTriangleMesh renders like the whole scene through the C ABI."""
+    from eos_amd import _lib as E
+    s = scenes.config_a(64, 48)
+    sphere = s.mat == s.mat[0]
+    assert sphere.sum() == 5120 and np.all(np.flatnonzero(sphere) == np.arange(5120))
+    rest = scenes.SceneData(s.verts[~sphere], s.normals[~sphere], s.mat[~sphere], s.materials, s.lights, s.camera,
+                            s.depth, "plane")
+    sc, obj = tmp_path / "plane.bin", tmp_path / "sphere.obj"
+    _write_scene(sc, rest)
+    scenes.write_obj(str(obj), s.verts[sphere], s.normals[sphere])
+    o1, o2 = tmp_path / "multi.rgba", tmp_path / "single.rgba"
+    print("running the C++ example with an OBJ mesh", flush=True)
+    r = subprocess.run([DEMO, str(sc), str(o1), str(o2), str(obj), str(int(s.mat[0]))], capture_output=True,
+                       text=True, timeout=120, env=dict(os.environ, EOS_QUIET="1"))
+    assert r.returncode == 0, r.stderr
+    assert "has 5120 Triangles" in r.stdout
+    # the C++ scene holds the plane first, then the mesh
+    order = np.concatenate([np.flatnonzero(~sphere), np.flatnonzero(sphere)])
+    whole = scenes.SceneData(s.verts[order], s.normals[order], s.mat[order], s.materials, s.lights, s.camera,
+                             s.depth, "reordered")
+    ref = E.Scene(whole, leaf_max=4).render()["rgba"]
+    multi = np.fromfile(o1, np.uint8).reshape(s.height, s.width, 4)
+    single = np.fromfile(o2, np.uint8).reshape(s.height, s.width, 4)
+    assert np.array_equal(multi, ref) and np.array_equal(single, ref)
