@@ -212,7 +212,7 @@ class Scene:
         self._lights = np.ascontiguousarray(lights)
         check(lib().eos_update_lights(self.h, _ptr(self._lights), len(self._lights)))
 
-    DEFAULT_LAUNCH = {"workgroup": 256, "stack_lds": 16, "waves_per_cu": 20, "refill": 16, "batch": 128,
+    DEFAULT_LAUNCH = {"workgroup": 256, "stack_lds": 16, "waves_per_cu": 20, "refill": 16, "batch": 64,
                       "primary_packets": 0, "coop": 3}
 
     def set_launch_config(self, *args, **kw):

@@ -76,3 +76,31 @@ def load(path: str = DEFAULT_PATH) -> dict | None:
         return None
     with open(path) as f:
         return json.load(f)["config"]
+
+
+def main() -> int:
+    """Tunes on the 64-scene batch (config D, depth 2, hipGraph replay) and saves
+    the winner to eos_amd/tuned_launch.json; --single tunes one config C frame."""
+    import argparse
+
+    from . import scenes as S
+    ap = argparse.ArgumentParser(description=main.__doc__.split("\n")[0])
+    ap.add_argument("--single", action="store_true", help="tune on one config C frame instead of the batch")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--out", default=DEFAULT_PATH)
+    args = ap.parse_args()
+    if args.single:
+        data = [S.config_c()]
+        workload = "config C frame"
+    else:
+        data = S.config_d()
+        workload = "config D: 64 soups, depth 2"
+    sc = [E.Scene(d, leaf_max=4) for d in data]
+    res = tune(sc, depth=2 if not args.single else data[0].depth, reps=args.reps, batched=not args.single,
+               path=args.out, workload=workload)
+    print(json.dumps(res))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

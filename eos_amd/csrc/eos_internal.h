@@ -106,7 +106,7 @@ struct LaunchCfg {
     int stack_lds = 16;
     int waves_per_cu = 20;
     int refill = 16;
-    int batch = 128;       // most items a traversal wave claims per fetch
+    int batch = 64;        // most items a traversal wave claims per fetch
     int packets = 0;       // primary rays as one 8x8 packet per wave (4-wide tree)
     int coop = 3;          // quad-cooperative walk for launches with at most one item per lane
 };
