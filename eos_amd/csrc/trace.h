@@ -203,11 +203,13 @@ struct TraceArgs {
 // Compare-exchange on (entry distance, slot): a total order, so the result does
 // not depend on the network's shape.
 __device__ __forceinline__ void cmpswap(float& ta, int& ka, int& ra, float& tb, int& kb, int& rb) {
-    if (tb < ta || (tb == ta && kb < ka)) {
-        float t = ta; ta = tb; tb = t;
-        int k = ka; ka = kb; kb = k;
-        int r = ra; ra = rb; rb = r;
-    }
+    const bool sw = (tb < ta) | ((tb == ta) & (kb < ka));  // selects, no branch
+    const float t0 = sw ? tb : ta, t1 = sw ? ta : tb;
+    const int k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+    const int r0 = sw ? rb : ra, r1 = sw ? ra : rb;
+    ta = t0; tb = t1;
+    ka = k0; kb = k1;
+    ra = r0; rb = r1;
 }
 
 template <int WIDTH>
@@ -592,6 +594,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
         int best_slot = -1, best_orig = 0x7FFFFFFF;
         int cur = kDone, sp = 0;
+        int lpos = -1;   // next triangle of the current leaf (-1: not in a leaf)
         int budget = 0;  // node visits left for this ray: a tree visits each internal node at most once
         const int thresh = a.refill < 1 ? 1 : (a.refill > 64 ? 64 : a.refill);
 
@@ -622,6 +625,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     best_orig = 0x7FFFFFFF;
                     bu = bv = 0.0f;
                     sp = 0;
+                    lpos = -1;
                     budget = r.max_visits;
                     cur = r.valid ? r.root : kDone;
                 }
@@ -713,16 +717,20 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                             }
                         }
                     } else if (cur != kDone) {
+                        // One pair of triangles per step (a longer leaf takes several
+                        // steps, so leaf steps cost about what node steps cost and the
+                        // lanes of a wave diverge less); both triangles are fetched
+                        // before either is tested; an odd leaf tests its last triangle
+                        // twice (same result, counted once).
                         const int e = ~cur;
                         const int first = e & ((1 << kLeafShift) - 1);
                         const int cnt = ((e >> kLeafShift) & 7) + 1;
                         bool occluded = false;
-                        // Two triangles per iteration, both fetched before either is
-                        // tested; an odd leaf tests its last triangle twice (same
-                        // result, counted once).
                         const int last = first + cnt - 1;
-                        for (int s0 = first; s0 <= last; s0 += 2) {
+                        {
+                            const int s0 = lpos < 0 ? first : lpos;
                             const int s1 = s0 + 1 <= last ? s0 + 1 : s0;
+                            lpos = s0 + 2 <= last ? s0 + 2 : -1;
                             const float4 a0 = geo[3 * (size_t)s0], a1 = geo[3 * (size_t)s0 + 1], a2 = geo[3 * (size_t)s0 + 2];
                             const float4 b0 = geo[3 * (size_t)s1], b1 = geo[3 * (size_t)s1 + 1], b2 = geo[3 * (size_t)s1 + 2];
                             float ta, ua, va, tb, ub, vb;
@@ -733,10 +741,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                             // counted as the one-at-a-time walk would: an any-hit ray stops at its first hit
                             if (COUNT) tests += (ANY && ha) ? 1 : (s1 != s0 ? 2 : 1);
                             if (ANY) {
-                                if (ha | hb) {
-                                    occluded = true;
-                                    break;
-                                }
+                                if (ha | hb) occluded = true;
                             } else {
                                 const int oa = __float_as_int(a2.y), ob = __float_as_int(b2.y);
                                 if (ha && (ta < best_t || (ta == best_t && oa < best_orig))) {
@@ -757,8 +762,9 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                         }
                         if (ANY && occluded) {
                             best_slot = 0;  // marks "hit" for the any-hit result
+                            lpos = -1;
                             cur = kDone;
-                        } else {
+                        } else if (lpos < 0) {
                             cur = sp == 0 ? kDone : (--sp < STACK_LDS ? lds[sp * WG + tid] : gst[sp - STACK_LDS]);
                         }
                     }
