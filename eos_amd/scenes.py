@@ -104,6 +104,10 @@ class SceneData:
     depth: int
     name: str = ""
     info: dict = field(default_factory=dict)
+    # per-triangle local-to-world transformations: [T][16] float32 column-major
+    # (glm layout) and [N] int32 indices, -1 = none; None: no transformations
+    xforms: np.ndarray = None
+    xform_index: np.ndarray = None
 
     @property
     def n_tri(self) -> int:
@@ -127,7 +131,7 @@ class SceneData:
         cam["width"] = width
         cam["height"] = height
         return SceneData(self.verts, self.normals, self.mat, self.materials, self.lights,
-                         cam, self.depth, self.name, dict(self.info))
+                         cam, self.depth, self.name, dict(self.info), self.xforms, self.xform_index)
 
 
 # ----------------------------------------------------------------- builders

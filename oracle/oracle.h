@@ -39,6 +39,13 @@ typedef struct {
     const orc_material* mats;
     int32_t n_light;
     const orc_light* lights;
+    /* Per-triangle local-to-world transforms (the reference's Surface
+     * transformation(); src/BVH.cpp:342-383 tests the ray in the triangle's
+     * local space).  NULL / -1: none.  The inverse is glm's, the normal matrix
+     * the transpose of the inverse (src/triangleMesh.cpp:246-250). */
+    int32_t n_xform;
+    const float* xforms;        /* [n_xform][16] column-major (glm layout) */
+    const int32_t* xform_index; /* [n_tri], -1 = none; may be NULL */
 } orc_scene;
 
 typedef struct {
@@ -74,6 +81,9 @@ int orc_render(const orc_scene* s, const void* accel, const orc_camera* cam, int
                int row_start, int row_step, uint8_t* rgba, float* rgb, int32_t* hit_ids,
                orc_rec* recs, orc_counts* counts);
 
+/* glm 0.9.6 4x4 inverse (glm/detail/type_mat4x4.inl:37-96), column-major. */
+void orc_mat4_inverse(const float* m, float* out);
+
 /* Primitive-level entry points for the closed-form tests. */
 int orc_mt(const float* o, const float* d, const float* tri9, float* tuv);
 void orc_camera_ray(const orc_camera* cam, int x, int y, float* dir3);
@@ -91,12 +101,16 @@ void orc_morton_codes(int32_t n, const float* verts, uint32_t* codes, float* cbo
  *   kparent[2n-1]    parent of internal i at [i], of leaf s at [n-1+s] (-1 root),
  *   ibox[(n-1)*6]    exact internal boxes lo.xyz, hi.xyz,
  *   nodes[(n-1)*16]  64-byte traversal nodes (float view; child refs as int bits),
- *   tris[n*12]       sorted triangle records v1.xyz e1.x | e1.yz e2.xy | e2.z orig pad pad,
- *   root_ref[1]. */
+ *   tris[n*12]       sorted triangle records v1.xyz e1.x | e1.yz e2.xy | e2.z orig xform pad
+ *                    (local-space vertices; xform = transformation index or -1),
+ *   root_ref[1].
+ * xforms / xform_index (may be NULL): per-triangle transformations; Morton keys
+ * and boxes then come from the world-space vertices. */
 void orc_lbvh_build(int32_t n, const float* verts, int32_t leaf_max,
                     uint32_t* sorted_codes, int32_t* perm, int32_t* kchild, int32_t* kparent,
                     float* ibox, float* nodes, float* tris, int32_t* root_ref,
-                    float* nodes4, int32_t* n_nodes4, int32_t* root4);
+                    float* nodes4, int32_t* n_nodes4, int32_t* root4,
+                    const float* xforms, const int32_t* xform_index);
 /* nodes4: 128-byte 4-wide nodes [k][32]: lo.x[4] hi.x[4] lo.y[4] hi.y[4]
  * lo.z[4] hi.z[4] ref[4] pad[4]; empty slots have ref 0x7FFFFFFE and boxes at
  * +inf.  Pass nodes4 = NULL with n_nodes4 to get the count first. */
@@ -108,7 +122,7 @@ void orc_karras(int32_t n, const uint32_t* sorted_codes, int32_t* kchild, int32_
 void orc_lbvh_trace(const float* nodes, int32_t root_ref, const float* tris, int32_t n_rays,
                     const float* orig, const float* dirs, int32_t any_hit, int32_t* hit_orig,
                     float* hit_t, float* hit_u, float* hit_v, int64_t* visits, int64_t* tests,
-                    int32_t width);
+                    int32_t width, int32_t n_xform, const float* xforms);
 /* Primary rays of a frame walked as 8x8-pixel packets over the 4-wide tree, as
  * the GPU's packet kernel does: one shared walk per tile, every child some
  * valid lane hits is visited, children ordered by (entry distance of the
@@ -116,7 +130,7 @@ void orc_lbvh_trace(const float* nodes, int32_t root_ref, const float* tris, int
  * t/u/v (may be NULL); visits/tests accumulate per-tile totals. */
 void orc_lbvh4_packet_primary(const float* nodes4, int32_t root4, const float* tris, const orc_camera* cam,
                               int32_t* hit_orig, float* hit_t, float* hit_u, float* hit_v, int64_t* visits,
-                              int64_t* tests);
+                              int64_t* tests, int32_t n_xform, const float* xforms);
 /* Whitted render driven by the restated GPU traversal; fills rgba/rgb/hit_ids like
  * orc_render and counts node visits / triangle tests in counts. */
 int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, const float* tris,

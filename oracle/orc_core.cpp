@@ -64,12 +64,72 @@ inline bool mt_e(V3 o, V3 d, V3 v1, V3 e1, V3 e2, float& t, float& u, float& v) 
     if (v < 0.0f || u + v > 1.0f) return false;
     return true;
 }
+// glm mat4 * vec4 (glm/detail/type_mat4x4.inl:571-616): per row r,
+// (m[0][r]*x + m[1][r]*y) + (m[2][r]*z + m[3][r]*w); column-major m[4c + r].
+inline V3 xform(const float* m, V3 p, float w) {
+    float r[3];
+    for (int k = 0; k < 3; ++k) r[k] = (m[k] * p.x + m[4 + k] * p.y) + (m[8 + k] * p.z + m[12 + k] * w);
+    return {r[0], r[1], r[2]};
+}
+
+// glm 0.9.6 compute_inverse(tmat4x4) (glm/detail/type_mat4x4.inl:37-96): 2x2
+// cofactors of the lower rows, column vectors combined as (a - b) + c, signs,
+// determinant from the first column and row, one reciprocal.
+void inverse_glm(const float* m, float* out) {
+    auto M = [&](int c, int r) { return m[4 * c + r]; };
+    const float c00 = M(2, 2) * M(3, 3) - M(3, 2) * M(2, 3), c02 = M(1, 2) * M(3, 3) - M(3, 2) * M(1, 3),
+                c03 = M(1, 2) * M(2, 3) - M(2, 2) * M(1, 3);
+    const float c04 = M(2, 1) * M(3, 3) - M(3, 1) * M(2, 3), c06 = M(1, 1) * M(3, 3) - M(3, 1) * M(1, 3),
+                c07 = M(1, 1) * M(2, 3) - M(2, 1) * M(1, 3);
+    const float c08 = M(2, 1) * M(3, 2) - M(3, 1) * M(2, 2), c10 = M(1, 1) * M(3, 2) - M(3, 1) * M(1, 2),
+                c11 = M(1, 1) * M(2, 2) - M(2, 1) * M(1, 2);
+    const float c12 = M(2, 0) * M(3, 3) - M(3, 0) * M(2, 3), c14 = M(1, 0) * M(3, 3) - M(3, 0) * M(1, 3),
+                c15 = M(1, 0) * M(2, 3) - M(2, 0) * M(1, 3);
+    const float c16 = M(2, 0) * M(3, 2) - M(3, 0) * M(2, 2), c18 = M(1, 0) * M(3, 2) - M(3, 0) * M(1, 2),
+                c19 = M(1, 0) * M(2, 2) - M(2, 0) * M(1, 2);
+    const float c20 = M(2, 0) * M(3, 1) - M(3, 0) * M(2, 1), c22 = M(1, 0) * M(3, 1) - M(3, 0) * M(1, 1),
+                c23 = M(1, 0) * M(2, 1) - M(2, 0) * M(1, 1);
+    const float f0[4] = {c00, c00, c02, c03}, f1[4] = {c04, c04, c06, c07}, f2[4] = {c08, c08, c10, c11},
+                f3[4] = {c12, c12, c14, c15}, f4[4] = {c16, c16, c18, c19}, f5[4] = {c20, c20, c22, c23};
+    const float v0[4] = {M(1, 0), M(0, 0), M(0, 0), M(0, 0)}, v1[4] = {M(1, 1), M(0, 1), M(0, 1), M(0, 1)},
+                v2[4] = {M(1, 2), M(0, 2), M(0, 2), M(0, 2)}, v3[4] = {M(1, 3), M(0, 3), M(0, 3), M(0, 3)};
+    float inv[16];
+    for (int k = 0; k < 4; ++k) {
+        const float sa = (k & 1) ? -1.0f : 1.0f, sb = -sa;
+        inv[0 + k] = ((v1[k] * f0[k] - v2[k] * f1[k]) + v3[k] * f2[k]) * sa;
+        inv[4 + k] = ((v0[k] * f0[k] - v2[k] * f3[k]) + v3[k] * f4[k]) * sb;
+        inv[8 + k] = ((v0[k] * f1[k] - v1[k] * f3[k]) + v3[k] * f5[k]) * sa;
+        inv[12 + k] = ((v0[k] * f2[k] - v1[k] * f4[k]) + v2[k] * f5[k]) * sb;
+    }
+    const float d0 = M(0, 0) * inv[0], d1 = M(0, 1) * inv[4], d2 = M(0, 2) * inv[8], d3 = M(0, 3) * inv[12];
+    const float det = (d0 + d1) + (d2 + d3);
+    const float one_over = 1.0f / det;
+    for (int k = 0; k < 16; ++k) out[k] = inv[k] * one_over;
+}
+
+inline void transpose4(const float* a, float* out) {
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) out[4 * c + r] = a[4 * r + c];
+}
+
 inline bool mt(V3 o, V3 d, const float* tri9, float& t, float& u, float& v) {
     V3 v1 = ld3(tri9), v2 = ld3(tri9 + 3), v3 = ld3(tri9 + 6);
     return mt_e(o, d, v1, v2 - v1, v3 - v1, t, u, v);
 }
 
 // src/RayTracer.cpp:150-153: floor(c==1 ? 255 : min(c*256.0, 255.0)) in double.
+// One triangle against a world-space ray: in the triangle's local space when it
+// has a transformation (src/BVH.cpp:359-368: origin and direction through the
+// inverse, w = 1 and 0).
+inline int xform_of(const orc_scene& s, int tri) { return s.xform_index ? s.xform_index[tri] : -1; }
+inline bool tri_hit(const orc_scene& s, int tri, V3 o, V3 d, float& t, float& u, float& v) {
+    const int xi = xform_of(s, tri);
+    if (xi < 0) return mt(o, d, s.verts + 9 * (size_t)tri, t, u, v);
+    float inv[16];
+    inverse_glm(s.xforms + 16 * (size_t)xi, inv);
+    return mt(xform(inv, o, 1.0f), xform(inv, d, 0.0f), s.verts + 9 * (size_t)tri, t, u, v);
+}
+
 inline uint8_t quantize(float c) {
     double q = (c == 1.0f) ? 255.0 : std::min((double)c * 256.0, 255.0);
     return (uint8_t)std::floor(q);
@@ -160,7 +220,7 @@ struct Hit { int tri = -1; float t = INFINITY, u = 0, v = 0; };
 void nearest(const Accel& A, const orc_scene& s, V3 o, V3 d, Hit& h) {
     auto test = [&](int tri) {
         float t, u, v;
-        if (mt(o, d, s.verts + 9 * (size_t)tri, t, u, v)) {
+        if (tri_hit(s, tri, o, d, t, u, v)) {
             if (t < h.t || (t == h.t && tri < h.tri)) { h.tri = tri; h.t = t; h.u = u; h.v = v; }
         }
     };
@@ -194,7 +254,7 @@ bool anyhit(const Accel& A, const orc_scene& s, V3 o, V3 d) {
     float t, u, v;
     if (A.brute) {
         for (int i = 0; i < s.n_tri; ++i)
-            if (mt(o, d, s.verts + 9 * (size_t)i, t, u, v)) return true;
+            if (tri_hit(s, i, o, d, t, u, v)) return true;
         return false;
     }
     if (A.nodes.empty()) return false;
@@ -209,7 +269,7 @@ bool anyhit(const Accel& A, const orc_scene& s, V3 o, V3 d) {
         if (!abox(n, od, inv, 1e300)) continue;
         if (n.left < 0) {
             for (int i = n.first; i < n.first + n.count; ++i)
-                if (mt(o, d, s.verts + 9 * (size_t)A.idx[i], t, u, v)) return true;
+                if (tri_hit(s, A.idx[i], o, d, t, u, v)) return true;
         } else {
             stack[sp++] = n.right;
             stack[sp++] = n.left;
@@ -219,11 +279,21 @@ bool anyhit(const Accel& A, const orc_scene& s, V3 o, V3 d) {
 }
 
 // ------------------------------------------------------------ shading
-// src/Triangle.cpp:41-47 + src/BVH.cpp:371-381: point o + t*d, normal
-// normalize(n1*(1-u-v) + n2*u + n3*v), then normalised again after the
-// (identity) inverse-transpose transform.
+// src/Triangle.cpp:41-47 + src/BVH.cpp:371-381: point o + t*d and normal
+// normalize(n1*(1-u-v) + n2*u + n3*v) in the triangle's space, then the point
+// through M (w = 1) and the normal through the transpose of the inverse (w = 0),
+// normalised again.  Without a transformation these are the identity (the
+// point and the first normalisation pass through unchanged).
 inline void hit_geometry(const orc_scene& s, int tri, V3 o, V3 d, float t, float u, float v,
                          V3& P, V3& N) {
+    const int xi = xform_of(s, tri);
+    float inv[16], invT[16];
+    if (xi >= 0) {
+        inverse_glm(s.xforms + 16 * (size_t)xi, inv);
+        transpose4(inv, invT);
+        o = xform(inv, o, 1.0f);
+        d = xform(inv, d, 0.0f);
+    }
     P = o + mul(t, d);
     const float* n = s.normals + 9 * (size_t)tri;
     float w = (1.0f - u) - v;
@@ -231,7 +301,30 @@ inline void hit_geometry(const orc_scene& s, int tri, V3 o, V3 d, float t, float
     V3 a = {n1.x * w, n1.y * w, n1.z * w};
     V3 b = {n2.x * u, n2.y * u, n2.z * u};
     V3 c = {n3.x * v, n3.y * v, n3.z * v};
-    N = normalize(normalize((a + b) + c));
+    N = normalize((a + b) + c);
+    if (xi >= 0) {
+        P = xform(s.xforms + 16 * (size_t)xi, P, 1.0f);
+        N = xform(invT, N, 0.0f);
+    }
+    N = normalize(N);
+}
+
+// World-space vertices of the triangles (through M, w = 1) for bounding
+// boxes and build keys; the input array when there are no transformations.
+std::vector<float> world_verts(const orc_scene& s) {
+    std::vector<float> w(s.verts, s.verts + 9 * (size_t)s.n_tri);
+    if (!s.xform_index) return w;
+    for (int i = 0; i < s.n_tri; ++i) {
+        const int xi = s.xform_index[i];
+        if (xi < 0) continue;
+        for (int k = 0; k < 3; ++k) {
+            V3 p = xform(s.xforms + 16 * (size_t)xi, ld3(s.verts + 9 * (size_t)i + 3 * k), 1.0f);
+            w[9 * (size_t)i + 3 * k] = p.x;
+            w[9 * (size_t)i + 3 * k + 1] = p.y;
+            w[9 * (size_t)i + 3 * k + 2] = p.z;
+        }
+    }
+    return w;
 }
 
 // src/RayTracer.cpp:418-458 for one light (the light's w is 1, so the vec4
@@ -376,6 +469,8 @@ void orc_camera_ray(const orc_camera* cam, int x, int y, float* dir3) {
 
 uint8_t orc_quantize(float c) { return quantize(c); }
 
+void orc_mat4_inverse(const float* m, float* out) { inverse_glm(m, out); }
+
 uint32_t orc_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3(x, y, z); }
 
 float orc_shade_phong(const orc_scene* s, const float* cam_pos, int light, int mat, const float* p,
@@ -388,17 +483,18 @@ void* orc_accel_build(const orc_scene* s, int brute_force) {
     Accel* A = new Accel();
     A->brute = brute_force != 0;
     if (A->brute || s->n_tri == 0) return A;
+    const std::vector<float> wv = world_verts(*s);   // boxes in world space
     std::vector<double> cen(3 * (size_t)s->n_tri);
     for (int i = 0; i < s->n_tri; ++i)
         for (int k = 0; k < 3; ++k) {
-            const float* t = s->verts + 9 * (size_t)i;
+            const float* t = wv.data() + 9 * (size_t)i;
             cen[3 * (size_t)i + k] = ((double)t[k] + t[3 + k] + t[6 + k]) / 3.0;
         }
     A->idx.resize(s->n_tri);
     std::iota(A->idx.begin(), A->idx.end(), 0);
     A->nodes.reserve(2 * (size_t)s->n_tri / 2 + 8);
     A->nodes.push_back(ANode());
-    abuild(*A, s->verts, cen, 0, 0, s->n_tri);
+    abuild(*A, wv.data(), cen, 0, 0, s->n_tri);
     return A;
 }
 
@@ -538,7 +634,7 @@ void orc_karras(int32_t n, const uint32_t* k, int32_t* kchild, int32_t* kparent,
 void orc_lbvh_build(int32_t n, const float* verts, int32_t leaf_max, uint32_t* sorted_codes,
                     int32_t* perm_out, int32_t* kchild_out, int32_t* kparent_out, float* ibox_out,
                     float* nodes, float* tris, int32_t* root_ref, float* nodes4, int32_t* n_nodes4,
-                    int32_t* root4) {
+                    int32_t* root4, const float* xforms, const int32_t* xform_index) {
     if (n_nodes4) *n_nodes4 = 0;
     if (root4) *root4 = n <= 0 ? 0x7FFFFFFF : (n <= std::max(1, std::min(leaf_max, 8)) ? leaf_ref(0, n) : 0);
     if (n <= 0) {
@@ -547,6 +643,18 @@ void orc_lbvh_build(int32_t n, const float* verts, int32_t leaf_max, uint32_t* s
     }
     if (leaf_max < 1) leaf_max = 1;
     if (leaf_max > 8) leaf_max = 8;
+    // keys and boxes from world-space vertices; records keep the local ones
+    const float* local = verts;
+    std::vector<float> wv;
+    if (xform_index) {
+        orc_scene ws{};
+        ws.n_tri = n;
+        ws.verts = verts;
+        ws.xforms = xforms;
+        ws.xform_index = xform_index;
+        wv = world_verts(ws);
+        verts = wv.data();
+    }
     std::vector<uint32_t> codes(n);
     float cb[6];
     orc_morton_codes(n, verts, codes.data(), cb);
@@ -560,7 +668,7 @@ void orc_lbvh_build(int32_t n, const float* verts, int32_t leaf_max, uint32_t* s
     // triangle records in sorted order
     if (tris)
         for (int s = 0; s < n; ++s) {
-            const float* t = verts + 9 * (size_t)perm[s];
+            const float* t = local + 9 * (size_t)perm[s];
             float* r = tris + 12 * (size_t)s;
             V3 v1 = ld3(t), v2 = ld3(t + 3), v3 = ld3(t + 6);
             V3 e1 = v2 - v1, e2 = v3 - v1;
@@ -569,7 +677,9 @@ void orc_lbvh_build(int32_t n, const float* verts, int32_t leaf_max, uint32_t* s
             r[8] = e2.z;
             int32_t o = perm[s];
             std::memcpy(&r[9], &o, 4);
-            r[10] = 0.0f; r[11] = 0.0f;
+            const int32_t xi = xform_index ? xform_index[perm[s]] : -1;
+            std::memcpy(&r[10], &xi, 4);   // transformation index, -1 = none
+            r[11] = 0.0f;
         }
     float eps_scene = 0.0f;
     for (int k = 0; k < 6; ++k) eps_scene = std::max(eps_scene, std::fabs(cb[k]));
@@ -731,13 +841,35 @@ inline bool slab(const LRay& r, float lox, float hix, float loy, float hiy, floa
     return tmin <= tmx;
 }
 
+// One triangle record against a world-space ray, in the triangle's local space
+// when its record carries a transformation (inverses in xinv, [n][16]).
+inline bool rec_hit(const float* tris, const float* xinv, int32_t s, V3 o, V3 d, float& t, float& u, float& v,
+                    int32_t& orig) {
+    const float* tr = tris + 12 * (size_t)s;
+    V3 v1 = {tr[0], tr[1], tr[2]}, e1 = {tr[3], tr[4], tr[5]}, e2 = {tr[6], tr[7], tr[8]};
+    std::memcpy(&orig, &tr[9], 4);
+    int32_t xi;
+    std::memcpy(&xi, &tr[10], 4);
+    if (xi >= 0 && xinv) {
+        o = xform(xinv + 16 * (size_t)xi, o, 1.0f);
+        d = xform(xinv + 16 * (size_t)xi, d, 0.0f);
+    }
+    return mt_e(o, d, v1, e1, e2, t, u, v);
+}
+
+std::vector<float> inverses(int32_t n_xform, const float* xforms) {
+    std::vector<float> inv(16 * (size_t)std::max(n_xform, 0));
+    for (int i = 0; i < n_xform; ++i) inverse_glm(xforms + 16 * (size_t)i, inv.data() + 16 * (size_t)i);
+    return inv;
+}
+
 struct LHit { int32_t orig = -1, slot = -1; float t = INFINITY, u = 0, v = 0; };
 
 // The traversal order the HIP kernels implement: ordered (near child first,
 // child 0 on equal entry distance), far child pushed, pruning against the
 // current best t, leaves tested in slot order.  Returns visits/tests.
 template <bool ANY>
-bool lbvh_trace_one(const float* nodes, int32_t root_ref, const float* tris, V3 o, V3 d, LHit& h,
+bool lbvh_trace_one(const float* nodes, int32_t root_ref, const float* tris, const float* xinv, V3 o, V3 d, LHit& h,
                     int64_t& visits, int64_t& tests) {
     LRay r = make_lray(o, d);
     int32_t stack[128];
@@ -771,12 +903,9 @@ bool lbvh_trace_one(const float* nodes, int32_t root_ref, const float* tris, V3 
             int32_t first = e & 0x0FFFFFFF, cnt = ((e >> 28) & 7) + 1;
             for (int32_t s = first; s < first + cnt; ++s) {
                 tests++;
-                const float* tr = tris + 12 * (size_t)s;
-                V3 v1 = {tr[0], tr[1], tr[2]}, e1 = {tr[3], tr[4], tr[5]}, e2 = {tr[6], tr[7], tr[8]};
                 int32_t orig;
-                std::memcpy(&orig, &tr[9], 4);
                 float t, u, v;
-                if (mt_e(o, d, v1, e1, e2, t, u, v)) {
+                if (rec_hit(tris, xinv, s, o, d, t, u, v, orig)) {
                     if (ANY) { h.orig = orig; h.slot = s; h.t = t; h.u = u; h.v = v; return true; }
                     if (t < h.t || (t == h.t && orig < h.orig)) {
                         h.orig = orig; h.slot = s; h.t = t; h.u = u; h.v = v;
@@ -793,8 +922,8 @@ bool lbvh_trace_one(const float* nodes, int32_t root_ref, const float* tris, V3 
 // 4-wide: children hit sorted by (entry distance, slot); the nearest is taken,
 // the others pushed farthest first.
 template <bool ANY>
-bool lbvh4_trace_one(const float* nodes, int32_t root_ref, const float* tris, V3 o, V3 d, LHit& h,
-                     int64_t& visits, int64_t& tests) {
+bool lbvh4_trace_one(const float* nodes, int32_t root_ref, const float* tris, const float* xinv, V3 o, V3 d,
+                     LHit& h, int64_t& visits, int64_t& tests) {
     LRay r = make_lray(o, d);
     int32_t stack[256];
     int sp = 0;
@@ -836,12 +965,9 @@ bool lbvh4_trace_one(const float* nodes, int32_t root_ref, const float* tris, V3
             bool stop = false;
             for (int32_t s = first; s < first + cnt; ++s) {
                 tests++;
-                const float* tr = tris + 12 * (size_t)s;
-                V3 v1 = {tr[0], tr[1], tr[2]}, e1 = {tr[3], tr[4], tr[5]}, e2 = {tr[6], tr[7], tr[8]};
                 int32_t orig;
-                std::memcpy(&orig, &tr[9], 4);
                 float t, u, v;
-                if (mt_e(o, d, v1, e1, e2, t, u, v)) {
+                if (rec_hit(tris, xinv, s, o, d, t, u, v, orig)) {
                     if (ANY) { h.orig = orig; h.slot = s; h.t = t; h.u = u; h.v = v; stop = true; break; }
                     if (t < h.t || (t == h.t && orig < h.orig)) {
                         h.orig = orig; h.slot = s; h.t = t; h.u = u; h.v = v;
@@ -857,10 +983,10 @@ bool lbvh4_trace_one(const float* nodes, int32_t root_ref, const float* tris, V3
 }
 
 template <bool ANY>
-inline bool trace_w(int width, const float* nodes, int32_t root, const float* tris, V3 o, V3 d, LHit& h,
-                    int64_t& v, int64_t& t) {
-    return width == 4 ? lbvh4_trace_one<ANY>(nodes, root, tris, o, d, h, v, t)
-                      : lbvh_trace_one<ANY>(nodes, root, tris, o, d, h, v, t);
+inline bool trace_w(int width, const float* nodes, int32_t root, const float* tris, const float* xinv, V3 o, V3 d,
+                    LHit& h, int64_t& v, int64_t& t) {
+    return width == 4 ? lbvh4_trace_one<ANY>(nodes, root, tris, xinv, o, d, h, v, t)
+                      : lbvh_trace_one<ANY>(nodes, root, tris, xinv, o, d, h, v, t);
 }
 
 }  // namespace
@@ -869,15 +995,18 @@ extern "C" {
 
 void orc_lbvh_trace(const float* nodes, int32_t root_ref, const float* tris, int32_t n_rays,
                     const float* orig, const float* dirs, int32_t any_hit, int32_t* hit_orig, float* hit_t,
-                    float* hit_u, float* hit_v, int64_t* visits, int64_t* tests, int32_t width) {
+                    float* hit_u, float* hit_v, int64_t* visits, int64_t* tests, int32_t width, int32_t n_xform,
+                    const float* xforms) {
+    const std::vector<float> xv = inverses(n_xform, xforms);
+    const float* xinv = n_xform > 0 ? xv.data() : nullptr;
     int64_t vs = 0, ts = 0;
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : vs, ts)
     for (int i = 0; i < n_rays; ++i) {
         LHit h;
         V3 o = ld3(orig + 3 * (size_t)i), d = ld3(dirs + 3 * (size_t)i);
         int64_t v = 0, t = 0;
-        if (any_hit) trace_w<true>(width, nodes, root_ref, tris, o, d, h, v, t);
-        else trace_w<false>(width, nodes, root_ref, tris, o, d, h, v, t);
+        if (any_hit) trace_w<true>(width, nodes, root_ref, tris, xinv, o, d, h, v, t);
+        else trace_w<false>(width, nodes, root_ref, tris, xinv, o, d, h, v, t);
         vs += v; ts += t;
         if (hit_orig) hit_orig[i] = h.orig;
         if (hit_t) hit_t[i] = h.t;
@@ -890,7 +1019,9 @@ void orc_lbvh_trace(const float* nodes, int32_t root_ref, const float* tris, int
 
 void orc_lbvh4_packet_primary(const float* nodes4, int32_t root4, const float* tris, const orc_camera* cam,
                               int32_t* hit_orig, float* hit_t, float* hit_u, float* hit_v, int64_t* visits,
-                              int64_t* tests) {
+                              int64_t* tests, int32_t n_xform, const float* xforms) {
+    const std::vector<float> xv = inverses(n_xform, xforms);
+    const float* xinv = n_xform > 0 ? xv.data() : nullptr;
     const int W = cam->width, H = cam->height;
     const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
     const int n_tiles = tiles_x * tiles_y;
@@ -953,13 +1084,10 @@ void orc_lbvh4_packet_primary(const float* nodes4, int32_t root4, const float* t
                     const int32_t s1 = s0 + 1 <= last ? s0 + 1 : s0;
                     ts += s1 != s0 ? 2 : 1;
                     for (int32_t s : {s0, s1}) {
-                        const float* tr = tris + 12 * (size_t)s;
-                        const V3 v1 = {tr[0], tr[1], tr[2]}, e1 = {tr[3], tr[4], tr[5]}, e2 = {tr[6], tr[7], tr[8]};
-                        int32_t orig;
-                        std::memcpy(&orig, &tr[9], 4);
                         for (int i = 0; i < 64; ++i) {
                             float t, u, v;
-                            if (mt_e(o, dir[i], v1, e1, e2, t, u, v) &&
+                            int32_t orig;
+                            if (rec_hit(tris, xinv, s, o, dir[i], t, u, v, orig) &&
                                 (t < h[i].t || (t == h[i].t && orig < h[i].orig))) {
                                 h[i].orig = orig; h[i].slot = s; h[i].t = t; h[i].u = u; h[i].v = v;
                             }
@@ -993,6 +1121,8 @@ int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, co
     for (int y = row_start; y < H; y += row_step) rows.push_back(y);
     const int nrows = (int)rows.size();
     const bool empty = s->n_tri == 0;
+    const std::vector<float> xv = inverses(s->xform_index ? s->n_xform : 0, s->xforms);
+    const float* xinv = s->xform_index && s->n_xform > 0 ? xv.data() : nullptr;
 #pragma omp parallel for schedule(dynamic, 2) reduction(+ : prim, shad, sec, hits, vis, tst)
     for (int ri = 0; ri < nrows; ++ri) {
         int y = rows[ri];
@@ -1008,14 +1138,14 @@ int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, co
                 [&](V3 o, V3 dd, Hit& h) {
                     if (empty) return;
                     LHit lh;
-                    if (trace_w<false>(width, nodes, root_ref, tris, o, dd, lh, v, t)) {
+                    if (trace_w<false>(width, nodes, root_ref, tris, xinv, o, dd, lh, v, t)) {
                         h.tri = lh.orig; h.t = lh.t; h.u = lh.u; h.v = lh.v;
                     }
                 },
                 [&](V3 o, V3 dd) {
                     if (empty) return false;
                     LHit lh;
-                    return trace_w<true>(width, nodes, root_ref, tris, o, dd, lh, v, t);
+                    return trace_w<true>(width, nodes, root_ref, tris, xinv, o, dd, lh, v, t);
                 },
                 col, hit_ids ? hit_ids + p : nullptr, nullptr, 0, c);
             if (rgb)

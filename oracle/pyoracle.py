@@ -23,7 +23,32 @@ assert REC_DTYPE.itemsize == 64
 class OrcScene(C.Structure):
     _fields_ = [("n_tri", C.c_int32), ("verts", C.c_void_p), ("normals", C.c_void_p),
                 ("mat", C.c_void_p), ("n_mat", C.c_int32), ("mats", C.c_void_p),
-                ("n_light", C.c_int32), ("lights", C.c_void_p)]
+                ("n_light", C.c_int32), ("lights", C.c_void_p),
+                ("n_xform", C.c_int32), ("xforms", C.c_void_p), ("xform_index", C.c_void_p)]
+
+
+def _xform_arrays(scene):
+    """(n_xform, xforms [T][16] float32, xform_index [N] int32) or (0, None, None)."""
+    xf = getattr(scene, "xforms", None)
+    xi = getattr(scene, "xform_index", None)
+    if xf is None or xi is None:
+        return 0, None, None
+    xf = np.ascontiguousarray(xf, np.float32).reshape(-1, 16)
+    xi = np.ascontiguousarray(xi, np.int32)
+    return len(xf), xf, xi
+
+
+def _orc_scene(holder, scene):
+    """Fills holder's arrays from the scene and returns an OrcScene over them."""
+    holder.verts = np.ascontiguousarray(scene.verts, np.float32)
+    holder.normals = np.ascontiguousarray(scene.normals, np.float32)
+    holder.mat = np.ascontiguousarray(scene.mat, np.int32)
+    holder.materials = np.ascontiguousarray(scene.materials)
+    holder.lights = np.ascontiguousarray(scene.lights)
+    holder.n_xform, holder.xforms, holder.xform_index = _xform_arrays(scene)
+    return OrcScene(holder.verts.shape[0], _p(holder.verts), _p(holder.normals), _p(holder.mat),
+                    len(holder.materials), _p(holder.materials), len(holder.lights), _p(holder.lights),
+                    holder.n_xform, _p(holder.xforms), _p(holder.xform_index))
 
 
 class OrcCounts(C.Structure):
@@ -54,6 +79,7 @@ def lib():
         L.orc_render.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.orc_render_lbvh.argtypes = [vp, vp, i32, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, i32]
         L.orc_mt.argtypes = [vp, vp, vp, vp]
+        L.orc_mat4_inverse.argtypes = [vp, vp]
         L.orc_camera_ray.argtypes = [vp, C.c_int, C.c_int, vp]
         L.orc_quantize.restype = C.c_uint8
         L.orc_quantize.argtypes = [C.c_float]
@@ -62,10 +88,10 @@ def lib():
         L.orc_shade_phong.restype = C.c_float
         L.orc_shade_phong.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
         L.orc_morton_codes.argtypes = [i32, vp, vp, vp]
-        L.orc_lbvh_build.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_lbvh_build.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_karras.argtypes = [i32, vp, vp, vp, vp, vp]
-        L.orc_lbvh_trace.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, i32]
-        L.orc_lbvh4_packet_primary.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_lbvh_trace.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
+        L.orc_lbvh4_packet_primary.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp]
         L.orc_loss64.restype = C.c_double
         L.orc_loss64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, vp]
         L.orc_grad_fd64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, C.c_double, vp, i32, vp]
@@ -82,13 +108,7 @@ class Oracle:
 
     def __init__(self, scene, brute_force: bool = False):
         self.scene = scene
-        self.verts = np.ascontiguousarray(scene.verts, np.float32)
-        self.normals = np.ascontiguousarray(scene.normals, np.float32)
-        self.mat = np.ascontiguousarray(scene.mat, np.int32)
-        self.materials = np.ascontiguousarray(scene.materials)
-        self.lights = np.ascontiguousarray(scene.lights)
-        self._s = OrcScene(self.verts.shape[0], _p(self.verts), _p(self.normals), _p(self.mat),
-                           len(self.materials), _p(self.materials), len(self.lights), _p(self.lights))
+        self._s = _orc_scene(self, scene)
         self._accel = lib().orc_accel_build(C.byref(self._s), 1 if brute_force else 0)
 
     def set_params(self, materials=None, lights=None):
@@ -165,8 +185,12 @@ def karras(sorted_codes):
     return kc[:2 * (n - 1)].reshape(-1, 2), kp[:2 * n - 1], fi[:n - 1], la[:n - 1]
 
 
-def lbvh_build(verts, leaf_max=1):
+def lbvh_build(verts, leaf_max=1, xforms=None, xform_index=None):
+    """CPU LBVH restatement.  With transformations, keys and boxes come from the
+    world-space vertices and the records keep the local ones."""
     v = np.ascontiguousarray(verts, np.float32)
+    xf = None if xforms is None else np.ascontiguousarray(xforms, np.float32).reshape(-1, 16)
+    xi = None if xform_index is None else np.ascontiguousarray(xform_index, np.int32)
     n = v.shape[0]
     m = max(n - 1, 1)
     out = {
@@ -177,11 +201,11 @@ def lbvh_build(verts, leaf_max=1):
     n4 = np.zeros(1, np.int32)
     r4 = np.zeros(1, np.int32)
     lib().orc_lbvh_build(n, _p(v), leaf_max, None, None, None, None, None, None, None, None, None,
-                         _p(n4), None)
+                         _p(n4), None, _p(xf), _p(xi))
     out["nodes4"] = np.zeros(32 * max(int(n4[0]), 1), np.float32)
     lib().orc_lbvh_build(n, _p(v), leaf_max, _p(out["codes"]), _p(out["perm"]), _p(out["kchild"]),
                          _p(out["kparent"]), _p(out["ibox"]), _p(out["nodes"]), _p(out["tris"]),
-                         _p(out["root"]), _p(out["nodes4"]), _p(n4), _p(r4))
+                         _p(out["root"]), _p(out["nodes4"]), _p(n4), _p(r4), _p(xf), _p(xi))
     out["nodes4"] = out["nodes4"][:32 * int(n4[0])].reshape(-1, 32)
     out["root4"] = int(r4[0])
     out["kchild"] = out["kchild"][:2 * (n - 1)].reshape(-1, 2)
@@ -192,7 +216,7 @@ def lbvh_build(verts, leaf_max=1):
     return out
 
 
-def lbvh_trace(b, origins, dirs, any_hit=False, width=2):
+def lbvh_trace(b, origins, dirs, any_hit=False, width=2, xforms=None):
     o = np.ascontiguousarray(origins, np.float32)
     d = np.ascontiguousarray(dirs, np.float32)
     n = o.shape[0]
@@ -205,8 +229,10 @@ def lbvh_trace(b, origins, dirs, any_hit=False, width=2):
     nodes = np.ascontiguousarray(b["nodes4"] if width == 4 else b["nodes"], np.float32)
     tris = np.ascontiguousarray(b["tris"], np.float32)
     root = b["root4"] if width == 4 else b["root"]
+    xf = None if xforms is None else np.ascontiguousarray(xforms, np.float32).reshape(-1, 16)
     lib().orc_lbvh_trace(_p(nodes), root, _p(tris), n, _p(o), _p(d), 1 if any_hit else 0,
-                         _p(hit), _p(t), _p(u), _p(v), _p(vis), _p(tst), width)
+                         _p(hit), _p(t), _p(u), _p(v), _p(vis), _p(tst), width, 0 if xf is None else len(xf),
+                         _p(xf))
     return {"hit": hit, "t": t, "u": u, "v": v, "visits": int(vis[0]), "tests": int(tst[0])}
 
 
@@ -223,9 +249,19 @@ def packet_primary(scene, b, camera=None):
     tst = np.zeros(1, np.int64)
     nodes = np.ascontiguousarray(b["nodes4"], np.float32)
     tris = np.ascontiguousarray(b["tris"], np.float32)
+    nx, xf, _ = _xform_arrays(scene)
     lib().orc_lbvh4_packet_primary(_p(nodes), b["root4"], _p(tris), cam.ctypes.data, _p(hit), _p(t), _p(u), _p(v),
-                                   _p(vis), _p(tst))
+                                   _p(vis), _p(tst), nx, _p(xf))
     return {"hit": hit, "t": t, "u": u, "v": v, "visits": int(vis[0]), "tests": int(tst[0])}
+
+
+def mat4_inverse(m):
+    """glm 0.9.6 inverse of a column-major 4x4 float32 matrix ([16] or [4][4]
+    with m[c][r]), in glm's operation order."""
+    a = np.ascontiguousarray(m, np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    lib().orc_mat4_inverse(_p(a), _p(out))
+    return out.reshape(np.shape(m))
 
 
 def camera_rays(scene, camera=None):
@@ -241,13 +277,8 @@ def camera_rays(scene, camera=None):
 
 def render_lbvh(scene, b, camera=None, depth=None, row_start=0, row_step=1, width=2):
     o = Oracle.__new__(Oracle)
-    verts = np.ascontiguousarray(scene.verts, np.float32)
-    normals = np.ascontiguousarray(scene.normals, np.float32)
-    mat = np.ascontiguousarray(scene.mat, np.int32)
-    mats = np.ascontiguousarray(scene.materials)
-    lights = np.ascontiguousarray(scene.lights)
-    s = OrcScene(verts.shape[0], _p(verts), _p(normals), _p(mat), len(mats), _p(mats),
-                 len(lights), _p(lights))
+    o._accel = None
+    s = _orc_scene(o, scene)
     cam = np.array(scene.camera if camera is None else camera, copy=True)
     depth = scene.depth if depth is None else depth
     W, H = int(cam["width"]), int(cam["height"])
@@ -346,13 +377,9 @@ class RefPort:
     def __init__(self, scene, parity: bool = False):
         self.L = refport_lib(parity)
         self.scene = scene
-        self.verts = np.ascontiguousarray(scene.verts, np.float32)
-        self.normals = np.ascontiguousarray(scene.normals, np.float32)
-        self.mat = np.ascontiguousarray(scene.mat, np.int32)
-        self.materials = np.ascontiguousarray(scene.materials)
-        self.lights = np.ascontiguousarray(scene.lights)
-        self._s = OrcScene(self.verts.shape[0], _p(self.verts), _p(self.normals), _p(self.mat),
-                           len(self.materials), _p(self.materials), len(self.lights), _p(self.lights))
+        if _xform_arrays(scene)[0]:
+            raise ValueError("the CPU baseline port has identity transformations only")
+        self._s = _orc_scene(self, scene)
         bs = np.zeros(1, np.float64)
         self.h = self.L.rp_build(C.byref(self._s), bs.ctypes.data)
         self.build_seconds = float(bs[0])
