@@ -17,6 +17,10 @@ Also reported:
                 by the launch's mean duration from HIP events on its stream.
   cpu_baseline  the timing-faithful port of the reference's OpenMP CPU path on a
                 stratified pixel sample of the same frame.
+  instanced     config F: one 65,536-triangle mesh placed 64 times through
+                per-triangle transformations (4,194,304 instanced triangles),
+                3840x2160, depth 2, against the same triangles moved to world
+                space beforehand (the cost of local-space tests).
 """
 from __future__ import annotations
 
@@ -133,6 +137,56 @@ def bench_batched(E, torch, reps: int):
             "launch_config": tuned, "launch_config_source": source, "hipgraph": True}
 
 
+def _world_space(scenes, s):
+    """The scene with its transformations applied to the vertices and normals
+    (float64, rounded once): the same frame without per-triangle transforms."""
+    import numpy as np
+    v = s.verts.astype(np.float64)
+    n = s.normals.astype(np.float64)
+    for k, m in enumerate(s.xforms):
+        M = m.reshape(4, 4).T.astype(np.float64)
+        sel = s.xform_index == k
+        v[sel] = v[sel] @ M[:3, :3].T + M[:3, 3]
+        n[sel] = n[sel] @ np.linalg.inv(M[:3, :3])
+    return scenes.SceneData(v.astype(np.float32), n.astype(np.float32), s.mat, s.materials, s.lights, s.camera,
+                            s.depth, s.name + " (world space)")
+
+
+def bench_instanced(E, torch, reps: int):
+    """Config F frames with per-triangle transformations vs the same geometry in
+    world space; frames written to device memory, GPU time from HIP events."""
+    import numpy as np
+    from eos_amd import scenes
+    t0 = time.perf_counter()
+    s = scenes.config_f()
+    w = _world_space(scenes, s)
+    _progress(f"instanced: {s.n_tri} triangles ({s.info['instances']} x {s.info['per_instance']} + plane) "
+              f"generated in {time.perf_counter() - t0:.1f} s")
+    out = torch.empty(s.width * s.height * 4, dtype=torch.uint8, device="cuda:0")
+    res = {"workload": "config F: 64 instances x 65,536-tri displaced sphere via per-triangle transforms "
+                       "+ 2,048-tri plane, 3840x2160, 2 lights, depth 2",
+           "triangles": s.n_tri, "instances": s.info["instances"]}
+    for key, sc in (("transformed", s), ("world_space", w)):
+        g = E.Scene(sc, leaf_max=4)
+        build_ms = g.build_stats["build_ms"]
+        g.render_device(s.camera, s.depth, out.data_ptr())
+        counted = g.render_device(s.camera, s.depth, out.data_ptr(), count_visits=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        per = [g.render_device(s.camera, s.depth, out.data_ptr()) for _ in range(reps)]
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t) / reps
+        st = per[-1]
+        rays = st["primary_rays"] + st["shadow_rays"] + st["secondary_rays"]
+        res[key] = {"mrays_per_s": round(rays / wall / 1e6, 2), "ms_per_frame": round(wall * 1e3, 3),
+                    "gpu_render_ms": round(float(np.mean([p["render_ms"] for p in per])), 4),
+                    "rays_per_frame": int(rays), "hits": int(st["hits"]), "lbvh_build_ms": round(build_ms, 3),
+                    "node_visits": int(counted["node_visits"]), "tri_tests": int(counted["tri_tests"])}
+        del g
+    res["transform_cost"] = round(res["transformed"]["ms_per_frame"] / res["world_space"]["ms_per_frame"], 3)
+    return res
+
+
 def bench_adjoint(E, torch, reps: int, check: bool):
     """Config E: forward frame vs forward + hand-written adjoint (records replayed)."""
     import numpy as np
@@ -216,6 +270,7 @@ def main() -> int:
     ap.add_argument("--cpu-stride", type=int, default=32)
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-instanced", action="store_true")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="before the frames, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for calibrating rocprofv3 byte counters)")
@@ -327,6 +382,10 @@ def main() -> int:
     if not args.skip_adjoint:
         result["adjoint"] = bench_adjoint(E, torch, reps=3, check=not args.no_cpu)
         _progress(f"adjoint: fwd+adj / fwd = {result['adjoint']['ratio']}")
+    if not args.skip_instanced:
+        result["instanced"] = bench_instanced(E, torch, reps=5)
+        _progress(f"instanced: {result['instanced']['transformed']['mrays_per_s']} Mrays/s, "
+                  f"{result['instanced']['transform_cost']}x the world-space frame")
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         _progress(f"CPU baseline (reference port, {threads} threads)")

@@ -15,13 +15,13 @@ LIB_PATH = os.path.join(_HERE, "libeos_amd.so")
 EOS_OK, EOS_ERR_INVALID, EOS_ERR_HIP, EOS_ERR_NOMEM, EOS_ERR_STATE, EOS_ERR_LIMIT, EOS_ERR_NODEVICE = range(7)
 EOS_MEM_HOST, EOS_MEM_DEVICE = 0, 1
 DBG = {"morton_sorted": 1, "perm": 2, "kchild": 3, "kparent": 4, "ibox": 5, "nodes": 6, "tris": 7,
-       "morton": 8, "nodes4": 9}
+       "morton": 8, "nodes4": 9, "xforms": 10}
 
 # Every symbol include/eos_c.h declares.
 EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_build_lbvh_ex", "eos_update_materials",
            "eos_update_lights", "eos_render", "eos_render_adjoint", "eos_render_batch",
            "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
-           "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free"]
+           "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free", "eos_set_transforms"]
 
 
 class EosError(RuntimeError):
@@ -48,7 +48,8 @@ class BuildOpts(C.Structure):
 
 class RenderOpts(C.Structure):
     _fields_ = [("depth", C.c_int32), ("output_memory", C.c_int32), ("rgb_out", C.c_void_p),
-                ("hit_out", C.c_void_p), ("keep_records", C.c_int32), ("count_visits", C.c_int32)]
+                ("hit_out", C.c_void_p), ("keep_records", C.c_int32), ("count_visits", C.c_int32),
+                ("aa_samples", C.c_int32)]
 
 
 class RenderStats(C.Structure):
@@ -127,6 +128,7 @@ def lib():
         L.eos_device_name.argtypes = [C.c_char_p, i32]
         L.eos_mesh_load_obj.argtypes = [C.c_char_p, vp]
         L.eos_mesh_free.argtypes = [vp]
+        L.eos_set_transforms.argtypes = [vp, i32, vp, vp, i32]
         for name in EXPORTS:
             if name != "eos_last_error" and name != "eos_abi_version":
                 getattr(L, name).restype = C.c_int
@@ -179,12 +181,23 @@ class Scene:
         check(lib().eos_create_scene(C.byref(d), C.byref(h)))
         self.h = h
         self.build_stats = None
+        xf = getattr(scene, "xforms", None)
+        if xf is not None:
+            self.set_transforms(xf, scene.xform_index)
         if build:
             self.build(leaf_max, width)
 
     @property
     def handle(self):
         return self.h
+
+    def set_transforms(self, xforms, xform_index):
+        """eos_set_transforms: [T][16] column-major float32 matrices and a per-triangle
+        index (-1 = world space).  The scene must be built again before rendering."""
+        self._xf = np.ascontiguousarray(xforms, np.float32).reshape(-1, 16)
+        self._xi = np.ascontiguousarray(xform_index, np.int32)
+        check(lib().eos_set_transforms(self.h, len(self._xf), _ptr(self._xf), _ptr(self._xi), EOS_MEM_HOST))
+        self.n_xforms = len(self._xf)
 
     def build(self, leaf_max: int = 4, width: int = 4) -> dict:
         st = BuildStats()
@@ -230,23 +243,25 @@ class Scene:
         return {k: getattr(c, k) for k, _ in c._fields_}
 
     def render(self, camera=None, depth=None, want_rgb=False, want_hits=False, keep_records=False,
-               count_visits=False, want_rgba=True):
+               count_visits=False, want_rgba=True, aa_samples=0):
         cam = np.array(self.data.camera if camera is None else camera, copy=True)
         depth = self.data.depth if depth is None else depth
         W, H = int(cam["width"]), int(cam["height"])
         rgba = np.zeros((H, W, 4), np.uint8) if want_rgba else None
         rgb = np.zeros((H, W, 3), np.float32) if want_rgb else None
         hits = np.zeros((H, W), np.int32) if want_hits else None
-        o = RenderOpts(depth, EOS_MEM_HOST, _ptr(rgb), _ptr(hits), int(keep_records), int(count_visits))
+        o = RenderOpts(depth, EOS_MEM_HOST, _ptr(rgb), _ptr(hits), int(keep_records), int(count_visits),
+                       int(aa_samples))
         st = RenderStats()
         check(lib().eos_render(self.h, cam.ctypes.data, C.byref(o), _ptr(rgba), C.byref(st)))
         return {"rgba": rgba, "rgb": rgb, "hit": hits, "stats": st.as_dict()}
 
     def render_device(self, camera, depth, rgba_dev_ptr, rgb_dev_ptr=None, hit_dev_ptr=None,
-                      keep_records=False, count_visits=False) -> dict:
+                      keep_records=False, count_visits=False, aa_samples=0) -> dict:
         """Render into device buffers (raw pointers, e.g. torch tensors' data_ptr())."""
         cam = np.array(camera, copy=True)
-        o = RenderOpts(depth, EOS_MEM_DEVICE, rgb_dev_ptr, hit_dev_ptr, int(keep_records), int(count_visits))
+        o = RenderOpts(depth, EOS_MEM_DEVICE, rgb_dev_ptr, hit_dev_ptr, int(keep_records), int(count_visits),
+                       int(aa_samples))
         st = RenderStats()
         check(lib().eos_render(self.h, cam.ctypes.data, C.byref(o), rgba_dev_ptr, C.byref(st)))
         return st.as_dict()
@@ -279,7 +294,8 @@ class Scene:
                   "kchild": ((m, 2), np.int32), "kparent": ((max(2 * n - 1, 0),), np.int32),
                   "ibox": ((m, 6), np.float32), "nodes": ((m, 16), np.float32),
                   "tris": ((n, 12), np.float32), "morton": ((n,), np.uint32),
-                  "nodes4": ((self.build_stats["num_nodes4"] if self.build_stats else 0, 32), np.float32)}
+                  "nodes4": ((self.build_stats["num_nodes4"] if self.build_stats else 0, 32), np.float32),
+                  "xforms": ((getattr(self, "n_xforms", 0), 36), np.float32)}
         shape, dt = shapes[what]
         out = np.zeros(shape, dt)
         check(lib().eos_debug_read(self.h, DBG[what], _ptr(out), out.nbytes))

@@ -1,6 +1,8 @@
 // eos_api.cpp -- C ABI entry points: scene lifetime, parameters, diagnostics.
+#include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "eos_internal.h"
 
@@ -46,6 +48,8 @@ SceneView view_of(const eos_scene* s) {
     v.shade = s->shade.as<const float4>();
     v.mats = s->mats.as<const DevMaterial>();
     v.lights = s->lights.as<const DevLight>();
+    v.xf = s->n_xf > 0 ? s->xf.as<const float4>() : nullptr;
+    v.n_xf = s->n_xf;
     v.root = s->root;
     v.n_tri = s->n_tri;
     v.n_light = s->n_light;
@@ -66,8 +70,9 @@ static void free_scene(eos_scene* s) {
                       &s->shade, &s->codes, &s->codes_sorted, &s->perm, &s->kchild, &s->kparent,
                       &s->first_last, &s->ibox, &s->lbox, &s->flags, &s->cbounds, &s->sort_tmp_k,
                       &s->sort_tmp_v, &s->sort_hist, &s->bounds_part, &s->nodes4, &s->flags4, &s->scan_tmp,
+                      &s->xf, &s->xf_index, &s->wverts,
                       &s->fb.hit0, &s->fb.q[0], &s->fb.q[1],
-                      &s->fb.hitq, &s->fb.levcol, &s->fb.levels, &s->fb.counters, &s->fb.gstack};
+                      &s->fb.hitq, &s->fb.levcol, &s->fb.levels, &s->fb.counters, &s->fb.gstack, &s->fb.aa_acc};
     for (DevBuf* b : bufs) b->release();
     s->fb.surf.release();
     s->fb.vis.release();
@@ -82,6 +87,8 @@ static void free_scene(eos_scene* s) {
 static eos_status check_materials(const eos_material* m, int n) {
     for (int i = 0; i < n; ++i) {
         if (!(m[i].shininess >= 0.0f)) return fail(EOS_ERR_INVALID, "material shininess must be >= 0");
+        if (m[i].transparent)
+            return fail(EOS_ERR_INVALID, "transparent materials are not rendered by this build yet");
     }
     return EOS_OK;
 }
@@ -91,6 +98,39 @@ static eos_status check_lights(const eos_light* l, int n) {
         if (l[i].position[3] != 1.0f)
             return fail(EOS_ERR_INVALID, "light position w must be 1 (src/RayTracer.cpp:433 subtracts a w=1 point)");
     return EOS_OK;
+}
+
+// glm 0.9.6 compute_inverse for a column-major 4x4 (glm/detail/type_mat4x4.inl:
+// 37-96): the 2x2 cofactors of the lower rows, combined per column as
+// (a - b) + c with alternating signs, the determinant from the first row and
+// column, and one reciprocal that scales every entry.  Float, in that order, so
+// the inverse is the one the reference's glm::inverse computes.
+static void inverse_glm(const float* m, float* out) {
+    auto M = [&](int c, int r) { return m[4 * c + r]; };
+    const float a[18] = {
+        M(2, 2) * M(3, 3) - M(3, 2) * M(2, 3), M(1, 2) * M(3, 3) - M(3, 2) * M(1, 3),
+        M(1, 2) * M(2, 3) - M(2, 2) * M(1, 3), M(2, 1) * M(3, 3) - M(3, 1) * M(2, 3),
+        M(1, 1) * M(3, 3) - M(3, 1) * M(1, 3), M(1, 1) * M(2, 3) - M(2, 1) * M(1, 3),
+        M(2, 1) * M(3, 2) - M(3, 1) * M(2, 2), M(1, 1) * M(3, 2) - M(3, 1) * M(1, 2),
+        M(1, 1) * M(2, 2) - M(2, 1) * M(1, 2), M(2, 0) * M(3, 3) - M(3, 0) * M(2, 3),
+        M(1, 0) * M(3, 3) - M(3, 0) * M(1, 3), M(1, 0) * M(2, 3) - M(2, 0) * M(1, 3),
+        M(2, 0) * M(3, 2) - M(3, 0) * M(2, 2), M(1, 0) * M(3, 2) - M(3, 0) * M(1, 2),
+        M(1, 0) * M(2, 2) - M(2, 0) * M(1, 2), M(2, 0) * M(3, 1) - M(3, 0) * M(2, 1),
+        M(1, 0) * M(3, 1) - M(3, 0) * M(1, 1), M(1, 0) * M(2, 1) - M(2, 0) * M(1, 1)};
+    // factor k of cofactor group g: group g's first entry serves columns 0 and 1
+    auto F = [&](int g, int k) { return a[3 * g + (k == 0 ? 0 : k - 1)]; };
+    auto V = [&](int r, int k) { return k == 0 ? M(1, r) : M(0, r); };
+    float inv[16];
+    for (int k = 0; k < 4; ++k) {
+        const float sa = (k & 1) ? -1.0f : 1.0f, sb = -sa;
+        inv[0 + k] = ((V(1, k) * F(0, k) - V(2, k) * F(1, k)) + V(3, k) * F(2, k)) * sa;
+        inv[4 + k] = ((V(0, k) * F(0, k) - V(2, k) * F(3, k)) + V(3, k) * F(4, k)) * sb;
+        inv[8 + k] = ((V(0, k) * F(1, k) - V(1, k) * F(3, k)) + V(3, k) * F(5, k)) * sa;
+        inv[12 + k] = ((V(0, k) * F(2, k) - V(1, k) * F(4, k)) + V(2, k) * F(5, k)) * sb;
+    }
+    const float det = (M(0, 0) * inv[0] + M(0, 1) * inv[4]) + (M(0, 2) * inv[8] + M(0, 3) * inv[12]);
+    const float one_over = 1.0f / det;
+    for (int k = 0; k < 16; ++k) out[k] = inv[k] * one_over;
 }
 
 }  // namespace eos
@@ -193,6 +233,50 @@ eos_status eos_destroy_scene(eos_scene* s) {
     return EOS_OK;
 }
 
+eos_status eos_set_transforms(eos_scene* s, int32_t nt, const float* mats, const int32_t* index, int32_t memory) {
+    if (!s) return fail(EOS_ERR_INVALID, "eos_set_transforms: null scene");
+    if (nt < 0) return fail(EOS_ERR_INVALID, "eos_set_transforms: negative transformation count");
+    if (nt > 0 && (!mats || (s->n_tri > 0 && !index)))
+        return fail(EOS_ERR_INVALID, "eos_set_transforms: null transformation arrays");
+    if (memory != EOS_MEM_HOST && memory != EOS_MEM_DEVICE)
+        return fail(EOS_ERR_INVALID, "eos_set_transforms: index_memory must be EOS_MEM_HOST or EOS_MEM_DEVICE");
+    EOS_HIP(hipSetDevice(s->device));
+    s->built = false;  // the LBVH is built over world-space vertices
+    if (nt == 0) {
+        s->n_xf = 0;
+        return EOS_OK;
+    }
+    std::vector<float> rows((size_t)nt * kXfRows * 4);
+    for (int t = 0; t < nt; ++t) {
+        const float* m = mats + 16 * (size_t)t;
+        float inv[16];
+        inverse_glm(m, inv);
+        bool finite = true;
+        for (float x : inv) finite &= std::isfinite(x);
+        if (!finite)
+            return fail(EOS_ERR_INVALID, "eos_set_transforms: transformation " + std::to_string(t) +
+                                             " is not invertible");
+        float* r = rows.data() + (size_t)t * kXfRows * 4;
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 4; ++c) {
+                r[4 * (kXfInv + k) + c] = inv[4 * c + k];
+                r[4 * (kXfM + k) + c] = m[4 * c + k];
+                r[4 * (kXfInvT + k) + c] = inv[4 * k + c];  // transpose(inverse)[c][k] = inverse[k][c]
+            }
+    }
+    if (memory == EOS_MEM_HOST)
+        for (int i = 0; i < s->n_tri; ++i)
+            if (index[i] < -1 || index[i] >= nt)
+                return fail(EOS_ERR_INVALID, "eos_set_transforms: transformation index out of range at triangle " +
+                                                 std::to_string(i));
+    s->n_xf = 0;
+    EOS_TRY(copy_in(s->xf, rows.data(), rows.size() * sizeof(float), EOS_MEM_HOST, s->stream));
+    EOS_TRY(copy_in(s->xf_index, index, (size_t)s->n_tri * sizeof(int32_t), memory, s->stream));
+    EOS_HIP(hipStreamSynchronize(s->stream));
+    s->n_xf = nt;
+    return EOS_OK;
+}
+
 eos_status eos_update_materials(eos_scene* s, const eos_material* m, int32_t n) {
     if (!s || !m) return fail(EOS_ERR_INVALID, "eos_update_materials: null argument");
     if (n != s->n_mat) return fail(EOS_ERR_INVALID, "eos_update_materials: material count differs from the scene's");
@@ -234,6 +318,10 @@ eos_status eos_render(eos_scene* s, const eos_camera* cam, const eos_render_opts
     if (cam->width <= 0 || cam->height <= 0 || (int64_t)cam->width * cam->height > (int64_t)1 << 30)
         return fail(EOS_ERR_INVALID, "eos_render: bad image size");
     if (o->depth < 0 || o->depth > EOS_MAX_DEPTH) return fail(EOS_ERR_INVALID, "eos_render: depth must be in [0, 8]");
+    if (o->aa_samples < 0 || o->aa_samples > EOS_MAX_AA)
+        return fail(EOS_ERR_INVALID, "eos_render: aa_samples must be in [0, 16]");
+    if (o->aa_samples > 1 && o->keep_records)
+        return fail(EOS_ERR_INVALID, "eos_render: keep_records needs one sample per pixel (aa_samples 0 or 1)");
     EOS_HIP(hipSetDevice(s->device));
     return render(s, cam, o, rgba, st);
 }
@@ -309,6 +397,7 @@ eos_status eos_debug_read(const eos_scene* s, int32_t what, void* dst, int64_t b
         case EOS_DBG_TRIS: b = &s->geo; need = 48 * n; break;
         case EOS_DBG_MORTON: b = &s->codes; need = 4 * n; break;
         case EOS_DBG_NODES4: b = &s->nodes4; need = 128 * (size_t)s->n_nodes4; break;
+        case EOS_DBG_XFORMS: b = &s->xf; need = 16 * kXfRows * (size_t)s->n_xf; break;
         default: return fail(EOS_ERR_INVALID, "eos_debug_read: unknown array");
     }
     if (!s->built) return fail(EOS_ERR_STATE, "eos_debug_read: no LBVH built");

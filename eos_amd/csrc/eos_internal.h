@@ -62,7 +62,9 @@ struct DevMaterial {
     float kd[4];
     float ks[4];
     float shininess, reflectivity;
-    int32_t reflective, pad;
+    int32_t reflective, transparent;
+    float refractive_index;
+    int32_t pad[3];
 };
 struct DevLight {
     float position[4];
@@ -76,6 +78,14 @@ static_assert(sizeof(DevMaterial) == sizeof(eos_material), "material layout");
 static_assert(sizeof(DevLight) == sizeof(eos_light), "light layout");
 static_assert(sizeof(DevCamera) == sizeof(eos_camera), "camera layout");
 
+// Per-triangle transformation on the device: rows 0..2 of the inverse, of M
+// and of transpose(inverse), each row (m[r], m[4+r], m[8+r], m[12+r]) of the
+// column-major matrix, so row r of M * (x, y, z, w) is
+// (row.x*x + row.y*y) + (row.z*z + row.w*w), glm's order.  Row 3 is never
+// needed: only the xyz of a transformed point or vector is used.
+constexpr int kXfRows = 9;
+constexpr int kXfInv = 0, kXfM = 3, kXfInvT = 6;
+
 // Device view of one scene, passed by value to kernels.
 struct SceneView {
     const Node* nodes;
@@ -84,12 +94,14 @@ struct SceneView {
     const float4* shade;
     const DevMaterial* mats;
     const DevLight* lights;
+    const float4* xf;         // [n_xf][kXfRows] transformations (null: none)
     int32_t root;
     int32_t root4;
     int32_t width;            // 2 or 4: which tree the traversal walks
     int32_t n_tri;
     int32_t n_light;
     int32_t n_mat;
+    int32_t n_xf;
 };
 
 // ---------------------------------------------------------------- buffers
@@ -132,6 +144,7 @@ struct eos_frame_buffers {
     eos::DevBuf levels;     // uint8  [npix]
     eos::DevBuf counters;   // int64  [64]
     eos::DevBuf gstack;     // int32  overflow stacks of persistent lanes
+    eos::DevBuf aa_acc;     // float4 [npix]       anti-aliasing sample sums
     size_t npix = 0, n_ids = 0;
     int depth = -1, n_light = -1;
     int rec_valid = 0;      // records of the last frame kept (adjoint replay)
@@ -155,6 +168,10 @@ struct eos_scene {
     eos::DevBuf codes, codes_sorted, perm, kchild, kparent, first_last, ibox, lbox, flags, cbounds;
     eos::DevBuf sort_tmp_k, sort_tmp_v, sort_hist, bounds_part;
     eos::DevBuf nodes4, flags4, scan_tmp;
+    // per-triangle transformations: xf [n_xf][kXfRows] float4, xf_index [N]
+    // (original order, -1 = none), world-space vertices for the build
+    eos::DevBuf xf, xf_index, wverts;
+    int n_xf = 0;
     int root = 0;
     int root4 = 0;
     int n_nodes4 = 0;
@@ -178,7 +195,16 @@ eos_status render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint
 eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n, const eos_batch_opts* o,
                         uint8_t* const* out, eos_batch_stats* st);
 SceneView view_of(const eos_scene* s);
+// One anti-aliasing sample frame: offset inside the pixel and what the resolve
+// does with the sample's colour (src/RayTracer.cpp:214-242).
+enum { AA_NONE = 0, AA_FIRST = 1, AA_NEXT = 2, AA_LAST = 3 };
+struct AaSample {
+    float ox = 0.0f, oy = 0.0f;   // p * (1/n), q * (1/n)
+    int mode = AA_NONE;
+    float inv_n2 = 1.0f;          // (1/n)^2
+    float4* acc = nullptr;        // [npix] running sums
+};
 // One frame of the wavefront pipeline (render.hip); outputs may be null.
 eos_status run_frame(eos_scene* s, const eos_camera* cam, int depth, bool count, uchar4* rgba_dev, float* rgb_dev,
-                     int32_t* hits_dev, eos_render_stats* st);
+                     int32_t* hits_dev, eos_render_stats* st, const AaSample& aa = AaSample());
 }  // namespace eos

@@ -135,17 +135,26 @@ cudaScene_t* createCudaScene(Scene* h) {
     if (!h->isUsingBvh()) return nullptr;   // src/cudaStructures.cu:500-505
     const int n = h->getNumSurfaces();
     std::vector<float> verts(9 * (size_t)n), normals(9 * (size_t)n);
-    std::vector<int32_t> mat((size_t)n);
+    std::vector<int32_t> mat((size_t)n), xform((size_t)n, -1);
+    bool any_xform = false;
     for (int i = 0; i < n; ++i) {
         Triangle* t = dynamic_cast<Triangle*>(h->getSurface(i));
         if (!t) {   // src/cudaStructures.cu:340-347
             std::fprintf(stderr, "Eos Error: surface %d is not a triangle; only triangles run on the GPU\n", i);
             std::exit(1);
         }
+        // the scene's transformation list, as copyTransformations / copyTriangles
+        // hand it to the device (src/cudaStructures.cu:220-262,359); identities
+        // are dropped so scenes without real transformations keep the plain walk
         const int ti = t->getTransformationIndex();
-        if (ti >= 0 && (ti >= h->getNumTransformations() || !h->getTransformationAtIndex(ti).is_identity())) {
-            std::fprintf(stderr, "Eos Error: triangle %d has a non-identity transformation (not supported)\n", i);
+        if (ti >= h->getNumTransformations()) {
+            std::fprintf(stderr, "Eos Error: triangle %d has transformation index %d of %d\n", i, ti,
+                         h->getNumTransformations());
             std::exit(1);
+        }
+        if (ti >= 0 && !h->getTransformationAtIndex(ti).is_identity()) {
+            xform[i] = ti;
+            any_xform = true;
         }
         const eos::vec3* v[6] = {&t->m_V1, &t->m_V2, &t->m_V3, &t->m_N1, &t->m_N2, &t->m_N3};
         for (int k = 0; k < 3; ++k) {
@@ -172,7 +181,9 @@ cudaScene_t* createCudaScene(Scene* h) {
         o.shininess = (float)m.getShininess();
         o.reflectivity = m.getReflectiveIntensity();
         o.reflective = m.isReflective() ? 1 : 0;
-        o.pad = 0;
+        o.transparent = 0;
+        o.refractive_index = 1.0f;
+        o.pad[0] = o.pad[1] = o.pad[2] = 0;
     }
     std::vector<eos_light> lights((size_t)h->getNumLightSources());
     for (int i = 0; i < h->getNumLightSources(); ++i) {
@@ -192,6 +203,13 @@ cudaScene_t* createCudaScene(Scene* h) {
     d.lights = lights.data();
     cudaScene_t* out = new cudaScene_t();
     check(eos_create_scene(&d, &out->scene), "createCudaScene");
+    if (any_xform) {
+        std::vector<float> m(16 * (size_t)h->getNumTransformations());
+        for (int k = 0; k < h->getNumTransformations(); ++k)
+            std::memcpy(&m[16 * (size_t)k], &h->getTransformationAtIndex(k).m[0][0], 16 * sizeof(float));
+        check(eos_set_transforms(out->scene, h->getNumTransformations(), m.data(), xform.data(), EOS_MEM_HOST),
+              "createCudaScene (transformations)");
+    }
     check(eos_build_lbvh(out->scene, 4, nullptr), "createCudaScene (LBVH build)");
     return out;
 }
@@ -233,6 +251,7 @@ void DeviceRenderer::renderCudaSceneToHostBufferMegaKernel(cudaScene_t* scene, v
     eos_render_opts o{};
     o.depth = m_Depth;
     o.output_memory = EOS_MEM_HOST;
+    o.aa_samples = m_AASamples;
     eos_render_stats st;
     check(eos_render(scene->scene, &cam, &o, static_cast<uint8_t*>(image), &st), "render");
     m_LastMs = (float)st.render_ms;

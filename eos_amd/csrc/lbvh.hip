@@ -313,13 +313,48 @@ __global__ __launch_bounds__(256) void k_nodes(const int2* __restrict__ kchild, 
     nodes[i] = nd;
 }
 
+// World-space vertices of transformed triangles (M * (v, 1), glm's order of
+// operations; the rows of M are kXfM..kXfM+2 of the triangle's record), the
+// input vertices for the rest.  Keys and boxes of the build come from these;
+// the triangle records keep the local vertices, as rays meet triangles in
+// local space.
+__global__ __launch_bounds__(256) void k_world_verts(const float* __restrict__ verts, const int32_t* __restrict__ xi,
+                                                     int n, const float4* __restrict__ xf, int n_xf,
+                                                     float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* t = verts + 9 * (size_t)i;
+    float* w = out + 9 * (size_t)i;
+    const int x = xi[i];
+    if (x < 0 || x >= n_xf) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = t[k];
+        return;
+    }
+    const float4 r[3] = {xf[kXfRows * (size_t)x + kXfM], xf[kXfRows * (size_t)x + kXfM + 1],
+                         xf[kXfRows * (size_t)x + kXfM + 2]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            w[3 * c + k] = (r[k].x * t[3 * c] + r[k].y * t[3 * c + 1]) + (r[k].z * t[3 * c + 2] + r[k].w * 1.0f);
+}
+
+// Triangle records in leaf order: geo = (v1, e1, e2, original index,
+// transformation index), shade = (n1 material, n2 transformation index, n3).
 __global__ __launch_bounds__(256) void k_tris(const float* __restrict__ verts, const float* __restrict__ normals,
                                               const int32_t* __restrict__ mat, const int32_t* __restrict__ perm, int n,
-                                              int n_mat, float4* __restrict__ geo, float4* __restrict__ shade,
+                                              int n_mat, const int32_t* __restrict__ xi, int n_xf,
+                                              float4* __restrict__ geo, float4* __restrict__ shade,
                                               int32_t* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int o = perm[s];
+    int x = xi ? xi[o] : -1;
+    if (x < -1 || x >= n_xf) {
+        atomicAdd(err + 2, 1);
+        x = -1;
+    }
     const float* t = verts + 9 * (size_t)o;
     const float* nn = normals + 9 * (size_t)o;
     const float v1x = t[0], v1y = t[1], v1z = t[2];
@@ -327,14 +362,14 @@ __global__ __launch_bounds__(256) void k_tris(const float* __restrict__ verts, c
     const float e2x = t[6] - v1x, e2y = t[7] - v1y, e2z = t[8] - v1z;
     geo[3 * (size_t)s + 0] = make_float4(v1x, v1y, v1z, e1x);
     geo[3 * (size_t)s + 1] = make_float4(e1y, e1z, e2x, e2y);
-    geo[3 * (size_t)s + 2] = make_float4(e2z, __int_as_float(o), 0.0f, 0.0f);
+    geo[3 * (size_t)s + 2] = make_float4(e2z, __int_as_float(o), __int_as_float(x), 0.0f);
     int m = mat[o];
     if (m < 0 || m >= n_mat) {
         atomicAdd(err, 1);
         m = 0;
     }
     shade[3 * (size_t)s + 0] = make_float4(nn[0], nn[1], nn[2], __int_as_float(m));
-    shade[3 * (size_t)s + 1] = make_float4(nn[3], nn[4], nn[5], 0.0f);
+    shade[3 * (size_t)s + 1] = make_float4(nn[3], nn[4], nn[5], __int_as_float(x));
     shade[3 * (size_t)s + 2] = make_float4(nn[6], nn[7], nn[8], 0.0f);
 }
 
@@ -514,10 +549,18 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st
     EOS_TRY(s->shade.ensure(48 * un));
     EOS_HIP(hipEventRecord(s->ev[0], S));
     int32_t* err = s->cbounds.as<int32_t>() + 8;
-    int32_t* root_pass = err + 1;   // refit pass that boxed the root (0: none)
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S, err, 2, 0);
+    int32_t* root_pass = err + 1;   // refit pass that boxed the root (0: none); err[2]: bad transformation indices
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S, err, 3, 0);
+    const bool xf = s->n_xf > 0;
     if (n > 0) {
-        const float* V = s->verts.as<float>();
+        const float* V = s->verts.as<float>();   // keys and boxes: world space
+        if (xf) {
+            EOS_TRY(s->wverts.ensure(36 * un));
+            hipLaunchKernelGGL(k_world_verts, dim3(blocks(un, 256)), dim3(256), 0, S, V,
+                               s->xf_index.as<const int32_t>(), n, s->xf.as<const float4>(), s->n_xf,
+                               s->wverts.as<float>());
+            V = s->wverts.as<const float>();
+        }
         float* cb = s->cbounds.as<float>();
         hipLaunchKernelGGL(k_cbounds_partial, dim3(nparts), dim3(256), 0, S, V, n, s->bounds_part.as<float>());
         hipLaunchKernelGGL(k_cbounds_final, dim3(1), dim3(64), 0, S, s->bounds_part.as<const float>(), nparts, cb);
@@ -566,9 +609,10 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st
             hipLaunchKernelGGL(k_nodes, dim3(blocks(m, 256)), dim3(256), 0, S, s->kchild.as<const int2>(),
                                s->first_last.as<const int2>(), s->ibox.as<const float>(), s->lbox.as<const float>(),
                                cb, n, leaf_max, s->nodes.as<Node>());
-        hipLaunchKernelGGL(k_tris, dim3(blocks(un, 256)), dim3(256), 0, S, V, s->normals.as<const float>(),
-                           s->matidx.as<const int32_t>(), vA, n, s->n_mat, s->geo.as<float4>(), s->shade.as<float4>(),
-                           err);
+        hipLaunchKernelGGL(k_tris, dim3(blocks(un, 256)), dim3(256), 0, S, s->verts.as<const float>(),
+                           s->normals.as<const float>(), s->matidx.as<const int32_t>(), vA, n, s->n_mat,
+                           xf ? s->xf_index.as<const int32_t>() : nullptr, s->n_xf, s->geo.as<float4>(),
+                           s->shade.as<float4>(), err);
         if (n > leaf_max) {
             // 4-wide collapse: flags, their exclusive scan (node numbers), node count, nodes
             const int nb = (int)blocks(m, kScanBlock);
@@ -595,11 +639,14 @@ eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st
     }
     EOS_HIP(hipGetLastError());
     EOS_HIP(hipEventRecord(s->ev[1], S));
-    int32_t herr[2] = {0, 0};
-    EOS_HIP(hipMemcpyAsync(herr, err, 8, hipMemcpyDeviceToHost, S));
+    int32_t herr[3] = {0, 0, 0};
+    EOS_HIP(hipMemcpyAsync(herr, err, 12, hipMemcpyDeviceToHost, S));
     EOS_HIP(hipStreamSynchronize(S));
     if (herr[0] != 0)
         return fail(EOS_ERR_INVALID, "eos_build_lbvh: " + std::to_string(herr[0]) + " material indices out of range");
+    if (herr[2] != 0)
+        return fail(EOS_ERR_INVALID,
+                    "eos_build_lbvh: " + std::to_string(herr[2]) + " transformation indices out of range");
     if (n > 1 && herr[1] == 0) return fail(EOS_ERR_STATE, "eos_build_lbvh: refit did not reach the root");
     float ms = 0.0f;
     EOS_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));

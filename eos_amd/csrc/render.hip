@@ -29,6 +29,7 @@ namespace {
 struct SetupArgs {
     SceneView sc;
     DevCamera cam;
+    float aa_ox, aa_oy;       // anti-aliasing sample offset inside the pixel
     int tiles_x;
     const BatchScene* table;
     int n_scenes;
@@ -45,9 +46,11 @@ struct SetupArgs {
     int32_t* hit_ids;         // level 0, optional
 };
 
-// src/Triangle.cpp:41-47 and src/BVH.cpp:371-381: point o + t*d; normal
-// normalize(n1*(1-u-v) + n2*u + n3*v), normalised again after the (identity)
-// inverse-transpose transform.
+// src/Triangle.cpp:41-47 and src/BVH.cpp:362-381: point o + t*d and normal
+// normalize(n1*(1-u-v) + n2*u + n3*v) in the triangle's local space (the ray
+// through the inverse), then the point through M and the normal through
+// transpose(inverse), normalised again.  Without a transformation the point is
+// the world one and the second normalisation repeats the first.
 template <bool L0, bool BATCH>
 __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
     const int n = L0 ? a.n_ids : *a.count_ptr;
@@ -63,6 +66,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
         bool pix_ok = valid;
         const float4* shade = a.sc.shade;
         const float4* geo = a.sc.geo;
+        const float4* xf = a.sc.xf;
         const DevMaterial* mats = a.sc.mats;
         if (L0) {
             int x = 0, y = 0;
@@ -73,16 +77,17 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
                     pix_ok = tile_pixel(i - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
                     pixel = B.pix_off + y * B.cam.width + x;
                     o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
-                    if (pix_ok) d = camera_dir(B.cam, x, y);
+                    if (pix_ok) d = camera_dir(B.cam, x, y, 0.0f, 0.0f);
                     shade = B.sc.shade;
                     geo = B.sc.geo;
+                    xf = B.sc.xf;
                     mats = B.sc.mats;
                 }
             } else {
                 pix_ok = valid && tile_pixel(i, a.tiles_x, a.cam.width, a.cam.height, x, y);
                 pixel = y * a.cam.width + x;
                 o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-                if (pix_ok) d = camera_dir(a.cam, x, y);
+                if (pix_ok) d = camera_dir(a.cam, x, y, a.aa_ox, a.aa_oy);
             }
         } else if (valid) {
             const float4 q0 = a.q_in[2 * (size_t)i], q1 = a.q_in[2 * (size_t)i + 1];
@@ -94,6 +99,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
                 const BatchScene& B = a.table[scene];
                 shade = B.sc.shade;
                 geo = B.sc.geo;
+                xf = B.sc.xf;
                 mats = B.sc.mats;
             }
         }
@@ -103,13 +109,22 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
         bool refl = false;
         if (hit) {
             const float4 s0 = shade[3 * (size_t)slot], s1 = shade[3 * (size_t)slot + 1], s2 = shade[3 * (size_t)slot + 2];
-            P = add(o, scale(h.x, d));
+            const int xi = __float_as_int(s1.w);
+            const float4* X = xi >= 0 ? xf + kXfRows * (size_t)xi : nullptr;
+            const F3 lo = X ? xform3(X + kXfInv, o, 1.0f) : o;
+            const F3 ld = X ? xform3(X + kXfInv, d, 0.0f) : d;
+            P = add(lo, scale(h.x, ld));
             const float u = h.z, v = h.w;
             const float w = (1.0f - u) - v;
             const F3 na = {s0.x * w, s0.y * w, s0.z * w};
             const F3 nb = {s1.x * u, s1.y * u, s1.z * u};
             const F3 nc = {s2.x * v, s2.y * v, s2.z * v};
-            N = normalize3(normalize3(add(add(na, nb), nc)));
+            N = normalize3(add(add(na, nb), nc));
+            if (X) {
+                P = xform3(X + kXfM, P, 1.0f);
+                N = xform3(X + kXfInvT, N, 0.0f);
+            }
+            N = normalize3(N);
             mat = __float_as_int(s0.w);
             refl = mats[mat].reflective != 0 && a.level < a.depth;
         }
@@ -206,8 +221,15 @@ struct ResolveArgs {
     const float4* levcol;
     uchar4* rgba;
     float* rgb;
+    float4* acc;              // anti-aliasing: running sum of the pixel's samples
+    int aa_mode;              // AA_NONE, AA_FIRST, AA_NEXT, AA_LAST
+    float inv_n2;             // (1/n)^2, the final scale (AA_LAST)
 };
 
+// src/RayTracer.cpp:214-242: a pixel's n x n sample colours are added in
+// (q, p) order to a zero vec4, and the sum is scaled by (1/n)^2 before it is
+// quantised.  One sample per frame: the first stores 0 + c, the next ones add
+// c, the last adds c, scales and writes the pixel.
 __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.npix; p += gridDim.x * blockDim.x) {
         const int nl = a.levels[p];
@@ -217,6 +239,19 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
             c0 = col.x + col.w * c0;
             c1 = col.y + col.w * c1;
             c2 = col.z + col.w * c2;
+        }
+        if (a.aa_mode != AA_NONE) {
+            const float4 s = a.aa_mode == AA_FIRST ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : a.acc[p];
+            c0 = s.x + c0;
+            c1 = s.y + c1;
+            c2 = s.z + c2;
+            if (a.aa_mode != AA_LAST) {
+                a.acc[p] = make_float4(c0, c1, c2, 0.0f);
+                continue;
+            }
+            c0 = c0 * a.inv_n2;
+            c1 = c1 * a.inv_n2;
+            c2 = c2 * a.inv_n2;
         }
         if (a.rgba) a.rgba[p] = make_uchar4(quantize(c0), quantize(c1), quantize(c2), 255);
         if (a.rgb) {
@@ -234,19 +269,26 @@ __global__ void k_zero_i32(int* p, int n) {
 // ---- traversal launch table ------------------------------------------------
 // 4-wide trees: every workgroup / LDS-stack configuration; binary trees: the
 // default configuration only (they exist for comparison and parity checks).
+// Scenes with transformations: the default configuration (256 threads, 16 LDS
+// stack entries), no packets; the frame sets its configuration to match.
 template <int MODE, bool COUNT, bool BATCH>
-eos_status launch_trace(const LaunchCfg& cfg, int width, int grid, hipStream_t s, const TraceArgs& a) {
-#define EOS_TRACE_CASE(WG, SL)                                                                           \
-    case WG * 100 + SL:                                                                                   \
-        hipLaunchKernelGGL((k_trace<MODE, WG, SL, COUNT, BATCH, 4>), dim3(grid), dim3(WG), 0, s, a);      \
+eos_status launch_trace(const LaunchCfg& cfg, int width, bool xf, int grid, hipStream_t s, const TraceArgs& a) {
+#define EOS_TRACE_CASE(WG, SL)                                                                             \
+    case WG * 100 + SL:                                                                                     \
+        hipLaunchKernelGGL((k_trace<MODE, WG, SL, COUNT, BATCH, 4, false>), dim3(grid), dim3(WG), 0, s, a); \
         break;
-    if (MODE == MODE_PRIMARY && width == 4 && cfg.packets) {
+    if (xf) {
+        if (cfg.workgroup != 256 || cfg.stack_lds != 16 || cfg.packets)
+            return fail(EOS_ERR_INVALID, "transformed scenes trace with the default launch configuration");
+        if (width == 2) hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 2, true>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 4, true>), dim3(grid), dim3(256), 0, s, a);
+    } else if (MODE == MODE_PRIMARY && width == 4 && cfg.packets) {
         hipLaunchKernelGGL((k_trace_packet<256, COUNT, BATCH>), dim3(std::max(1, grid * cfg.workgroup / 256)),
                            dim3(256), 0, s, a);
     } else if (width == 2) {
-        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 2>), dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 2, false>), dim3(grid), dim3(256), 0, s, a);
     } else if (COUNT) {
-        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 4>), dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 4, false>), dim3(grid), dim3(256), 0, s, a);
     } else {
         switch (cfg.workgroup * 100 + cfg.stack_lds) {
             EOS_TRACE_CASE(64, 8)
@@ -267,11 +309,11 @@ eos_status launch_trace(const LaunchCfg& cfg, int width, int grid, hipStream_t s
 }
 
 template <int MODE>
-eos_status launch_trace_any(bool count, bool batch, int width, const LaunchCfg& cfg, int grid, hipStream_t s,
+eos_status launch_trace_any(bool count, bool batch, int width, bool xf, const LaunchCfg& cfg, int grid, hipStream_t s,
                             const TraceArgs& a) {
-    if (batch) return launch_trace<MODE, false, true>(cfg, width, grid, s, a);
-    return count ? launch_trace<MODE, true, false>(cfg, width, grid, s, a)
-                 : launch_trace<MODE, false, false>(cfg, width, grid, s, a);
+    if (batch) return launch_trace<MODE, false, true>(cfg, width, xf, grid, s, a);
+    return count ? launch_trace<MODE, true, false>(cfg, width, xf, grid, s, a)
+                 : launch_trace<MODE, false, false>(cfg, width, xf, grid, s, a);
 }
 
 inline unsigned blocks_for(size_t n, unsigned t, unsigned cap) {
@@ -292,6 +334,8 @@ struct Frame {
     int depth = 0;
     int lstride = 0;       // light slots per hit (max lights over the batch)
     int width = 4;         // tree width walked by the traversal
+    bool xf = false;       // some triangle has a transformation
+    AaSample aa;           // anti-aliasing sample of this frame (mode AA_NONE: plain frame)
     bool count = false;
     LaunchCfg cfg;
     int num_cu = 256;
@@ -394,7 +438,7 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             sh.vis_out = fb.vis_at(l0);
             sh.surf_stride = (long long)fb.surf_stride();
             sh.vis_stride = (long long)fb.vis_stride();
-            EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.width, f.cfg, trace_grid, st, sh));
+            EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, st, sh));
         }
         F.s[2 * slot + 1] = rec(st);
         for (int l = l0; l <= l1; ++l) {
@@ -422,6 +466,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         TraceArgs ta{};
         ta.sc = f.sc;
         ta.cam = f.cam;
+        ta.aa_ox = f.aa.ox;
+        ta.aa_oy = f.aa.oy;
         ta.tiles_x = f.tiles_x;
         ta.table = f.table;
         ta.n_scenes = f.n_scenes;
@@ -444,12 +490,14 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             ta.hit_out = fb.hitq.as<float4>();
         }
         F.c[2 * l] = rec(S);
-        if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(f.count, f.batch, f.width, f.cfg, trace_grid, S, ta));
-        else EOS_TRY(launch_trace_any<MODE_QUEUE>(f.count, f.batch, f.width, f.cfg, trace_grid, S, ta));
+        if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, S, ta));
+        else EOS_TRY(launch_trace_any<MODE_QUEUE>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, S, ta));
         F.c[2 * l + 1] = rec(S);
         SetupArgs sa{};
         sa.sc = f.sc;
         sa.cam = f.cam;
+        sa.aa_ox = f.aa.ox;
+        sa.aa_oy = f.aa.oy;
         sa.tiles_x = f.tiles_x;
         sa.table = f.table;
         sa.n_scenes = f.n_scenes;
@@ -492,7 +540,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         EOS_HIP(hipEventRecord(fs.sync[1], S2));
         EOS_HIP(hipStreamWaitEvent(S, fs.sync[1], 0));
     }
-    ResolveArgs ra{(int)f.npix, fb.levels.as<uint8_t>(), fb.levcol.as<float4>(), f.rgba, f.rgb};
+    ResolveArgs ra{(int)f.npix, fb.levels.as<uint8_t>(), fb.levcol.as<float4>(), f.rgba, f.rgb,
+                   f.aa.acc, f.aa.mode, f.aa.inv_n2};
     F.resolve0 = rec(S);
     hipLaunchKernelGGL(k_resolve, dim3(blocks_for(f.npix, 256, shade_grid)), dim3(256), 0, S, ra);
     F.end = rec(S);
@@ -559,8 +608,9 @@ eos_status finish_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hi
 
 // Frame driver shared by eos_render and the adjoint's re-trace.
 eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev,
-                     float* rgb_dev, int32_t* hits_dev, eos_render_stats* st) {
+                     float* rgb_dev, int32_t* hits_dev, eos_render_stats* st, const AaSample& aa) {
     Frame f;
+    f.aa = aa;
     f.cam = *reinterpret_cast<const DevCamera*>(camp);
     f.sc = view_of(s);
     f.tiles_x = (f.cam.width + 7) / 8;
@@ -572,10 +622,12 @@ eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count
     f.count = count;
     f.cfg = s->cfg;
     f.width = s->width;
-    if (count || f.width == 2) {
+    f.xf = s->n_xf > 0;
+    if (count || f.width == 2 || f.xf) {
         f.cfg.workgroup = 256;
         f.cfg.stack_lds = 16;
     }
+    if (f.xf) f.cfg.packets = 0;
     f.num_cu = s->num_cu;
     f.rgba = rgba_dev;
     f.rgb = rgb_dev;
@@ -619,7 +671,43 @@ eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o,
         }
     }
     fb.rec_valid = 0;
-    EOS_TRY(run_frame(s, cam, o->depth, o->count_visits != 0, rgba_dev, rgb_dev, hit_dev, st));
+    const int n = o->aa_samples > 1 ? o->aa_samples : 1;
+    if (n == 1) {
+        EOS_TRY(run_frame(s, cam, o->depth, o->count_visits != 0, rgba_dev, rgb_dev, hit_dev, st));
+    } else {
+        // one frame per sample, in the reference's (q, p) order, summed by the resolve
+        EOS_TRY(fb.aa_acc.ensure(16 * npix));
+        AaSample aa;
+        aa.acc = fb.aa_acc.as<float4>();
+        const float inv_n = (float)(1.0 / (double)(float)n);   // `float inv = 1.0 / (float) n`
+        aa.inv_n2 = inv_n * inv_n;
+        eos_render_stats sum{}, one{};
+        for (int q = 0; q < n; ++q)
+            for (int p = 0; p < n; ++p) {
+                const int k = q * n + p;
+                aa.ox = ((float)p + 0.0f) * inv_n;
+                aa.oy = ((float)q + 0.0f) * inv_n;
+                aa.mode = k == 0 ? AA_FIRST : (k == n * n - 1 ? AA_LAST : AA_NEXT);
+                EOS_TRY(run_frame(s, cam, o->depth, o->count_visits != 0, aa.mode == AA_LAST ? rgba_dev : nullptr,
+                                  aa.mode == AA_LAST ? rgb_dev : nullptr, k == 0 ? hit_dev : nullptr, &one, aa));
+                sum.primary_rays += one.primary_rays;
+                sum.shadow_rays += one.shadow_rays;
+                sum.secondary_rays += one.secondary_rays;
+                sum.hits += one.hits;
+                sum.node_visits += one.node_visits;
+                sum.tri_tests += one.tri_tests;
+                sum.render_ms += one.render_ms;
+                sum.trace_closest_ms += one.trace_closest_ms;
+                sum.trace_shadow_ms += one.trace_shadow_ms;
+                sum.shade_ms += one.shade_ms;
+                sum.trace_closest_launches += one.trace_closest_launches;
+                sum.trace_shadow_launches += one.trace_shadow_launches;
+                sum.closest_visits_first += one.closest_visits_first;
+                sum.closest_tests_first += one.closest_tests_first;
+                sum.trace_first_ms += one.trace_first_ms;
+            }
+        if (st) *st = sum;
+    }
     if (!dev_out) {
         if (rgba) EOS_HIP(hipMemcpyAsync(rgba, rgba_dev, 4 * npix, hipMemcpyDeviceToHost, s->stream));
         if (o->rgb_out) EOS_HIP(hipMemcpyAsync(o->rgb_out, rgb_dev, 12 * npix, hipMemcpyDeviceToHost, s->stream));
@@ -721,10 +809,12 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
         f.lstride = lmax;
         f.cfg = cfg;
         f.width = s0->width;
-        if (f.width == 2) {
+        for (int i = 0; i < n; ++i) f.xf |= scenes[i]->n_xf > 0;
+        if (f.width == 2 || f.xf) {
             f.cfg.workgroup = 256;
             f.cfg.stack_lds = 16;
         }
+        if (f.xf) f.cfg.packets = 0;
         f.num_cu = s0->num_cu;
         f.rgba = b.rgba.as<uchar4>();
         f.sc = view_of(s0);

@@ -39,12 +39,15 @@ __device__ __forceinline__ F3 reflect3(F3 I, F3 N) {
 }
 
 // src/RayTracer.cpp:119-142: aa = (x - W/2) * (1/(W/2)), bb likewise,
-// dir = normalize(aa*right + bb*up + view).  Row 0 is the bottom row.
-__device__ __forceinline__ F3 camera_dir(const DevCamera& c, int x, int y) {
+// dir = normalize(aa*right + bb*up + view).  Row 0 is the bottom row.  With
+// anti-aliasing (src/RayTracer.cpp:222-230) the sample's offset inside the
+// pixel, ox = p * (1/n) and oy = q * (1/n), is added to x and y first; a zero
+// offset leaves x and y exact, so the plain frame is the ox = oy = 0 case.
+__device__ __forceinline__ F3 camera_dir(const DevCamera& c, int x, int y, float ox, float oy) {
     const float nw = (float)c.width / 2.0f, nh = (float)c.height / 2.0f;
     const float inw = 1.0f / nw, inh = 1.0f / nh;
-    const float bb = ((float)y - nh) * inh;
-    const float aa = ((float)x - nw) * inw;
+    const float bb = (((float)y + oy) - nh) * inh;
+    const float aa = (((float)x + ox) - nw) * inw;
     const F3 r = {c.right[0], c.right[1], c.right[2]};
     const F3 u = {c.up[0], c.up[1], c.up[2]};
     const F3 v = {c.view[0], c.view[1], c.view[2]};
@@ -79,6 +82,26 @@ __device__ __forceinline__ bool moller_trumbore(F3 o, F3 d, F3 v1, F3 e1, F3 e2,
     const bool rej = (det > -0.00001f && det < 0.00001f) | (t < 0.0f) | (u < 0.0f || u > 1.0f) |
                      (v < 0.0f || u + v > 1.0f);
     return !rej;
+}
+
+// Row r of a column-major matrix applied to (p, w): (row.x*x + row.y*y) +
+// (row.z*z + row.w*w), glm's mat4 * vec4 (glm/detail/type_mat4x4.inl:571-616).
+__device__ __forceinline__ float xrow(float4 r, F3 p, float w) {
+    return (r.x * p.x + r.y * p.y) + (r.z * p.z + r.w * w);
+}
+__device__ __forceinline__ F3 xform3(const float4* X, F3 p, float w) {
+    return f3(xrow(X[0], p, w), xrow(X[1], p, w), xrow(X[2], p, w));
+}
+
+// A world-space ray in the local space of transformation xi (-1: none): origin
+// with w = 1 and direction with w = 0 through the inverse (src/BVH.cpp:362-364).
+template <bool XF>
+__device__ __forceinline__ void to_local(const float4* xf, int xi, F3& o, F3& d) {
+    if (XF && xi >= 0) {
+        const float4* X = xf + kXfRows * (size_t)xi + kXfInv;
+        o = xform3(X, o, 1.0f);
+        d = xform3(X, d, 0.0f);
+    }
 }
 
 __device__ __forceinline__ float safe_inv(float d) {
@@ -178,6 +201,7 @@ __device__ __forceinline__ int scene_of_id(const BatchScene* t, int n, int r) {
 struct TraceArgs {
     SceneView sc;
     DevCamera cam;
+    float aa_ox, aa_oy;      // anti-aliasing sample offset inside the pixel (0: pixel corner)
     int tiles_x;
     const BatchScene* table;  // batched frames
     int n_scenes;
@@ -292,6 +316,7 @@ struct RayItem {
     const Node* nodes;
     const Node4* nodes4;
     const float4* geo;
+    const float4* xf;        // transformations of the scene (null: none)
     size_t out_idx;          // MODE_SHADOW: visibility byte of the item
 };
 
@@ -301,6 +326,7 @@ __device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& 
     r.nodes = a.sc.nodes;
     r.nodes4 = a.sc.nodes4;
     r.geo = a.sc.geo;
+    r.xf = a.sc.xf;
     r.root = WIDTH == 4 ? a.sc.root4 : a.sc.root;
     r.max_visits = a.sc.n_tri + 64;
     r.out_idx = 0;
@@ -308,6 +334,7 @@ __device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& 
         r.nodes = B.sc.nodes;
         r.nodes4 = B.sc.nodes4;
         r.geo = B.sc.geo;
+        r.xf = B.sc.xf;
         r.root = WIDTH == 4 ? B.sc.root4 : B.sc.root;
         r.max_visits = B.sc.n_tri + 64;
     };
@@ -318,11 +345,11 @@ __device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& 
             use_scene(B);
             r.valid = tile_pixel(my - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
             r.o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
-            r.d = camera_dir(B.cam, x, y);
+            r.d = camera_dir(B.cam, x, y, 0.0f, 0.0f);
         } else {
             r.valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
             r.o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-            r.d = camera_dir(a.cam, x, y);
+            r.d = camera_dir(a.cam, x, y, a.aa_ox, a.aa_oy);
         }
     } else if (MODE == MODE_QUEUE) {
         const float4 q0 = a.q[2 * (size_t)my], q1 = a.q[2 * (size_t)my + 1];
@@ -385,8 +412,8 @@ __device__ __forceinline__ bool use_coop(int mode, long long n, long long lanes)
 // visits the same nodes in the same order as the per-lane walk, ends with the
 // same hit and is counted the same way; what changes is the length of the chain
 // of dependent instructions per step, which is what bounds a launch that is one
-// round of rays.
-template <int MODE, int WG, bool COUNT, bool BATCH>
+// round of rays.  XF: triangles may carry transformations (tested in local space).
+template <int MODE, int WG, bool COUNT, bool BATCH, bool XF>
 __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* wave_stack, unsigned long long& visits,
                                            unsigned long long& tests) {
     constexpr bool ANY = MODE == MODE_SHADOW;
@@ -407,6 +434,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
     int cur = kDone, sp = 0, budget = 0;
     const Node4* nodes4 = a.sc.nodes4;
     const float4* geo = a.sc.geo;
+    const float4* xf = a.sc.xf;
     for (;;) {
         // one bit per idle quad, at its lane 0
         const uint64_t idle = __ballot(sub == 0 && item < 0);
@@ -423,6 +451,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
                 d = r.d;
                 nodes4 = r.nodes4;
                 geo = r.geo;
+                xf = r.xf;
                 ix = safe_inv(d.x);
                 iy = safe_inv(d.y);
                 iz = safe_inv(d.z);
@@ -488,7 +517,9 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
                         const int sl = first + (act ? k : 0);
                         const float4 g0 = geo[3 * (size_t)sl], g1 = geo[3 * (size_t)sl + 1], g2 = geo[3 * (size_t)sl + 2];
                         float tt, uu, vv;
-                        const bool hit = moller_trumbore(o, d, f3(g0.x, g0.y, g0.z), f3(g0.w, g1.x, g1.y),
+                        F3 lo = o, ld = d;
+                        to_local<XF>(xf, __float_as_int(g2.z), lo, ld);
+                        const bool hit = moller_trumbore(lo, ld, f3(g0.x, g0.y, g0.z), f3(g0.w, g1.x, g1.y),
                                                          f3(g1.z, g1.w, g2.x), tt, uu, vv) & act;
                         if (ANY) {
                             if (hit && k < lfirst) lfirst = k;
@@ -562,8 +593,10 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
 // the best hit so far.  Per-ray visit order is deterministic and equal to the
 // CPU restatement in the oracle.  A launch with few items for its lanes (4-wide
 // tree, a.coop) takes the quad-cooperative walk instead.  BATCH: every ray looks
-// up its own scene (BVH, triangles, lights, camera) in the batch table.
-template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH, int WIDTH>
+// up its own scene (BVH, triangles, lights, camera) in the batch table.  XF:
+// triangles may carry transformations; a ray meets such a triangle in the
+// triangle's local space.
+template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH, int WIDTH, bool XF>
 __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     constexpr bool ANY = MODE == MODE_SHADOW;
     constexpr int MAXS = max_stack<WIDTH>();
@@ -579,12 +612,13 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     }
     unsigned long long visits = 0, tests = 0;
     if (WIDTH == 4 && use_coop(a.coop, n, (long long)gridDim.x * WG)) {
-        trace_coop<MODE, WG, COUNT, BATCH>(a, n, lds + (tid >> 6) * 16 * kMaxStack4, visits, tests);
+        trace_coop<MODE, WG, COUNT, BATCH, XF>(a, n, lds + (tid >> 6) * 16 * kMaxStack4, visits, tests);
     } else {
         int32_t* gst = a.gstack + (size_t)(blockIdx.x * WG + tid) * (MAXS - STACK_LDS);
         const Node* __restrict__ nodes = a.sc.nodes;
         const Node4* __restrict__ nodes4 = a.sc.nodes4;
         const float4* __restrict__ geo = a.sc.geo;
+        const float4* __restrict__ xf = a.sc.xf;
         int item = -1;
         size_t out_idx = 0;
         Fetch f = fetch_start();
@@ -614,6 +648,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     nodes = r.nodes;
                     nodes4 = r.nodes4;
                     geo = r.geo;
+                    xf = r.xf;
                     ix = safe_inv(d.x);
                     iy = safe_inv(d.y);
                     iz = safe_inv(d.z);
@@ -734,9 +769,12 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                             const float4 a0 = geo[3 * (size_t)s0], a1 = geo[3 * (size_t)s0 + 1], a2 = geo[3 * (size_t)s0 + 2];
                             const float4 b0 = geo[3 * (size_t)s1], b1 = geo[3 * (size_t)s1 + 1], b2 = geo[3 * (size_t)s1 + 2];
                             float ta, ua, va, tb, ub, vb;
-                            const bool ha = moller_trumbore(o, d, f3(a0.x, a0.y, a0.z), f3(a0.w, a1.x, a1.y),
+                            F3 oa = o, da = d, ob = o, db = d;
+                            to_local<XF>(xf, __float_as_int(a2.z), oa, da);
+                            to_local<XF>(xf, __float_as_int(b2.z), ob, db);
+                            const bool ha = moller_trumbore(oa, da, f3(a0.x, a0.y, a0.z), f3(a0.w, a1.x, a1.y),
                                                             f3(a1.z, a1.w, a2.x), ta, ua, va);
-                            const bool hb = moller_trumbore(o, d, f3(b0.x, b0.y, b0.z), f3(b0.w, b1.x, b1.y),
+                            const bool hb = moller_trumbore(ob, db, f3(b0.x, b0.y, b0.z), f3(b0.w, b1.x, b1.y),
                                                             f3(b1.z, b1.w, b2.x), tb, ub, vb);
                             // counted as the one-at-a-time walk would: an any-hit ray stops at its first hit
                             if (COUNT) tests += (ANY && ha) ? 1 : (s1 != s0 ? 2 : 1);
@@ -846,11 +884,11 @@ __global__ __launch_bounds__(WG) void k_trace_packet(TraceArgs a) {
             max_visits = B.sc.n_tri + 64;
             valid = tile_pixel(my - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
             o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
-            d = camera_dir(B.cam, x, y);
+            d = camera_dir(B.cam, x, y, 0.0f, 0.0f);
         } else {
             valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
             o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-            d = camera_dir(a.cam, x, y);
+            d = camera_dir(a.cam, x, y, a.aa_ox, a.aa_oy);
         }
         const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
         const float px = o.x * ix, py = o.y * iy, pz = o.z * iz;

@@ -4,13 +4,14 @@
 // MultiDeviceRenderer and render once to a host buffer; then the single-GPU
 // route (createCudaScene + DeviceCameraHandler + DeviceRenderer).
 //
-// usage: render_scene <scene.bin> <out_multi.rgba> <out_single.rgba> [mesh.obj mesh_material]
+// usage: render_scene <scene.bin> <out_multi.rgba> <out_single.rgba> [mesh.obj mesh_material [tx ty tz]]
 // With a mesh, a TriangleMesh read from the OBJ file is added after the
 // scene's own triangles, the way the reference's driver adds its OBJ model
-// (src/main.cpp:386-403): material and transformation index set before
-// loadFromFile, then Scene::addTriangleMesh.
+// (src/main.cpp:386-403): a translation added with Scene::addTransformation
+// (identity when tx ty tz are not given) and set on the mesh, material and
+// transformation index set before loadFromFile, then Scene::addTriangleMesh.
 // scene.bin: int32 n_tri, n_mat, n_light, width, height, depth; float32
-// verts[n][9], normals[n][9]; int32 mat[n]; 48-byte materials; 32-byte lights;
+// verts[n][9], normals[n][9]; int32 mat[n]; 64-byte materials; 32-byte lights;
 // float32 camera position, view, right, up (12 floats).
 #include <cstdio>
 #include <cstdlib>
@@ -27,8 +28,9 @@ static void rd(FILE* f, T* p, size_t n) {
 }
 
 int main(int argc, char** argv) {
-    if (argc != 4 && argc != 6) {
-        std::fprintf(stderr, "usage: %s scene.bin out_multi.rgba out_single.rgba [mesh.obj mesh_material]\n", argv[0]);
+    if (argc != 4 && argc != 6 && argc != 9) {
+        std::fprintf(stderr, "usage: %s scene.bin out_multi.rgba out_single.rgba [mesh.obj mesh_material [tx ty tz]]\n",
+                     argv[0]);
         return 2;
     }
     FILE* f = std::fopen(argv[1], "rb");
@@ -74,8 +76,14 @@ int main(int argc, char** argv) {
         tris.push_back(t);
     }
     TriangleMesh mesh;
-    if (argc == 6) {
-        mesh.setTransformationIndex(ident);
+    if (argc >= 6) {
+        // glm::translate(glm::mat4(1.0f), t): the identity with t in column 3
+        eos::mat4 meshTransformation(1.0f);
+        if (argc == 9)
+            for (int k = 0; k < 3; ++k) meshTransformation.m[3][k] = (float)std::atof(argv[6 + k]);
+        const int meshTransformationIndex = scene.addTransformation(meshTransformation);
+        mesh.setTransformation(meshTransformation);
+        mesh.setTransformationIndex(meshTransformationIndex);
         mesh.setMaterialIndex(matIndex[std::atoi(argv[5])]);
         if (!mesh.loadFromFile(argv[4])) {
             std::fprintf(stderr, "cannot read %s: %s\n", argv[4], eos_last_error());

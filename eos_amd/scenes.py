@@ -19,8 +19,11 @@ A scene is plain arrays in the layout the C ABI (``include/eos_c.h``) takes:
   (reference ``Camera``, src/Camera.h:55-63; no field of view: the lengths of
   ``right`` and ``up`` set it, as in src/RayTracer.cpp:135-141)
 
-All transforms are the identity (the reference's per-triangle transform is a
-mat4 that the workloads never set to anything else).
+* ``xforms``  float32 [T, 16]   local-to-world matrices, column-major (glm
+  layout); ``xform_index`` int32 [N], -1 = world space (reference
+  ``Scene::addTransformation`` + ``Triangle::setTransformationIndex``,
+  src/Scene.cpp:96-105, src/Triangle.h:59).  None in configs A-E; config F
+  instances one mesh many times through them.
 """
 from __future__ import annotations
 
@@ -31,12 +34,13 @@ import numpy as np
 
 MATERIAL_DTYPE = np.dtype([
     ("kd", "<f4", (4,)), ("ks", "<f4", (4,)), ("shininess", "<f4"),
-    ("reflectivity", "<f4"), ("reflective", "<i4"), ("pad", "<i4")])
+    ("reflectivity", "<f4"), ("reflective", "<i4"), ("transparent", "<i4"), ("refractive_index", "<f4"),
+    ("pad", "<i4", (3,))])
 LIGHT_DTYPE = np.dtype([("position", "<f4", (4,)), ("color", "<f4", (4,))])
 CAMERA_DTYPE = np.dtype([
     ("position", "<f4", (3,)), ("view", "<f4", (3,)), ("right", "<f4", (3,)),
     ("up", "<f4", (3,)), ("width", "<i4"), ("height", "<i4")])
-assert MATERIAL_DTYPE.itemsize == 48 and LIGHT_DTYPE.itemsize == 32
+assert MATERIAL_DTYPE.itemsize == 64 and LIGHT_DTYPE.itemsize == 32
 assert CAMERA_DTYPE.itemsize == 56
 
 BASE_SEED = 0x456F73          # "Eos"
@@ -108,6 +112,8 @@ class SceneData:
     # (glm layout) and [N] int32 indices, -1 = none; None: no transformations
     xforms: np.ndarray = None
     xform_index: np.ndarray = None
+    # Scene::setAmbientRefractiveIndex: the medium outside transparent materials
+    ambient_index: float = 1.0
 
     @property
     def n_tri(self) -> int:
@@ -131,7 +137,8 @@ class SceneData:
         cam["width"] = width
         cam["height"] = height
         return SceneData(self.verts, self.normals, self.mat, self.materials, self.lights,
-                         cam, self.depth, self.name, dict(self.info), self.xforms, self.xform_index)
+                         cam, self.depth, self.name, dict(self.info), self.xforms, self.xform_index,
+                         self.ambient_index)
 
 
 # ----------------------------------------------------------------- builders
@@ -158,13 +165,16 @@ def make_camera(eye, target, vfov_deg: float, width: int, height: int,
     return cam
 
 
-def make_material(kd, ks, shininess, reflectivity=0.0, reflective=False) -> np.ndarray:
+def make_material(kd, ks, shininess, reflectivity=0.0, reflective=False, transparent=False,
+                  refractive_index=1.0) -> np.ndarray:
     m = np.zeros((), MATERIAL_DTYPE)
     m["kd"] = list(kd) + [1.0] if len(kd) == 3 else kd
     m["ks"] = list(ks) + [1.0] if len(ks) == 3 else ks
     m["shininess"] = shininess
     m["reflectivity"] = reflectivity
     m["reflective"] = 1 if reflective else 0
+    m["transparent"] = 1 if transparent else 0
+    m["refractive_index"] = refractive_index
     return m
 
 
@@ -469,6 +479,102 @@ def config_e(width: int = 1024, height: int = 1024, nu: int = 1000, nv: int = 50
     lights["color"][:, :3] = 0.25 + 0.25 * Rng(seed ^ 0x21).uniform(3 * n_lights).reshape(-1, 3)
     cam = make_camera((0.0, 0.3, 3.2), (0.0, 0.0, 0.0), 60.0, width, height)
     return SceneData(verts, normals, mat, mats, lights, cam, 2, "E: adjoint sphere 1M")
+
+
+def affine(axis, angle: float, scale, translate) -> np.ndarray:
+    """Column-major float32 [16] matrix T * R(axis, angle) * S(scale) (glm layout)."""
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0.0, -a[2], a[1]], [a[2], 0.0, -a[0]], [-a[1], a[0], 0.0]])
+    R = np.eye(3) + math.sin(angle) * K + (1.0 - math.cos(angle)) * (K @ K)
+    M = np.eye(4)
+    M[:3, :3] = R @ np.diag(np.asarray(scale, np.float64))
+    M[:3, 3] = translate
+    return M.T.ravel().astype(np.float32)          # column c = M[:, c]
+
+
+def config_f(width: int = 3840, height: int = 2160, instances: int = 64, nu: int = 256, nv: int = 128,
+             rotate: bool = True) -> SceneData:
+    """Instanced meshes: one displaced UV sphere (2*nu*nv triangles, local space)
+    placed `instances` times on a grid through per-triangle transformations
+    (random rotation, non-uniform scale in [0.42, 1.3], one mirrored instance),
+    over a world-space ground plane; 9 materials, 2 lights, depth 2.  Defaults:
+    64 x 65,536 = 4,194,304 instanced triangles + 2,048 plane triangles.
+
+    rotate=False keeps every matrix axis-aligned (quarter turns about a
+    coordinate axis, exact 0/+-1 entries): the only kind the reference's BVH
+    bounds correctly, since its Box::transformBoundingBox maps just the min and
+    max corners of a local box (src/Box.cu:84-137)."""
+    seed = BASE_SEED + 5
+    sv, sn, _ = displaced_sphere(nu, nv, seed)
+    rng = Rng(seed ^ 0x50)
+    cols = int(math.ceil(math.sqrt(instances)))
+    rows = int(math.ceil(instances / cols))
+    spacing = 2.6
+    xforms = []
+    for k in range(instances):
+        u = rng.uniform(8)
+        axis = rng.normal(3)
+        s = (0.6 + 0.4 * u[0]) * (0.7 + 0.6 * u[1:4])
+        if k == 1:
+            s[0] = -s[0]                              # mirrored: negative determinant
+        x = (k % cols - 0.5 * (cols - 1)) * spacing
+        z = -(k // cols) * spacing
+        if rotate:
+            xforms.append(affine(axis, 2.0 * math.pi * u[4], s, (x, 0.0, z)))
+        else:
+            ax = np.zeros(3)
+            ax[min(int(u[5] * 3), 2)] = 1.0
+            m = affine(ax, 0.5 * math.pi * min(int(u[4] * 4), 3), np.ones(3), (0.0, 0.0, 0.0)).reshape(4, 4)
+            m = np.rint(m).astype(np.float64)         # exact quarter turn
+            m[:3, :3] *= s[:, None]                   # column-major: row c scales column c, i.e. R * S
+            m[3, :3] = (x, 0.0, z)
+            xforms.append(m.ravel().astype(np.float32))
+    xforms = np.stack(xforms)
+    extent = max(cols, rows) * spacing
+    pv, pn = quad_grid(32, 0.75 * extent + 2.0, -1.4)
+    pv[..., 2] -= 0.5 * (rows - 1) * spacing
+    n_inst = len(sv)
+    verts = np.concatenate([np.tile(sv, (instances, 1, 1)), pv])
+    normals = np.concatenate([np.tile(sn, (instances, 1, 1)), pn])
+    xform_index = np.concatenate([np.repeat(np.arange(instances, dtype=np.int32), n_inst),
+                                  np.full(len(pv), -1, np.int32)])
+    mat = np.concatenate([np.repeat(np.arange(instances, dtype=np.int32) % 8, n_inst),
+                          np.full(len(pv), 8, np.int32)])
+    mats = np.concatenate([_random_materials(Rng(seed ^ 0x10), 8, True, 0.1, 0.4),
+                           make_material((0.6, 0.6, 0.6), (0.1, 0.1, 0.1), 8, 0.0, False)[None]])
+    zc = -0.5 * (rows - 1) * spacing
+    lights = make_lights([(0.3 * extent, 0.8 * extent + 2.0, zc + 0.6 * extent),
+                          (-0.4 * extent, 0.6 * extent + 2.0, zc + 0.2 * extent)])
+    cam = make_camera((0.0, 0.3 * extent + 1.0, zc + 0.55 * extent + 1.5), (0.0, -0.4, zc + 0.1 * extent), 60.0,
+                      width, height)
+    return SceneData(verts, normals, mat, mats, lights, cam, 2, "F: instanced spheres",
+                     {"instances": instances, "per_instance": n_inst}, xforms, xform_index)
+
+
+def glass(width: int = 512, height: int = 384, level: int = 3, depth: int = 3) -> SceneData:
+    """Transparent materials: a glass icosphere (refractive index 1.5), a sphere
+    that is both reflective and transparent (water, 1.333; its hits spawn two
+    rays), a mirror sphere, over a diffuse plane; air outside (1.0002, the
+    reference driver's ambient index, src/Material.cpp:25 and src/main.cpp:364)."""
+    a, an = icosphere(level, 0.7, (-1.2, 0.0, 0.0))
+    b, bn = icosphere(level, 0.7, (0.3, 0.0, -0.6))
+    c, cn = icosphere(level, 0.5, (1.4, -0.2, 0.4))
+    pv, pn = quad_grid(24, 4.0, -0.7)
+    verts = np.concatenate([a, b, c, pv])
+    normals = np.concatenate([an, bn, cn, pn])
+    mat = np.concatenate([np.full(len(a), 0, np.int32), np.full(len(b), 1, np.int32),
+                          np.full(len(c), 2, np.int32), np.full(len(pv), 3, np.int32)])
+    mats = np.stack([
+        make_material((0.05, 0.05, 0.08), (0.6, 0.6, 0.6), 96, 0.0, False, True, 1.5),
+        make_material((0.05, 0.1, 0.15), (0.5, 0.5, 0.5), 48, 0.35, True, True, 1.333),
+        make_material((0.2, 0.2, 0.2), (0.8, 0.8, 0.8), 64, 0.6, True),
+        make_material((0.6, 0.55, 0.5), (0.1, 0.1, 0.1), 8, 0.0, False)])
+    lights = make_lights([(2.0, 4.0, 3.0), (-3.0, 3.0, 1.0)], [(0.8, 0.8, 0.8), (0.5, 0.5, 0.6)])
+    cam = make_camera((0.0, 0.9, 3.6), (0.0, -0.1, 0.0), 55.0, width, height)
+    s = SceneData(verts, normals, mat, mats, lights, cam, depth, "glass spheres")
+    s.ambient_index = 1.0002
+    return s
 
 
 def perturb_params(scene: SceneData, seed: int, rel: float = 0.10):

@@ -28,6 +28,7 @@
 
 #include <stdint.h>
 
+#include <utility>
 #include <vector>
 
 #include "../eos_c.h"
@@ -52,13 +53,20 @@ struct vec4 {
     vec4(const V& v) : x((float)v.x), y((float)v.y), z((float)v.z), w((float)v.w) {}  // NOLINT
 };
 
-/* Column-major 4x4 (glm layout); only the identity is supported on the GPU path. */
+/* Column-major 4x4 (glm layout, m[column][row]): a triangle's local-to-world
+ * transformation (Scene::addTransformation + Triangle::setTransformationIndex). */
 struct mat4 {
     float m[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
     mat4() = default;
     explicit mat4(float d) {
         for (int c = 0; c < 4; ++c)
             for (int r = 0; r < 4; ++r) m[c][r] = c == r ? d : 0.0f;
+    }
+    /* from glm::mat4 (or any type indexed [column][row]) */
+    template <class M, class = decltype(std::declval<const M&>()[0][0] + 0.0f)>
+    mat4(const M& g) {  // NOLINT
+        for (int c = 0; c < 4; ++c)
+            for (int r = 0; r < 4; ++r) m[c][r] = (float)g[c][r];
     }
     bool is_identity() const;
 };
@@ -177,7 +185,7 @@ public:
     TriangleMesh& operator=(const TriangleMesh&) = delete;
     bool loadFromFile(const char* filename);
     const eos::mat4& transformation() { return m_LocalToWorldTransformation; }
-    void setTransformation(eos::mat4& transformation) { m_LocalToWorldTransformation = transformation; }
+    void setTransformation(const eos::mat4& transformation) { m_LocalToWorldTransformation = transformation; }
     Material& getMaterial() { return m_Material; }
     void setMaterial(const Material& material) { m_Material = material; }
     unsigned int getNumTriangles() { return (unsigned int)m_Triangles.size(); }
@@ -272,12 +280,17 @@ public:
     /* Reflection bounces (the CPU RayTracer's setTracedDepth); default 4. */
     void setTracedDepth(int depth) { if (depth > 0) m_Depth = depth; }
     int getTracedDepth() const { return m_Depth; }
+    /* n x n anti-aliasing samples per pixel (the CPU RayTracer's setAASamples,
+     * src/RayTracer.cpp:33-36,176-266); 0, the default, renders one ray per pixel. */
+    void setAASamples(int samples) { if (samples > 0) m_AASamples = samples; }
+    int getAASamples() const { return m_AASamples; }
     float getLastRenderMilliseconds() const { return m_LastMs; }
 
 private:
     Camera* m_Camera;
     int m_Width, m_Height;
     int m_Depth = RAYTRACER_DEFAULT_TRACE_DEPTH;
+    int m_AASamples = 0;
     float m_LastMs = 0.0f;
 };
 

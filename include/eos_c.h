@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EOS_ABI_VERSION 1
+#define EOS_ABI_VERSION 2
 
 typedef enum eos_status {
     EOS_OK = 0,
@@ -53,14 +53,17 @@ typedef struct eos_camera {
 /* Material (src/Material.h:68-77): diffuse and specular colours (RGBA, A
  * unused), Phong exponent (the reference stores an int; integer values give
  * the reference's result, other values are used as is), reflection weight and
- * flag.  Transparency is not part of this path. */
+ * flag, transparency flag and refractive index (the Fresnel-weighted second
+ * ray of src/RayTracer.cpp:361-408). */
 typedef struct eos_material {
     float kd[4];
     float ks[4];
     float shininess;
     float reflectivity;
     int32_t reflective;
-    int32_t pad;
+    int32_t transparent;
+    float refractive_index;
+    int32_t pad[3];
 } eos_material;
 
 /* Point light (src/LightSource.h:35-36): position (w must be 1) and colour. */
@@ -70,7 +73,8 @@ typedef struct eos_light {
 } eos_light;
 
 /* Scene description (replaces the host Scene handed to createCudaScene,
- * src/cudaStructures.h:98).  Triangles with identity transforms. */
+ * src/cudaStructures.h:98).  Triangles are in world space unless
+ * eos_set_transforms gives them local-to-world transformations. */
 typedef struct eos_scene_desc {
     int32_t num_triangles;
     const float* vertices;          /* [N][3][3]: v1, v2, v3 */
@@ -85,6 +89,7 @@ typedef struct eos_scene_desc {
 
 #define EOS_MAX_LIGHTS 32
 #define EOS_MAX_DEPTH 8
+#define EOS_MAX_AA 16
 
 typedef struct eos_scene eos_scene;
 
@@ -111,6 +116,9 @@ typedef struct eos_render_opts {
     int32_t* hit_out;         /* [H][W] primary hit: original triangle index or -1 */
     int32_t keep_records;     /* keep hit records for eos_render_adjoint */
     int32_t count_visits;     /* run the counting traversal (node visits, triangle tests) */
+    int32_t aa_samples;       /* RayTracer::setAASamples (src/RayTracer.cpp:176-266): n x n samples per
+                                 pixel on a regular grid, averaged; 0 or 1: one ray through the pixel
+                                 corner.  hit_out then holds sample (0, 0); keep_records needs 0 or 1 */
 } eos_render_opts;
 
 typedef struct eos_render_stats {
@@ -147,6 +155,22 @@ eos_status eos_destroy_scene(eos_scene* scene);
 eos_status eos_build_lbvh(eos_scene* scene, int32_t leaf_max, eos_build_stats* stats);
 /* Same, choosing the tree width (2 = binary nodes, 4 = 4-wide nodes). */
 eos_status eos_build_lbvh_ex(eos_scene* scene, const eos_build_opts* opts, eos_build_stats* stats);
+
+/* Per-triangle local-to-world transformations.  Replaces Scene::addTransformation
+ * (src/Scene.cpp:96-105) with Triangle::setTransformationIndex (src/Triangle.h:59),
+ * which createCudaScene copies to the device (copyTransformations and the
+ * triangles' transformationIndex, src/cudaStructures.cu:220-262,285-401).
+ * transforms: [T][16] float, column-major (glm layout, m[4*c + r]), host memory;
+ * transform_index: [N] int32 (-1 = world space), memory per index_memory.
+ * Each matrix must be invertible; the inverse is computed once, in glm's
+ * operation order, and the transpose of the inverse transforms normals, as the
+ * reference's TriangleMesh::setTransformation does (src/triangleMesh.cpp:246-250).
+ * A ray meets a triangle in the triangle's local space (src/BVH.cpp:359-375);
+ * the LBVH is built over world-space vertices.  num_transforms = 0 removes all
+ * transformations.  Takes effect at the next eos_build_lbvh, which rendering
+ * requires after this call. */
+eos_status eos_set_transforms(eos_scene* scene, int32_t num_transforms, const float* transforms,
+                              const int32_t* transform_index, int32_t index_memory);
 
 /* Parameter updates without a rebuild (materials / lights only). */
 eos_status eos_update_materials(eos_scene* scene, const eos_material* materials, int32_t n);
@@ -250,7 +274,8 @@ enum {
     EOS_DBG_NODES = 6,           /* float  [N-1][16] 64-byte traversal nodes */
     EOS_DBG_TRIS = 7,            /* float  [N][12] sorted triangle records */
     EOS_DBG_MORTON = 8,          /* uint32 [N] unsorted codes */
-    EOS_DBG_NODES4 = 9           /* float  [num_nodes4][32] 128-byte 4-wide nodes */
+    EOS_DBG_NODES4 = 9,          /* float  [num_nodes4][32] 128-byte 4-wide nodes */
+    EOS_DBG_XFORMS = 10          /* float  [T][36] rows 0-2 of inverse, M and transpose(inverse) */
 };
 eos_status eos_debug_read(const eos_scene* scene, int32_t what, void* host_dst, int64_t bytes);
 

@@ -27,7 +27,13 @@ extern "C" {
 #endif
 
 typedef struct { float position[3], view[3], right[3], up[3]; int32_t width, height; } orc_camera;
-typedef struct { float kd[4], ks[4]; float shininess, reflectivity; int32_t reflective, pad; } orc_material;
+typedef struct {
+    float kd[4], ks[4];
+    float shininess, reflectivity;
+    int32_t reflective, transparent;
+    float refractive_index;
+    int32_t pad[3];
+} orc_material;
 typedef struct { float position[4], color[4]; } orc_light;
 
 typedef struct {
@@ -46,6 +52,9 @@ typedef struct {
     int32_t n_xform;
     const float* xforms;        /* [n_xform][16] column-major (glm layout) */
     const int32_t* xform_index; /* [n_tri], -1 = none; may be NULL */
+    /* Scene::setAmbientRefractiveIndex (src/Scene.cpp:134-136): n1 or n2 of
+     * the Fresnel term at transparent materials (src/RayTracer.cpp:372-384). */
+    float ambient_index;
 } orc_scene;
 
 typedef struct {
@@ -80,6 +89,10 @@ void orc_nearest(const orc_scene* s, const void* accel, int32_t n, const float* 
 int orc_render(const orc_scene* s, const void* accel, const orc_camera* cam, int depth,
                int row_start, int row_step, uint8_t* rgba, float* rgb, int32_t* hit_ids,
                orc_rec* recs, orc_counts* counts);
+/* src/RayTracer.cpp:176-266: n x n samples per pixel (zero jitter), summed in
+ * (q, p) order, scaled by (1/n)^2, quantised.  rgb optional. */
+int orc_render_aa(const orc_scene* s, const void* accel, const orc_camera* cam, int depth, int n, uint8_t* rgba,
+                  float* rgb, orc_counts* counts);
 
 /* glm 0.9.6 4x4 inverse (glm/detail/type_mat4x4.inl:37-96), column-major. */
 void orc_mat4_inverse(const float* m, float* out);
@@ -87,6 +100,8 @@ void orc_mat4_inverse(const float* m, float* out);
 /* Primitive-level entry points for the closed-form tests. */
 int orc_mt(const float* o, const float* d, const float* tri9, float* tuv);
 void orc_camera_ray(const orc_camera* cam, int x, int y, float* dir3);
+/* Sample (p, q) of n x n in pixel (x, y), renderWithAA's formula (src/RayTracer.cpp:222-230). */
+void orc_camera_ray_aa(const orc_camera* cam, int x, int y, int p, int q, int n, float* dir3);
 uint8_t orc_quantize(float c);
 float orc_shade_phong(const orc_scene* s, const float* cam_pos, int light, int mat,
                       const float* p, const float* n, float* out3);

@@ -146,6 +146,22 @@ inline V3 camera_dir(const orc_camera& c, int x, int y) {
     return normalize(d);
 }
 
+// src/RayTracer.cpp:196-230: sample (p, q) of an n x n grid in pixel (x, y),
+// aa = (x + (p + ksi) * (1/n) - W/2) * (2/W) and the same for y with q.  The
+// jitter ksi = rand() / RAND_MAX is an integer division (:222), 0 unless
+// rand() returns RAND_MAX; it is taken as 0.  1/n is a double division
+// rounded to float (`float inv = 1.0 / (float) n`).
+inline V3 camera_dir_aa(const orc_camera& c, int x, int y, int p, int q, float inv_n) {
+    float nw = (float)c.width / 2.0f, nh = (float)c.height / 2.0f;
+    float inw = 1.0f / nw, inh = 1.0f / nh;
+    const float ksi = 0.0f;
+    float bb = (((float)y + (((float)q + ksi) * inv_n)) - nh) * inh;
+    float aa = (((float)x + (((float)p + ksi) * inv_n)) - nw) * inw;
+    V3 r = ld3(c.right), u = ld3(c.up), v = ld3(c.view);
+    V3 d = (mul(aa, r) + mul(bb, u)) + v;
+    return normalize(d);
+}
+
 // ------------------------------------------------------------ exact accel
 // A median-split BVH over double-precision boxes inflated far beyond any
 // rounding of the float Moller-Trumbore test; traversal only prunes with
@@ -407,6 +423,113 @@ void shade_path(const orc_scene& s, const orc_camera& cam, int depth, V3 o, V3 d
     for (int k = 0; k < 3; ++k) rgb[k] = c[k];
 }
 
+// src/RayTracer.cpp:52-65 (Bram de Greve's Fresnel reflectance): 1 on total
+// internal reflection, else the mean of the squared s and p amplitudes.
+inline float fresnel(V3 incident, V3 normal, float n1, float n2) {
+    const float n = n1 / n2;
+    const float cosI = -dot(normal, incident);
+    const float sinT2 = n * n * (1.0f - cosI * cosI);
+    if (sinT2 > 1.0f) return 1.0f;
+    const float cosT = std::sqrt(1.0f - sinT2);
+    const float rOrth = (n1 * cosI - n2 * cosT) / (n1 * cosI + n2 * cosT);
+    const float rPar = (n2 * cosI - n1 * cosT) / (n2 * cosI + n1 * cosT);
+    return (rOrth * rOrth + rPar * rPar) / 2.0f;
+}
+
+// The second ray of a transparent material (src/RayTracer.cpp:361-408): the
+// normal faces the incoming ray (negated, as 0 - N, when the ray leaves the
+// material), K = fresnel(normalize(d), normal, n1, n2), and the ray is the
+// reflection of d about that normal from P + 1e-3 * normal.  The reference
+// traces this same ray a second time where it means to trace the refracted
+// one (:403), so its colour t enters as K*t + (1-K)*t.
+struct FresnelRay {
+    V3 o, d;
+    float K;
+};
+inline FresnelRay fresnel_ray(const orc_scene& s, const orc_material& m, V3 d, V3 P, V3 N) {
+    const V3 inc = normalize(d);
+    const float idn = dot(inc, N);
+    float n1, n2;
+    V3 nn = N;
+    if (idn < 0.0f) {
+        n1 = s.ambient_index;
+        n2 = m.refractive_index;
+    } else {
+        n1 = m.refractive_index;
+        n2 = s.ambient_index;
+        nn = V3{0.0f - N.x, 0.0f - N.y, 0.0f - N.z};
+    }
+    FresnelRay f;
+    f.K = fresnel(inc, nn, n1, n2);
+    f.d = normalize(reflect(d, nn));
+    f.o = P + mul(1e-3f, nn);
+    return f;
+}
+
+// RayTracer::rayTrace / shadeIntersection as the reference's recursion, for
+// scenes with transparent materials (a ray may have two children): colour =
+// 0 + ((phong + 0 + r * c_reflect) + (0 + (K * t + (1 - K) * t))).  Without
+// transparent materials it equals shade_path's chain bit for bit.
+template <class NearestF, class AnyF>
+void shade_tree(const orc_scene& s, const orc_camera& cam, int depth, int level, V3 o, V3 d, NearestF& nearest_f,
+                AnyF& any_f, float rgb[3], int32_t* prim_hit, orc_counts& cnt) {
+    rgb[0] = rgb[1] = rgb[2] = 0.0f;
+    Hit h;
+    nearest_f(o, d, h);
+    if (level == 0 && prim_hit) *prim_hit = h.tri;
+    if (h.tri < 0) return;
+    cnt.hits++;
+    V3 P, N;
+    hit_geometry(s, h.tri, o, d, h.t, h.u, h.v, P, N);
+    const orc_material& m = s.mats[s.mat[h.tri]];
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < s.n_light; ++i) {
+        const orc_light& l = s.lights[i];
+        V3 Ls = normalize(ld3(l.position) - P);
+        V3 so = P + mul(1e-3f, Ls);
+        cnt.shadow++;
+        if (!any_f(so, Ls)) {
+            float c[3];
+            phong(m, l, ld3(cam.position), P, N, c);
+            for (int k = 0; k < 3; ++k) acc[k] = acc[k] + c[k];
+        }
+    }
+    float refl[3] = {0.0f, 0.0f, 0.0f}, refr[3] = {0.0f, 0.0f, 0.0f};
+    if (m.reflective) {
+        float c[3] = {0.0f, 0.0f, 0.0f};
+        if (level < depth) {
+            V3 R = normalize(reflect(d, N));
+            cnt.secondary++;
+            shade_tree(s, cam, depth, level + 1, P + mul(1e-3f, R), R, nearest_f, any_f, c, nullptr, cnt);
+        }
+        for (int k = 0; k < 3; ++k) refl[k] = 0.0f + m.reflectivity * c[k];
+    }
+    if (m.transparent) {
+        const FresnelRay f = fresnel_ray(s, m, d, P, N);
+        float t[3] = {0.0f, 0.0f, 0.0f};
+        if (level < depth) {
+            cnt.secondary++;
+            shade_tree(s, cam, depth, level + 1, f.o, f.d, nearest_f, any_f, t, nullptr, cnt);
+        }
+        for (int k = 0; k < 3; ++k) refr[k] = 0.0f + (f.K * t[k] + (1.0f - f.K) * t[k]);
+    }
+    for (int k = 0; k < 3; ++k) rgb[k] = 0.0f + ((acc[k] + refl[k]) + refr[k]);
+}
+
+inline bool any_transparent(const orc_scene& s) {
+    for (int i = 0; i < s.n_mat; ++i)
+        if (s.mats[i].transparent) return true;
+    return false;
+}
+
+template <class NearestF, class AnyF>
+void shade_ray(const orc_scene& s, const orc_camera& cam, int depth, V3 o, V3 d, NearestF&& nearest_f, AnyF&& any_f,
+               float rgb[3], int32_t* prim_hit, orc_rec* recs, int64_t stride, orc_counts& cnt, bool tree) {
+    if (tree) shade_tree(s, cam, depth, 0, o, d, nearest_f, any_f, rgb, prim_hit, cnt);
+    else shade_path(s, cam, depth, o, d, nearest_f, any_f, rgb, prim_hit, recs, stride, cnt);
+}
+
+
 // ------------------------------------------------------------ Morton / LBVH
 // src/morton.h:30-48
 inline uint32_t part1by2(uint32_t x) {
@@ -464,6 +587,11 @@ int orc_mt(const float* o, const float* d, const float* tri9, float* tuv) {
 
 void orc_camera_ray(const orc_camera* cam, int x, int y, float* dir3) {
     V3 d = camera_dir(*cam, x, y);
+    dir3[0] = d.x; dir3[1] = d.y; dir3[2] = d.z;
+}
+
+void orc_camera_ray_aa(const orc_camera* cam, int x, int y, int p, int q, int n, float* dir3) {
+    V3 d = camera_dir_aa(*cam, x, y, p, q, (float)(1.0 / (double)(float)n));
     dir3[0] = d.x; dir3[1] = d.y; dir3[2] = d.z;
 }
 
@@ -531,6 +659,8 @@ int orc_render(const orc_scene* s, const void* accel, const orc_camera* cam, int
                orc_counts* counts) {
     const Accel& A = *static_cast<const Accel*>(accel);
     const int W = cam->width, H = cam->height;
+    const bool tree = any_transparent(*s);
+    if (tree && recs) return -1;   // hit records describe chains of reflections only
     if (row_step < 1) row_step = 1;
     const int64_t npix = (int64_t)W * H;
     if (recs)
@@ -551,17 +681,61 @@ int orc_render(const orc_scene* s, const void* accel, const orc_camera* cam, int
             V3 d = camera_dir(*cam, x, y);
             float col[3];
             c.primary++;
-            shade_path(
+            shade_ray(
                 *s, *cam, depth, ld3(cam->position), d,
                 [&](V3 o, V3 dd, Hit& h) { nearest(A, *s, o, dd, h); },
                 [&](V3 o, V3 dd) { return anyhit(A, *s, o, dd); }, col, hit_ids ? hit_ids + p : nullptr,
-                recs ? recs + p : nullptr, npix, c);
+                recs ? recs + p : nullptr, npix, c, tree);
             if (rgb)
                 for (int k = 0; k < 3; ++k) rgb[3 * p + k] = col[k];
             if (rgba) {
                 for (int k = 0; k < 3; ++k) rgba[4 * p + k] = quantize(col[k]);
                 rgba[4 * p + 3] = 255;
             }
+        }
+        prim += c.primary; shad += c.shadow; sec += c.secondary; hits += c.hits;
+    }
+    if (counts) {
+        counts->primary = prim; counts->shadow = shad; counts->secondary = sec; counts->hits = hits;
+        counts->node_visits = 0; counts->tri_tests = 0;
+    }
+    return 0;
+}
+
+// src/RayTracer.cpp:176-266 (renderWithAA): n x n samples per pixel, colours
+// summed in (q, p) order into a zero vec4, scaled by (1/n)^2, quantised.
+int orc_render_aa(const orc_scene* s, const void* accel, const orc_camera* cam, int depth, int n, uint8_t* rgba,
+                  float* rgb, orc_counts* counts) {
+    const Accel& A = *static_cast<const Accel*>(accel);
+    const int W = cam->width, H = cam->height;
+    const bool tree = any_transparent(*s);
+    if (n < 1) n = 1;
+    const float inv_n = (float)(1.0 / (double)(float)n);
+    const float inv_n2 = inv_n * inv_n;
+    int64_t prim = 0, shad = 0, sec = 0, hits = 0;
+#pragma omp parallel for schedule(dynamic, 2) reduction(+ : prim, shad, sec, hits)
+    for (int y = 0; y < H; ++y) {
+        orc_counts c{};
+        for (int x = 0; x < W; ++x) {
+            float acc[3] = {0.0f, 0.0f, 0.0f};
+            for (int q = 0; q < n; ++q)
+                for (int pp = 0; pp < n; ++pp) {
+                    V3 d = camera_dir_aa(*cam, x, y, pp, q, inv_n);
+                    float col[3];
+                    c.primary++;
+                    shade_ray(
+                        *s, *cam, depth, ld3(cam->position), d,
+                        [&](V3 o, V3 dd, Hit& h) { nearest(A, *s, o, dd, h); },
+                        [&](V3 o, V3 dd) { return anyhit(A, *s, o, dd); }, col, nullptr, nullptr, 0, c, tree);
+                    for (int k = 0; k < 3; ++k) acc[k] = acc[k] + col[k];
+                }
+            const int64_t p = (int64_t)y * W + x;
+            for (int k = 0; k < 3; ++k) {
+                const float v = acc[k] * inv_n2;
+                if (rgb) rgb[3 * p + k] = v;
+                if (rgba) rgba[4 * p + k] = quantize(v);
+            }
+            if (rgba) rgba[4 * p + 3] = 255;
         }
         prim += c.primary; shad += c.shadow; sec += c.secondary; hits += c.hits;
     }
@@ -1115,6 +1289,7 @@ int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, co
                     const orc_camera* cam, int depth, int row_start, int row_step, uint8_t* rgba, float* rgb,
                     int32_t* hit_ids, orc_counts* counts, int32_t width) {
     const int W = cam->width, H = cam->height;
+    const bool tree = any_transparent(*s);
     if (row_step < 1) row_step = 1;
     int64_t prim = 0, shad = 0, sec = 0, hits = 0, vis = 0, tst = 0;
     std::vector<int> rows;
@@ -1133,7 +1308,7 @@ int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, co
             V3 d = camera_dir(*cam, x, y);
             float col[3];
             c.primary++;
-            shade_path(
+            shade_ray(
                 *s, *cam, depth, ld3(cam->position), d,
                 [&](V3 o, V3 dd, Hit& h) {
                     if (empty) return;
@@ -1147,7 +1322,7 @@ int orc_render_lbvh(const orc_scene* s, const float* nodes, int32_t root_ref, co
                     LHit lh;
                     return trace_w<true>(width, nodes, root_ref, tris, xinv, o, dd, lh, v, t);
                 },
-                col, hit_ids ? hit_ids + p : nullptr, nullptr, 0, c);
+                col, hit_ids ? hit_ids + p : nullptr, nullptr, 0, c, tree);
             if (rgb)
                 for (int k = 0; k < 3; ++k) rgb[3 * p + k] = col[k];
             if (rgba) {

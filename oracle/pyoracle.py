@@ -24,7 +24,8 @@ class OrcScene(C.Structure):
     _fields_ = [("n_tri", C.c_int32), ("verts", C.c_void_p), ("normals", C.c_void_p),
                 ("mat", C.c_void_p), ("n_mat", C.c_int32), ("mats", C.c_void_p),
                 ("n_light", C.c_int32), ("lights", C.c_void_p),
-                ("n_xform", C.c_int32), ("xforms", C.c_void_p), ("xform_index", C.c_void_p)]
+                ("n_xform", C.c_int32), ("xforms", C.c_void_p), ("xform_index", C.c_void_p),
+                ("ambient_index", C.c_float)]
 
 
 def _xform_arrays(scene):
@@ -48,7 +49,8 @@ def _orc_scene(holder, scene):
     holder.n_xform, holder.xforms, holder.xform_index = _xform_arrays(scene)
     return OrcScene(holder.verts.shape[0], _p(holder.verts), _p(holder.normals), _p(holder.mat),
                     len(holder.materials), _p(holder.materials), len(holder.lights), _p(holder.lights),
-                    holder.n_xform, _p(holder.xforms), _p(holder.xform_index))
+                    holder.n_xform, _p(holder.xforms), _p(holder.xform_index),
+                    float(getattr(scene, "ambient_index", 1.0)))
 
 
 class OrcCounts(C.Structure):
@@ -79,8 +81,10 @@ def lib():
         L.orc_render.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.orc_render_lbvh.argtypes = [vp, vp, i32, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, i32]
         L.orc_mt.argtypes = [vp, vp, vp, vp]
+        L.orc_render_aa.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp]
         L.orc_mat4_inverse.argtypes = [vp, vp]
         L.orc_camera_ray.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.orc_camera_ray_aa.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]
         L.orc_quantize.restype = C.c_uint8
         L.orc_quantize.argtypes = [C.c_float]
         L.orc_morton3.restype = C.c_uint32
@@ -143,6 +147,18 @@ class Oracle:
         if any_hit:
             return tri.astype(bool)
         return {"tri": tri, "t": t, "u": u, "v": v, "point": P, "normal": N}
+
+    def render_aa(self, n, camera=None, depth=None):
+        """renderWithAA: n x n samples per pixel."""
+        cam = np.array(self.scene.camera if camera is None else camera, copy=True)
+        depth = self.scene.depth if depth is None else depth
+        W, H = int(cam["width"]), int(cam["height"])
+        rgba = np.zeros((H, W, 4), np.uint8)
+        rgb = np.zeros((H, W, 3), np.float32)
+        cnt = OrcCounts()
+        lib().orc_render_aa(C.byref(self._s), self._accel, cam.ctypes.data, depth, n, _p(rgba), _p(rgb),
+                            C.byref(cnt))
+        return {"rgba": rgba, "rgb": rgb, "counts": cnt.as_dict()}
 
     def render(self, camera=None, depth=None, row_start=0, row_step=1, want_recs=False):
         cam = np.array(self.scene.camera if camera is None else camera, copy=True)

@@ -334,6 +334,18 @@ struct Ctx {
     int64_t shadow, secondary, hits;
 };
 
+// src/RayTracer.cpp:52-65
+inline float fresnel(V3 incident, V3 normal, float n1, float n2) {
+    const float n = n1 / n2;
+    const float cosI = -dot(normal, incident);
+    const float sinT2 = n * n * (1.0f - cosI * cosI);
+    if (sinT2 > 1.0f) return 1.0f;
+    const float cosT = std::sqrt(1.0f - sinT2);
+    const float rOrth = (n1 * cosI - n2 * cosT) / (n1 * cosI + n2 * cosT);
+    const float rPar = (n2 * cosI - n1 * cosT) / (n2 * cosI + n1 * cosT);
+    return (rOrth * rOrth + rPar * rPar) / 2.0f;
+}
+
 // src/RayTracer.cpp:268-458
 void trace(Ctx& c, const RRay& ray, int depth, float out[3]) {
     out[0] = out[1] = out[2] = 0.0f;
@@ -374,7 +386,31 @@ void trace(Ctx& c, const RRay& ray, int depth, float out[3]) {
         trace(c, tmp, depth + 1, sub);
         for (int k = 0; k < 3; ++k) refl[k] = m.reflectivity * sub[k];
     }
-    for (int k = 0; k < 3; ++k) out[k] = phong[k] + refl[k];
+    float refr[3] = {0, 0, 0};
+    if (m.transparent) {
+        // src/RayTracer.cpp:361-408, tracing the reflected ray twice as :401-403 do
+        V3 inc = normalize(ray.d);
+        float n1 = m.refractive_index, n2 = c.s->ambient_index;
+        V3 nn = x.n;
+        if (dot(inc, x.n) < 0.0f) {
+            n1 = c.s->ambient_index;
+            n2 = m.refractive_index;
+        } else {
+            nn = V3{0.0f - x.n.x, 0.0f - x.n.y, 0.0f - x.n.z};
+        }
+        const float K = fresnel(inc, nn, n1, n2);
+        float dNI = dot(nn, ray.d);
+        V3 t = {(nn.x * dNI) * 2.0f, (nn.y * dNI) * 2.0f, (nn.z * dNI) * 2.0f};
+        V3 Rd = normalize(ray.d - t);
+        RRay fr;
+        fr.set(x.p + mul(1e-3f, nn), Rd);
+        if (depth + 1 <= c.depth) c.secondary += 2;
+        float a[3], b[3];
+        trace(c, fr, depth + 1, a);
+        trace(c, fr, depth + 1, b);
+        for (int k = 0; k < 3; ++k) refr[k] = 0.0f + (K * a[k] + (1.0f - K) * b[k]);
+    }
+    for (int k = 0; k < 3; ++k) out[k] = (phong[k] + refl[k]) + refr[k];
 }
 
 inline uint8_t quantize(float c) {

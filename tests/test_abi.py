@@ -49,7 +49,7 @@ def test_struct_layouts_match_header(tmp_path):
         assert int(got[cname]) == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"
-    assert int(got["eos_camera"]) == 56 and int(got["eos_material"]) == 48 and int(got["eos_light"]) == 32
+    assert int(got["eos_camera"]) == 56 and int(got["eos_material"]) == 64 and int(got["eos_light"]) == 32
 
 
 def test_null_arguments_are_rejected():
@@ -63,7 +63,8 @@ def test_null_arguments_are_rejected():
     assert L.eos_render_batch(None, None, 0, None, None, None) == E.EOS_ERR_INVALID
     assert L.eos_debug_read(None, 1, None, 0) == E.EOS_ERR_INVALID
     assert L.eos_destroy_scene(None) == E.EOS_OK
-    assert L.eos_abi_version() == 1
+    assert L.eos_abi_version() == 2
+    assert L.eos_set_transforms(None, 0, None, None, 0) == E.EOS_ERR_INVALID
 
 
 def test_no_device_is_reported():

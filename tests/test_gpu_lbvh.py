@@ -49,7 +49,7 @@ def test_lbvh_bit_exact(name, leaf_max):
     assert np.array_equal(g.debug_read("morton_sorted"), ref["codes"])
     assert np.array_equal(g.debug_read("perm"), ref["perm"])
     assert g.build_stats["root_ref"] == ref["root"]
-    assert np.array_equal(_u32(g.debug_read("tris"))[:, :10], _u32(ref["tris"])[:, :10])
+    assert np.array_equal(_u32(g.debug_read("tris"))[:, :11], _u32(ref["tris"])[:, :11])
     assert g.build_stats["num_nodes4"] == len(ref["nodes4"])
     assert g.build_stats["root4"] == ref["root4"]
     if len(ref["nodes4"]):

@@ -73,3 +73,39 @@ def test_cpp_api_triangle_mesh_from_obj(tmp_path):
     multi = np.fromfile(o1, np.uint8).reshape(s.height, s.width, 4)
     single = np.fromfile(o2, np.uint8).reshape(s.height, s.width, 4)
     assert np.array_equal(multi, ref) and np.array_equal(single, ref)
+
+
+def test_cpp_api_translated_mesh(tmp_path):
+    """The reference driver's route for its OBJ model (src/main.cpp:386-403): a
+    translation through Scene::addTransformation and the mesh's transformation
+    index renders like the C ABI with that per-triangle transformation."""
+    from eos_amd import _lib as E
+    s = scenes.config_a(64, 48)
+    sphere = s.mat == s.mat[0]
+    rest = scenes.SceneData(s.verts[~sphere], s.normals[~sphere], s.mat[~sphere], s.materials, s.lights, s.camera,
+                            s.depth, "plane")
+    sc, obj = tmp_path / "plane.bin", tmp_path / "sphere.obj"
+    _write_scene(sc, rest)
+    scenes.write_obj(str(obj), s.verts[sphere], s.normals[sphere])
+    o1, o2 = tmp_path / "multi.rgba", tmp_path / "single.rgba"
+    t = ("0.75", "-0.25", "-0.5")
+    print("running the C++ example with a translated OBJ mesh", flush=True)
+    r = subprocess.run([DEMO, str(sc), str(o1), str(o2), str(obj), str(int(s.mat[0])), *t], capture_output=True,
+                       text=True, timeout=120, env=dict(os.environ, EOS_QUIET="1"))
+    assert r.returncode == 0, r.stderr
+    order = np.concatenate([np.flatnonzero(~sphere), np.flatnonzero(sphere)])
+    m = np.eye(4, dtype=np.float32)
+    m[3, :3] = [float(x) for x in t]               # column-major: column 3 holds the translation
+    xi = np.concatenate([np.full((~sphere).sum(), -1, np.int32), np.zeros(sphere.sum(), np.int32)])
+    whole = scenes.SceneData(s.verts[order], s.normals[order], s.mat[order], s.materials, s.lights, s.camera,
+                             s.depth, "translated", {}, m.ravel()[None], xi)
+    ref = E.Scene(whole, leaf_max=4).render(want_hits=True)
+    multi = np.fromfile(o1, np.uint8).reshape(s.height, s.width, 4)
+    single = np.fromfile(o2, np.uint8).reshape(s.height, s.width, 4)
+    assert np.array_equal(multi, ref["rgba"]) and np.array_equal(single, ref["rgba"])
+    # the sphere moved: the oracle with the same transformation agrees
+    oo = po.Oracle(whole).render()
+    assert (ref["hit"] == oo["hit"]).mean() >= 0.9999
+    plain = E.Scene(scenes.SceneData(whole.verts, whole.normals, whole.mat, s.materials, s.lights, s.camera,
+                                     s.depth, "unmoved"), leaf_max=4).render()["rgba"]
+    assert not np.array_equal(plain, ref["rgba"])

@@ -406,3 +406,103 @@ def test_fp64_forward_matches_fp32_image_and_gradients_agree():
     assert scale > 0
     err = np.abs(g_fwd - g_fd) / np.maximum(np.abs(g_fwd), 1e-3 * scale)
     assert err.max() < 1e-3, err.max()
+
+
+# ---------------------------------------------------------------- transformations
+
+def _small_instances(rotate=True):
+    return scenes.config_f(64, 48, instances=8, nu=20, nv=10, rotate=rotate)
+
+
+def _world_scene(s):
+    """The same scene with every transformation applied to its vertices (float64,
+    then rounded) and normals (inverse transpose, not renormalised, so the
+    interpolated world normal is the transformed local one up to scale): world
+    space, no transformations."""
+    v = s.verts.astype(np.float64).copy()
+    n = s.normals.astype(np.float64).copy()
+    for k, m in enumerate(s.xforms):
+        M = m.reshape(4, 4).T.astype(np.float64)
+        sel = s.xform_index == k
+        v[sel] = v[sel] @ M[:3, :3].T + M[:3, 3]
+        n[sel] = n[sel] @ np.linalg.inv(M[:3, :3])
+    w = scenes.SceneData(v.astype(np.float32), n.astype(np.float32), s.mat, s.materials, s.lights, s.camera,
+                         s.depth, s.name)
+    return w
+
+
+def test_mat4_inverse_closed_forms():
+    # rotation: inverse = transpose; scale: reciprocals; translation: negated
+    for rot in _small_instances().xforms[:4]:
+        M = rot.reshape(4, 4).T.astype(np.float64)
+        inv = po.mat4_inverse(rot).reshape(4, 4).T.astype(np.float64)
+        assert np.allclose(inv, np.linalg.inv(M), rtol=1e-5, atol=1e-5)
+    m = scenes.affine((0, 0, 1), 0.0, (2.0, 4.0, 8.0), (1.0, 2.0, 3.0))
+    inv = po.mat4_inverse(m).reshape(4, 4).T
+    assert np.array_equal(np.diag(inv)[:3], np.array([0.5, 0.25, 0.125], np.float32))
+    assert np.array_equal(inv[:3, 3], np.array([-0.5, -0.5, -0.375], np.float32))
+    # a transposed matrix is a different matrix: the inverse notices
+    g = scenes.affine((1, 2, 3), 0.7, (1.0, 2.0, 0.5), (4.0, -1.0, 2.0))
+    a = po.mat4_inverse(g).reshape(4, 4).T.astype(np.float64)
+    b = po.mat4_inverse(g.reshape(4, 4).T.ravel().copy()).reshape(4, 4).T.astype(np.float64)
+    assert not np.allclose(a, b, atol=1e-3)
+
+
+@pytest.mark.parametrize("rotate", [True, False])
+def test_transformed_scene_equals_brute_force_and_lbvh(rotate):
+    s = _small_instances(rotate)
+    acc = po.Oracle(s).render()
+    brute = po.Oracle(s, brute_force=True).render()
+    assert np.array_equal(acc["hit"], brute["hit"]) and np.array_equal(acc["rgba"], brute["rgba"])
+    b = po.lbvh_build(s.verts, 4, s.xforms, s.xform_index)
+    for width in (2, 4):
+        lb = po.render_lbvh(s, b, width=width)
+        assert np.array_equal(lb["hit"], acc["hit"]) and np.array_equal(lb["rgba"], acc["rgba"])
+    pk = po.packet_primary(s, b)
+    assert np.array_equal(pk["hit"], acc["hit"])
+    inst = s.xform_index[acc["hit"][acc["hit"] >= 0]]
+    assert len(np.unique(inst[inst >= 0])) >= 6          # most instances are in view
+
+
+def test_transformed_scene_matches_pre_transformed_geometry():
+    """Local-space tests against the same triangles moved to world space in
+    float64: the same hits up to rounding at triangle edges, the same points
+    and normals to float precision."""
+    s = _small_instances()
+    w = _world_scene(s)
+    d = po.camera_rays(s).reshape(-1, 3)
+    org = np.repeat(s.camera["position"][None, :], len(d), 0).astype(np.float32)
+    a = po.Oracle(s).nearest(org, d)
+    b = po.Oracle(w).nearest(org, d)
+    same = a["tri"] == b["tri"]
+    assert same.mean() > 0.995
+    h = same & (a["tri"] >= 0)
+    assert h.sum() > 1000
+    assert np.allclose(a["t"][h], b["t"][h], rtol=2e-4, atol=1e-5)
+    assert np.allclose(a["point"][h], b["point"][h], atol=5e-4)
+    assert np.allclose(a["normal"][h], b["normal"][h], atol=5e-3)
+
+
+def test_lbvh_keys_come_from_world_vertices():
+    """The build over transformed triangles sorts and boxes exactly as a build
+    over their world-space vertices (computed here in glm's order), while the
+    records keep the local vertices and carry the transformation index."""
+    s = _small_instances()
+    wv = s.verts.copy()
+    for k, m in enumerate(s.xforms):
+        sel = s.xform_index == k
+        p = s.verts[sel]
+        col = m.reshape(4, 4)                               # col[c] = column c
+        out = np.empty_like(p)
+        for r in range(3):
+            out[..., r] = (col[0, r] * p[..., 0] + col[1, r] * p[..., 1]) + (col[2, r] * p[..., 2] + col[3, r])
+        wv[sel] = out
+    a = po.lbvh_build(s.verts, 4, s.xforms, s.xform_index)
+    b = po.lbvh_build(wv, 4)
+    for key in ("codes", "perm", "kchild", "ibox", "nodes", "nodes4"):
+        assert np.array_equal(np.asarray(a[key]).view(np.uint32), np.asarray(b[key]).view(np.uint32)), key
+    perm = a["perm"]
+    loc = s.verts[perm]
+    assert np.array_equal(a["tris"][:, 0:3], loc[:, 0])
+    assert np.array_equal(a["tris"][:, 10].view(np.int32), s.xform_index[perm])
+    assert np.array_equal(b["tris"][:, 10].view(np.int32), np.full(s.n_tri, -1, np.int32))
