@@ -21,7 +21,8 @@ DBG = {"morton_sorted": 1, "perm": 2, "kchild": 3, "kparent": 4, "ibox": 5, "nod
 EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_build_lbvh_ex", "eos_update_materials",
            "eos_update_lights", "eos_render", "eos_render_adjoint", "eos_render_batch",
            "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
-           "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free", "eos_set_transforms"]
+           "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free", "eos_set_transforms",
+           "eos_set_ambient_refractive_index"]
 
 
 class EosError(RuntimeError):
@@ -129,6 +130,7 @@ def lib():
         L.eos_mesh_load_obj.argtypes = [C.c_char_p, vp]
         L.eos_mesh_free.argtypes = [vp]
         L.eos_set_transforms.argtypes = [vp, i32, vp, vp, i32]
+        L.eos_set_ambient_refractive_index.argtypes = [vp, C.c_float]
         for name in EXPORTS:
             if name != "eos_last_error" and name != "eos_abi_version":
                 getattr(L, name).restype = C.c_int
@@ -184,6 +186,9 @@ class Scene:
         xf = getattr(scene, "xforms", None)
         if xf is not None:
             self.set_transforms(xf, scene.xform_index)
+        amb = float(getattr(scene, "ambient_index", 1.0))
+        if amb != 1.0:
+            check(lib().eos_set_ambient_refractive_index(self.h, amb))
         if build:
             self.build(leaf_max, width)
 

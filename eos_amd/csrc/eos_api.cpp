@@ -50,6 +50,7 @@ SceneView view_of(const eos_scene* s) {
     v.lights = s->lights.as<const DevLight>();
     v.xf = s->n_xf > 0 ? s->xf.as<const float4>() : nullptr;
     v.n_xf = s->n_xf;
+    v.ambient = s->ambient_index;
     v.root = s->root;
     v.n_tri = s->n_tri;
     v.n_light = s->n_light;
@@ -87,8 +88,8 @@ static void free_scene(eos_scene* s) {
 static eos_status check_materials(const eos_material* m, int n) {
     for (int i = 0; i < n; ++i) {
         if (!(m[i].shininess >= 0.0f)) return fail(EOS_ERR_INVALID, "material shininess must be >= 0");
-        if (m[i].transparent)
-            return fail(EOS_ERR_INVALID, "transparent materials are not rendered by this build yet");
+        if (m[i].transparent && !(m[i].refractive_index > 0.0f && std::isfinite(m[i].refractive_index)))
+            return fail(EOS_ERR_INVALID, "a transparent material needs a positive finite refractive index");
     }
     return EOS_OK;
 }
@@ -204,6 +205,7 @@ eos_status eos_create_scene(const eos_scene_desc* d, eos_scene** out) {
     }
     s->n_tri = d->num_triangles;
     s->n_mat = d->num_materials;
+    for (int i = 0; i < d->num_materials; ++i) s->any_transparent |= d->materials[i].transparent != 0;
     s->n_light = d->num_lights;
     const size_t n = (size_t)d->num_triangles;
     eos_status st = EOS_OK;
@@ -284,6 +286,16 @@ eos_status eos_update_materials(eos_scene* s, const eos_material* m, int32_t n) 
     EOS_HIP(hipMemcpyAsync(s->mats.ptr, m, sizeof(eos_material) * n, hipMemcpyHostToDevice, s->stream));
     EOS_HIP(hipStreamSynchronize(s->stream));
     s->fb.rec_valid = 0;  // reflection flags and weights shape the stored records
+    s->any_transparent = false;
+    for (int i = 0; i < n; ++i) s->any_transparent |= m[i].transparent != 0;
+    return EOS_OK;
+}
+
+eos_status eos_set_ambient_refractive_index(eos_scene* s, float n) {
+    if (!s) return fail(EOS_ERR_INVALID, "eos_set_ambient_refractive_index: null scene");
+    if (!(n > 0.0f && std::isfinite(n)))
+        return fail(EOS_ERR_INVALID, "eos_set_ambient_refractive_index: the index must be positive and finite");
+    s->ambient_index = n;
     return EOS_OK;
 }
 
@@ -322,6 +334,8 @@ eos_status eos_render(eos_scene* s, const eos_camera* cam, const eos_render_opts
         return fail(EOS_ERR_INVALID, "eos_render: aa_samples must be in [0, 16]");
     if (o->aa_samples > 1 && o->keep_records)
         return fail(EOS_ERR_INVALID, "eos_render: keep_records needs one sample per pixel (aa_samples 0 or 1)");
+    if (o->keep_records && s->any_transparent)
+        return fail(EOS_ERR_INVALID, "eos_render: keep_records needs chains of reflections (no transparent materials)");
     EOS_HIP(hipSetDevice(s->device));
     return render(s, cam, o, rgba, st);
 }
@@ -330,6 +344,8 @@ eos_status eos_render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adj
                               double* gm, double* gl, eos_adjoint_stats* st) {
     if (!s || !cam || !o || !dl || !gm || !gl) return fail(EOS_ERR_INVALID, "eos_render_adjoint: null argument");
     if (!s->built) return fail(EOS_ERR_STATE, "eos_render_adjoint: call eos_build_lbvh first");
+    if (s->any_transparent)
+        return fail(EOS_ERR_INVALID, "eos_render_adjoint: scenes with transparent materials have no adjoint");
     if (o->depth < 0 || o->depth > EOS_MAX_DEPTH) return fail(EOS_ERR_INVALID, "eos_render_adjoint: bad depth");
     EOS_HIP(hipSetDevice(s->device));
     return render_adjoint(s, cam, o, dl, gm, gl, st);

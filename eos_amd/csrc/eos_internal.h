@@ -102,6 +102,7 @@ struct SceneView {
     int32_t n_light;
     int32_t n_mat;
     int32_t n_xf;
+    float ambient;            // refractive index outside transparent materials
 };
 
 // ---------------------------------------------------------------- buffers
@@ -145,6 +146,13 @@ struct eos_frame_buffers {
     eos::DevBuf counters;   // int64  [64]
     eos::DevBuf gstack;     // int32  overflow stacks of persistent lanes
     eos::DevBuf aa_acc;     // float4 [npix]       anti-aliasing sample sums
+    // Ray trees (scenes with transparent materials): level l holds at most
+    // tcap[l] rays / hits at offset toff[l] of rayhit (hit index per ray; level
+    // 0 per pixel), aux (Fresnel weight, children) and hitcol (colour per hit);
+    // surf / vis / queues then hold one level at a time.
+    bool tree = false;
+    size_t tcap[EOS_MAX_DEPTH + 1] = {}, toff[EOS_MAX_DEPTH + 2] = {};
+    eos::DevBuf rayhit, aux, hitcol;
     size_t npix = 0, n_ids = 0;
     int depth = -1, n_light = -1;
     int rec_valid = 0;      // records of the last frame kept (adjoint replay)
@@ -172,6 +180,8 @@ struct eos_scene {
     // (original order, -1 = none), world-space vertices for the build
     eos::DevBuf xf, xf_index, wverts;
     int n_xf = 0;
+    float ambient_index = 1.0f;   // Scene::setAmbientRefractiveIndex
+    bool any_transparent = false; // some material is transparent: frames trace ray trees
     int root = 0;
     int root4 = 0;
     int n_nodes4 = 0;

@@ -170,10 +170,6 @@ cudaScene_t* createCudaScene(Scene* h) {
     std::vector<eos_material> mats((size_t)h->getNumMaterials());
     for (int i = 0; i < h->getNumMaterials(); ++i) {
         const Material& m = h->getMaterialAtIndex(i);
-        if (m.isTransparent()) {
-            std::fprintf(stderr, "Eos Error: material %d is transparent; refraction is not part of this path\n", i);
-            std::exit(1);
-        }
         const eos::vec4 &kd = m.getDiffuseColor(), &ks = m.getSpecularColor();
         eos_material& o = mats[i];
         o.kd[0] = kd.x; o.kd[1] = kd.y; o.kd[2] = kd.z; o.kd[3] = kd.w;
@@ -181,8 +177,8 @@ cudaScene_t* createCudaScene(Scene* h) {
         o.shininess = (float)m.getShininess();
         o.reflectivity = m.getReflectiveIntensity();
         o.reflective = m.isReflective() ? 1 : 0;
-        o.transparent = 0;
-        o.refractive_index = 1.0f;
+        o.transparent = m.isTransparent() ? 1 : 0;
+        o.refractive_index = m.getRefractiveIndex();
         o.pad[0] = o.pad[1] = o.pad[2] = 0;
     }
     std::vector<eos_light> lights((size_t)h->getNumLightSources());
@@ -203,6 +199,7 @@ cudaScene_t* createCudaScene(Scene* h) {
     d.lights = lights.data();
     cudaScene_t* out = new cudaScene_t();
     check(eos_create_scene(&d, &out->scene), "createCudaScene");
+    check(eos_set_ambient_refractive_index(out->scene, h->getAmbientRefractiveIndex()), "createCudaScene (ambient index)");
     if (any_xform) {
         std::vector<float> m(16 * (size_t)h->getNumTransformations());
         for (int k = 0; k < h->getNumTransformations(); ++k)

@@ -41,10 +41,29 @@ struct SetupArgs {
     int* hits_ctr;
     float4* q_out;
     int* q_out_ctr;
+    int q_cap;                // rays the next level's queue holds (appends beyond it are dropped and counted)
     int* shadow_ctr;
     uint8_t* levels;
     int32_t* hit_ids;         // level 0, optional
+    // ray trees (scenes with transparent materials): the hit index of every
+    // ray of this level (level 0: per pixel; -1: no hit), and per hit the
+    // Fresnel weight and the queue slots of its children
+    int32_t* rayhit;
+    float4* aux;              // (K, reflection child, Fresnel child, transparent)
 };
+
+// src/RayTracer.cpp:52-65 (Bram de Greve's Fresnel reflectance): 1 on total
+// internal reflection, else the mean of the squared s and p amplitudes.
+__device__ __forceinline__ float fresnel(F3 incident, F3 normal, float n1, float n2) {
+    const float n = n1 / n2;
+    const float cosI = -dot3(normal, incident);
+    const float sinT2 = n * n * (1.0f - cosI * cosI);
+    if (sinT2 > 1.0f) return 1.0f;
+    const float cosT = sqrtf(1.0f - sinT2);
+    const float rOrth = (n1 * cosI - n2 * cosT) / (n1 * cosI + n2 * cosT);
+    const float rPar = (n2 * cosI - n1 * cosT) / (n2 * cosI + n1 * cosT);
+    return (rOrth * rOrth + rPar * rPar) / 2.0f;
+}
 
 // src/Triangle.cpp:41-47 and src/BVH.cpp:362-381: point o + t*d and normal
 // normalize(n1*(1-u-v) + n2*u + n3*v) in the triangle's local space (the ray
@@ -141,12 +160,52 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
             a.surf_out[2 * (size_t)j + 1] = make_float4(N.x, N.y, N.z, __int_as_float(pack_mat(mat, scene)));
         }
         const int k = wave_append(a.q_out_ctr, refl);
-        if (refl) {
+        if (refl && k < a.q_cap) {
             // src/RayTracer.cpp:350-355
             const F3 R = normalize3(reflect3(d, N));
             const F3 ro = add(P, scale(1e-3f, R));
             a.q_out[2 * (size_t)k] = make_float4(ro.x, ro.y, ro.z, __int_as_float(pixel));
             a.q_out[2 * (size_t)k + 1] = make_float4(R.x, R.y, R.z, __int_as_float(scene));
+        }
+        if (a.rayhit) {
+            // Ray trees.  src/RayTracer.cpp:361-408: a transparent material's
+            // second ray reflects d about the normal facing the ray (0 - N when
+            // the ray leaves the material) from P + 1e-3 * that normal, with
+            // K = fresnel(normalize(d), normal, n1, n2); the reference traces
+            // this ray twice (:401-403), its colour t enters as K*t + (1-K)*t.
+            if (L0) {
+                if (pix_ok) a.rayhit[pixel] = hit ? j : -1;
+            } else if (valid) {
+                a.rayhit[i] = hit ? j : -1;
+            }
+            const bool transp = hit && mats[mat].transparent != 0;
+            float K = 0.0f;
+            F3 fo = {0, 0, 0}, fd = {0, 0, 0};
+            if (transp) {
+                const float amb = BATCH ? a.table[scene].sc.ambient : a.sc.ambient;
+                const float eta = mats[mat].refractive_index;
+                const F3 inc = normalize3(d);
+                F3 nn = N;
+                float n1 = eta, n2 = amb;
+                if (dot3(inc, N) < 0.0f) {
+                    n1 = amb;
+                    n2 = eta;
+                } else {
+                    nn = f3(0.0f - N.x, 0.0f - N.y, 0.0f - N.z);
+                }
+                K = fresnel(inc, nn, n1, n2);
+                fd = normalize3(reflect3(d, nn));
+                fo = add(P, scale(1e-3f, nn));
+            }
+            const bool child = transp && a.level < a.depth;
+            const int kt = wave_append(a.q_out_ctr, child);
+            if (child && kt < a.q_cap) {
+                a.q_out[2 * (size_t)kt] = make_float4(fo.x, fo.y, fo.z, __int_as_float(pixel));
+                a.q_out[2 * (size_t)kt + 1] = make_float4(fd.x, fd.y, fd.z, __int_as_float(scene));
+            }
+            if (hit)
+                a.aux[j] = make_float4(K, __int_as_float(refl && k < a.q_cap ? k : -1),
+                                       __int_as_float(child && kt < a.q_cap ? kt : -1), __int_as_float(transp ? 1 : 0));
         }
     }
 }
@@ -180,6 +239,7 @@ struct PhongArgs {
     const float4* surf;
     const uint8_t* vis;
     float4* levcol;
+    float4* hitcol;           // ray trees: colour per hit of this level instead of per pixel
 };
 
 template <bool BATCH>
@@ -203,7 +263,8 @@ __global__ __launch_bounds__(256) void k_phong(PhongArgs a) {
             for (int k = 0; k < 3; ++k) acc[k] = acc[k] + c[k];
         }
         const float w = m.reflective ? m.reflectivity : 0.0f;
-        a.levcol[(size_t)a.level * a.npix + pixel] = make_float4(acc[0], acc[1], acc[2], w);
+        if (a.hitcol) a.hitcol[j] = make_float4(acc[0], acc[1], acc[2], w);
+        else a.levcol[(size_t)a.level * a.npix + pixel] = make_float4(acc[0], acc[1], acc[2], w);
     }
 }
 
@@ -226,10 +287,34 @@ struct ResolveArgs {
     float inv_n2;             // (1/n)^2, the final scale (AA_LAST)
 };
 
-// src/RayTracer.cpp:214-242: a pixel's n x n sample colours are added in
-// (q, p) order to a zero vec4, and the sum is scaled by (1/n)^2 before it is
-// quantised.  One sample per frame: the first stores 0 + c, the next ones add
-// c, the last adds c, scales and writes the pixel.
+// A pixel's final colour: quantised and written, or, with anti-aliasing
+// (src/RayTracer.cpp:214-242), one sample of n x n added in (q, p) order to a
+// zero vec4 whose sum is scaled by (1/n)^2 before it is quantised.  One sample
+// per frame: the first stores 0 + c, the next ones add c, the last adds c,
+// scales and writes the pixel.
+__device__ __forceinline__ void pixel_out(const ResolveArgs& a, int p, float c0, float c1, float c2) {
+    if (a.aa_mode != AA_NONE) {
+        const float4 s = a.aa_mode == AA_FIRST ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : a.acc[p];
+        c0 = s.x + c0;
+        c1 = s.y + c1;
+        c2 = s.z + c2;
+        if (a.aa_mode != AA_LAST) {
+            a.acc[p] = make_float4(c0, c1, c2, 0.0f);
+            return;
+        }
+        c0 = c0 * a.inv_n2;
+        c1 = c1 * a.inv_n2;
+        c2 = c2 * a.inv_n2;
+    }
+    if (a.rgba) a.rgba[p] = make_uchar4(quantize(c0), quantize(c1), quantize(c2), 255);
+    if (a.rgb) {
+        a.rgb[3 * (size_t)p] = c0;
+        a.rgb[3 * (size_t)p + 1] = c1;
+        a.rgb[3 * (size_t)p + 2] = c2;
+    }
+}
+
+// Chains of reflections: c_l = P_l + r_l * c_{l+1}, innermost first.
 __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.npix; p += gridDim.x * blockDim.x) {
         const int nl = a.levels[p];
@@ -240,25 +325,71 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
             c1 = col.y + col.w * c1;
             c2 = col.z + col.w * c2;
         }
-        if (a.aa_mode != AA_NONE) {
-            const float4 s = a.aa_mode == AA_FIRST ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : a.acc[p];
-            c0 = s.x + c0;
-            c1 = s.y + c1;
-            c2 = s.z + c2;
-            if (a.aa_mode != AA_LAST) {
-                a.acc[p] = make_float4(c0, c1, c2, 0.0f);
-                continue;
+        pixel_out(a, p, c0, c1, c2);
+    }
+}
+
+// Ray trees, one launch per level from the deepest up: a hit's colour is
+// 0 + ((phong + r * c_reflect) + (0 + (K * t + (1 - K) * t))) with c_reflect
+// and t the resolved colours of its children (0 for none), in place.
+__global__ __launch_bounds__(256) void k_resolve_tree(const int* count_ptr, float4* hitcol, const float4* aux,
+                                                      const int32_t* rayhit_next, const float4* hitcol_next) {
+    const int n = *count_ptr;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const float4 h = hitcol[j], x = aux[j];
+        const int kr = __float_as_int(x.y), kt = __float_as_int(x.z);
+        float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+        if (kr >= 0) {
+            const int jj = rayhit_next[kr];
+            if (jj >= 0) {
+                const float4 c = hitcol_next[jj];
+                r0 = c.x;
+                r1 = c.y;
+                r2 = c.z;
             }
-            c0 = c0 * a.inv_n2;
-            c1 = c1 * a.inv_n2;
-            c2 = c2 * a.inv_n2;
         }
-        if (a.rgba) a.rgba[p] = make_uchar4(quantize(c0), quantize(c1), quantize(c2), 255);
-        if (a.rgb) {
-            a.rgb[3 * (size_t)p] = c0;
-            a.rgb[3 * (size_t)p + 1] = c1;
-            a.rgb[3 * (size_t)p + 2] = c2;
+        if (kt >= 0) {
+            const int jj = rayhit_next[kt];
+            if (jj >= 0) {
+                const float4 c = hitcol_next[jj];
+                t0 = c.x;
+                t1 = c.y;
+                t2 = c.z;
+            }
         }
+        float c0 = h.x + h.w * r0, c1 = h.y + h.w * r1, c2 = h.z + h.w * r2;
+        if (__float_as_int(x.w)) {
+            const float K = x.x, L = 1.0f - K;
+            c0 = c0 + (0.0f + (K * t0 + L * t0));
+            c1 = c1 + (0.0f + (K * t1 + L * t1));
+            c2 = c2 + (0.0f + (K * t2 + L * t2));
+        }
+        hitcol[j] = make_float4(c0, c1, c2, h.w);
+    }
+}
+
+// Ray trees: the pixel's colour is its primary hit's resolved colour.
+__global__ __launch_bounds__(256) void k_resolve_pixels(ResolveArgs a, const int32_t* pixhit, const float4* hitcol) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.npix; p += gridDim.x * blockDim.x) {
+        const int j = pixhit[p];
+        float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+        if (j >= 0) {
+            const float4 c = hitcol[j];
+            c0 = c.x;
+            c1 = c.y;
+            c2 = c.z;
+        }
+        pixel_out(a, p, c0, c1, c2);
+    }
+}
+
+
+// A queue counter past its capacity: clamp it (the appends beyond were
+// dropped) and flag the frame.
+__global__ void k_clamp_count(int* ctr, int cap, int* err) {
+    if (threadIdx.x == 0 && *ctr > cap) {
+        *ctr = cap;
+        atomicOr(err, 4);
     }
 }
 
@@ -335,6 +466,7 @@ struct Frame {
     int lstride = 0;       // light slots per hit (max lights over the batch)
     int width = 4;         // tree width walked by the traversal
     bool xf = false;       // some triangle has a transformation
+    bool tree = false;     // some material is transparent: rays may have two children
     AaSample aa;           // anti-aliasing sample of this frame (mode AA_NONE: plain frame)
     bool count = false;
     LaunchCfg cfg;
@@ -355,7 +487,7 @@ struct FrameEvents {
     // per level l: [2l] before, [2l+1] after the launch
     int c[2 * EOS_MAX_DEPTH + 2];  // closest-hit trace (main stream)
     int u[2 * EOS_MAX_DEPTH + 2];  // hit setup (main stream)
-    int s[4];                      // shadow trace: level 0, levels 1..depth
+    int s[2 * EOS_MAX_DEPTH + 2];  // shadow trace: chains level 0, levels 1..depth; trees per level
     int h[2 * EOS_MAX_DEPTH + 2];  // Phong (second stream)
 };
 
@@ -371,16 +503,34 @@ struct FrameStreams {
     hipEvent_t* sync = nullptr;     // [2]: fork, join
 };
 
+// Largest queue of a ray tree level (1 << 26 rays, or a frame's pixels if more).
+constexpr size_t kTreeCap = (size_t)1 << 26;
+
 eos_status ensure_buffers(eos_frame_buffers& fb, const Frame& f, int trace_grid) {
-    const size_t cap = f.npix;
+    size_t cap = f.npix;
+    fb.tree = f.tree;
+    if (f.tree) {
+        // level l holds at most twice the rays of level l - 1 (two children per hit)
+        const size_t lim = std::max(f.npix, kTreeCap);
+        fb.toff[0] = 0;
+        for (int l = 0; l <= f.depth; ++l) {
+            fb.tcap[l] = l == 0 ? f.npix : std::min(2 * fb.tcap[l - 1], lim);
+            fb.toff[l + 1] = fb.toff[l] + fb.tcap[l];
+            cap = std::max(cap, fb.tcap[l]);
+        }
+        EOS_TRY(fb.rayhit.ensure(4 * fb.toff[f.depth + 1]));
+        EOS_TRY(fb.aux.ensure(16 * fb.toff[f.depth + 1]));
+        EOS_TRY(fb.hitcol.ensure(16 * fb.toff[f.depth + 1]));
+    }
+    const int surf_levels = f.tree ? 1 : f.depth + 1;   // trees shade one level at a time
     EOS_TRY(fb.hit0.ensure(16 * f.n_ids));
     EOS_TRY(fb.q[0].ensure(32 * cap));
     EOS_TRY(fb.q[1].ensure(32 * cap));
     EOS_TRY(fb.hitq.ensure(16 * cap));
     fb.cap = cap;
     fb.lslots = std::max(f.lstride, 1);
-    EOS_TRY(fb.surf.ensure(16 * fb.surf_stride() * (size_t)(f.depth + 1)));
-    EOS_TRY(fb.vis.ensure(fb.vis_stride() * (size_t)(f.depth + 1)));
+    EOS_TRY(fb.surf.ensure(16 * fb.surf_stride() * (size_t)surf_levels));
+    EOS_TRY(fb.vis.ensure(fb.vis_stride() * (size_t)surf_levels));
     EOS_TRY(fb.levcol.ensure(16 * f.npix * (size_t)(f.depth + 1)));
     EOS_TRY(fb.levels.ensure(f.npix));
     EOS_TRY(fb.counters.ensure(4 * (size_t)kCounterInts));
@@ -434,8 +584,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             sh.count_ptr = ctr + CTR_HITS + l0;
             sh.count_mul = f.lstride;
             sh.nlev = l1 - l0 + 1;
-            sh.surf = fb.surf_at(l0);
-            sh.vis_out = fb.vis_at(l0);
+            sh.surf = fb.surf_at(f.tree ? 0 : l0);
+            sh.vis_out = fb.vis_at(f.tree ? 0 : l0);
             sh.surf_stride = (long long)fb.surf_stride();
             sh.vis_stride = (long long)fb.vis_stride();
             EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, st, sh));
@@ -450,9 +600,10 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             pa.npix = (int)f.npix;
             pa.lstride = f.lstride;
             pa.count_ptr = ctr + CTR_HITS + l;
-            pa.surf = fb.surf_at(l);
-            pa.vis = fb.vis_at(l);
+            pa.surf = fb.surf_at(f.tree ? 0 : l);
+            pa.vis = fb.vis_at(f.tree ? 0 : l);
             pa.levcol = fb.levcol.as<float4>();
+            pa.hitcol = f.tree ? fb.hitcol.as<float4>() + fb.toff[l] : nullptr;
             F.h[2 * l] = rec(st);
             if (f.batch) hipLaunchKernelGGL(k_phong<true>, dim3(shade_grid), dim3(256), 0, st, pa);
             else hipLaunchKernelGGL(k_phong<false>, dim3(shade_grid), dim3(256), 0, st, pa);
@@ -507,10 +658,15 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         sa.count_ptr = ctr + CTR_Q + l;
         sa.q_in = qin;
         sa.hit_in = l == 0 ? fb.hit0.as<float4>() : fb.hitq.as<float4>();
-        sa.surf_out = fb.surf_at(l);
+        sa.surf_out = fb.surf_at(f.tree ? 0 : l);
         sa.hits_ctr = ctr + CTR_HITS + l;
         sa.q_out = qout;
         sa.q_out_ctr = ctr + CTR_Q + l + 1;
+        sa.q_cap = (int)(f.tree ? (l < f.depth ? fb.tcap[l + 1] : 0) : fb.cap);
+        if (f.tree) {
+            sa.rayhit = fb.rayhit.as<int32_t>() + fb.toff[l];
+            sa.aux = fb.aux.as<float4>() + fb.toff[l];
+        }
         sa.shadow_ctr = ctr + CTR_SHADOW;
         sa.levels = fb.levels.as<uint8_t>();
         sa.hit_ids = l == 0 ? f.hits : nullptr;
@@ -525,7 +681,13 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             else hipLaunchKernelGGL((k_setup<false, false>), dim3(shade_grid), dim3(256), 0, S, sa);
         }
         F.u[2 * l + 1] = rec(S);
-        if (l == 0) {
+        if (f.tree) {
+            // ray trees: one level's surfaces at a time, shaded before the next level is set up
+            if (l < f.depth)
+                hipLaunchKernelGGL(k_clamp_count, dim3(1), dim3(64), 0, S, ctr + CTR_Q + l + 1, (int)fb.tcap[l + 1],
+                                   ctr + CTR_ERR);
+            EOS_TRY(shadow_and_phong(l, l, S, 0, l));
+        } else if (l == 0) {
             // level 0's shadow rays and shading overlap the reflection chain
             if (two) {
                 EOS_HIP(hipEventRecord(fs.sync[0], S));
@@ -535,15 +697,26 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         }
     }
     // deeper levels' shadow rays in one launch once the last level is set up
-    if (f.depth >= 1) EOS_TRY(shadow_and_phong(1, f.depth, S, 0, 1));
-    if (two) {
+    if (!f.tree && f.depth >= 1) EOS_TRY(shadow_and_phong(1, f.depth, S, 0, 1));
+    if (two && !f.tree) {
         EOS_HIP(hipEventRecord(fs.sync[1], S2));
         EOS_HIP(hipStreamWaitEvent(S, fs.sync[1], 0));
     }
     ResolveArgs ra{(int)f.npix, fb.levels.as<uint8_t>(), fb.levcol.as<float4>(), f.rgba, f.rgb,
                    f.aa.acc, f.aa.mode, f.aa.inv_n2};
     F.resolve0 = rec(S);
-    hipLaunchKernelGGL(k_resolve, dim3(blocks_for(f.npix, 256, shade_grid)), dim3(256), 0, S, ra);
+    if (f.tree) {
+        float4* hc = fb.hitcol.as<float4>();
+        const float4* ax = fb.aux.as<const float4>();
+        const int32_t* rh = fb.rayhit.as<const int32_t>();
+        for (int l = f.depth; l >= 0; --l)
+            hipLaunchKernelGGL(k_resolve_tree, dim3(blocks_for(fb.tcap[l], 256, shade_grid)), dim3(256), 0, S,
+                               ctr + CTR_HITS + l, hc + fb.toff[l], ax + fb.toff[l], rh + fb.toff[l + 1],
+                               hc + fb.toff[l + 1]);
+        hipLaunchKernelGGL(k_resolve_pixels, dim3(blocks_for(f.npix, 256, shade_grid)), dim3(256), 0, S, ra, rh, hc);
+    } else {
+        hipLaunchKernelGGL(k_resolve, dim3(blocks_for(f.npix, 256, shade_grid)), dim3(256), 0, S, ra);
+    }
     F.end = rec(S);
     EOS_HIP(hipGetLastError());
     return EOS_OK;
@@ -561,6 +734,8 @@ eos_status finish_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hi
     EOS_HIP(hipStreamSynchronize(S));
     if (hctr[CTR_ERR] & 1) return fail(EOS_ERR_LIMIT, "traversal stack overflow (tree deeper than 64)");
     if (hctr[CTR_ERR] & 2) return fail(EOS_ERR_STATE, "traversal exceeded its visit bound (corrupt BVH)");
+    if (hctr[CTR_ERR] & 4)
+        return fail(EOS_ERR_LIMIT, "ray tree larger than the queue capacity (2^26 rays per level): lower the depth");
     if (!st) return EOS_OK;
     std::memset(st, 0, sizeof *st);
     st->primary_rays = (int64_t)f.npix;
@@ -582,7 +757,7 @@ eos_status finish_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hi
             st->shade_ms += c + d2;
             if (l == 0) st->trace_first_ms = a;
         }
-        for (int k = 0; k < (f.depth >= 1 ? 2 : 1); ++k) {
+        for (int k = 0; k < (f.tree ? f.depth + 1 : (f.depth >= 1 ? 2 : 1)); ++k) {
             float b = 0.f;
             (void)hipEventElapsedTime(&b, E[F->s[2 * k]], E[F->s[2 * k + 1]]);
             st->trace_shadow_ms += b;
@@ -592,7 +767,7 @@ eos_status finish_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hi
         st->shade_ms += r;
     }
     st->trace_closest_launches = f.depth + 1;
-    st->trace_shadow_launches = f.lstride > 0 ? (f.depth >= 1 ? 2 : 1) : 0;
+    st->trace_shadow_launches = f.lstride > 0 ? (f.tree ? f.depth + 1 : (f.depth >= 1 ? 2 : 1)) : 0;
     if (f.count) {
         for (int l = 0; l <= f.depth; ++l) {
             st->node_visits += (int64_t)(hstats[2 * l] + hstats[2 * (16 + l)]);
@@ -623,6 +798,7 @@ eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count
     f.cfg = s->cfg;
     f.width = s->width;
     f.xf = s->n_xf > 0;
+    f.tree = s->any_transparent;
     if (count || f.width == 2 || f.xf) {
         f.cfg.workgroup = 256;
         f.cfg.stack_lds = 16;
@@ -809,7 +985,10 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
         f.lstride = lmax;
         f.cfg = cfg;
         f.width = s0->width;
-        for (int i = 0; i < n; ++i) f.xf |= scenes[i]->n_xf > 0;
+        for (int i = 0; i < n; ++i) {
+            f.xf |= scenes[i]->n_xf > 0;
+            f.tree |= scenes[i]->any_transparent;
+        }
         if (f.width == 2 || f.xf) {
             f.cfg.workgroup = 256;
             f.cfg.stack_lds = 16;

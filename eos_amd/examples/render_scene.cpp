@@ -58,7 +58,8 @@ int main(int argc, char** argv) {
     for (const eos_material& m : mats) {
         Material mm(0.0f, eos::vec4(m.kd[0], m.kd[1], m.kd[2], m.kd[3]), eos::vec4(m.ks[0], m.ks[1], m.ks[2], m.ks[3]),
                     (int)m.shininess);
-        mm.setTransparent(false);
+        mm.setTransparent(m.transparent != 0);
+        mm.setRefractiveIndex(m.refractive_index);
         mm.setReflective(m.reflective != 0);
         mm.setReflectionIntensity(m.reflectivity);
         matIndex.push_back(scene.addMaterial(mm));

@@ -172,6 +172,12 @@ eos_status eos_build_lbvh_ex(eos_scene* scene, const eos_build_opts* opts, eos_b
 eos_status eos_set_transforms(eos_scene* scene, int32_t num_transforms, const float* transforms,
                               const int32_t* transform_index, int32_t index_memory);
 
+/* Scene::setAmbientRefractiveIndex (src/Scene.cpp:134-136): the refractive
+ * index outside transparent materials, n1 or n2 of their Fresnel term
+ * (src/RayTracer.cpp:372-384).  Default 1; the reference driver uses 1.0002
+ * (REFRACTIVE_INDEX_AIR, src/Material.cpp:25). */
+eos_status eos_set_ambient_refractive_index(eos_scene* scene, float index);
+
 /* Parameter updates without a rebuild (materials / lights only). */
 eos_status eos_update_materials(eos_scene* scene, const eos_material* materials, int32_t n);
 eos_status eos_update_lights(eos_scene* scene, const eos_light* lights, int32_t n);

@@ -27,10 +27,20 @@ def _write_scene(path, s):
                         s.camera["up"]]).astype(np.float32).tofile(f)
 
 
-@pytest.mark.parametrize("name", ["config_a", "soup"])
+def _shim_scene(name):
+    if name == "config_a":
+        return scenes.config_a(80, 60)
+    if name == "soup":
+        return scenes.config_b(72, 48, n=8000)
+    s = scenes.glass(80, 60, level=2)           # transparent materials through Material::setTransparent
+    s.ambient_index = 1.0                        # the C++ Scene's default
+    return s
+
+
+@pytest.mark.parametrize("name", ["config_a", "soup", "glass"])
 def test_cpp_api_renders_like_the_c_abi(tmp_path, name):
     from eos_amd import _lib as E
-    s = scenes.config_a(80, 60) if name == "config_a" else scenes.config_b(72, 48, n=8000)
+    s = _shim_scene(name)
     sc = tmp_path / "scene.bin"
     _write_scene(sc, s)
     o1, o2 = tmp_path / "multi.rgba", tmp_path / "single.rgba"
