@@ -7,7 +7,10 @@ is one full frame (primary rays, one shadow ray per hit and light, reflection
 rays up to depth 3, shading, quantisation) with the scene and its LBVH
 resident in HBM and the RGBA8 output written to device memory.
 
-value = (primary + shadow + secondary rays per frame) * steps / wall time.
+value = (primary + shadow + secondary rays per frame) * steps / wall time, the
+steps rendered by eos_render_frames (two frames in flight: a frame's primary
+rays trace while the previous frame's shadow and reflection launches finish);
+single_frame reports one eos_render call per frame (latency).
 
 Also reported:
   roofline      dominant kernel (the primary-ray closest-hit launch): algorithmic
@@ -17,6 +20,10 @@ Also reported:
                 by the launch's mean duration from HIP events on its stream.
   cpu_baseline  the timing-faithful port of the reference's OpenMP CPU path on a
                 stratified pixel sample of the same frame.
+  aa            the config C frame with 2x2 anti-aliasing samples per pixel
+                (renderWithAA), Mrays/s.
+  glass         ray trees: transparent, transparent-and-reflective and mirror
+                spheres (3 x 81,920 triangles) over a plane, 3840x2160, depth 3.
   instanced     config F: one 65,536-triangle mesh placed 64 times through
                 per-triangle transformations (4,194,304 instanced triangles),
                 3840x2160, depth 2, against the same triangles moved to world
@@ -187,6 +194,40 @@ def bench_instanced(E, torch, reps: int):
     return res
 
 
+def _frames(g, s, out, reps, **kw):
+    """Mean wall time and stats of `reps` frames after one warm-up frame."""
+    import torch
+    g.render_device(s.camera, s.depth, out.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    per = [g.render_device(s.camera, s.depth, out.data_ptr(), **kw) for _ in range(reps)]
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps, per[-1]
+
+
+def bench_aa(E, torch, g, scene, reps: int):
+    """The headline frame with 2x2 samples per pixel (four sample frames)."""
+    out = torch.empty(scene.width * scene.height * 4, dtype=torch.uint8, device="cuda:0")
+    wall, st = _frames(g, scene, out, reps, aa_samples=2)
+    rays = st["primary_rays"] + st["shadow_rays"] + st["secondary_rays"]
+    return {"workload": "config C, 2x2 samples per pixel", "samples": 4, "ms_per_frame": round(wall * 1e3, 3),
+            "mrays_per_s": round(rays / wall / 1e6, 2), "rays_per_frame": int(rays)}
+
+
+def bench_glass(E, torch, reps: int):
+    """Ray trees at 4K: three level-6 icospheres (glass, water-and-mirror, mirror)."""
+    from eos_amd import scenes
+    s = scenes.glass(3840, 2160, level=6, depth=3)
+    g = E.Scene(s, leaf_max=4)
+    out = torch.empty(s.width * s.height * 4, dtype=torch.uint8, device="cuda:0")
+    wall, st = _frames(g, s, out, reps)
+    rays = st["primary_rays"] + st["shadow_rays"] + st["secondary_rays"]
+    return {"workload": "glass spheres: 3 x 81,920 tris + plane, 3840x2160, 2 lights, depth 3, ray trees",
+            "triangles": s.n_tri, "ms_per_frame": round(wall * 1e3, 3), "mrays_per_s": round(rays / wall / 1e6, 2),
+            "rays_per_frame": int(rays), "secondary": int(st["secondary_rays"]), "hits": int(st["hits"]),
+            "gpu_render_ms": round(st["render_ms"], 4)}
+
+
 def bench_adjoint(E, torch, reps: int, check: bool):
     """Config E: forward frame vs forward + hand-written adjoint (records replayed)."""
     import numpy as np
@@ -271,6 +312,8 @@ def main() -> int:
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-instanced", action="store_true")
+    ap.add_argument("--skip-aa", action="store_true")
+    ap.add_argument("--skip-glass", action="store_true")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="before the frames, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for calibrating rocprofv3 byte counters)")
@@ -310,6 +353,7 @@ def main() -> int:
         _progress(f"calibration copy of {args.profile_copy_mib} MiB done")
     counted = g.render_device(cam, depth, out.data_ptr(), count_visits=True)
     _progress(f"counting frame done: {counted['render_ms']:.1f} ms")
+    # one frame per call (latency, per-launch GPU times for the roofline)
     for _ in range(args.warmup):
         g.render_device(cam, depth, out.data_ptr())
     torch.cuda.synchronize()
@@ -318,11 +362,22 @@ def main() -> int:
     for _ in range(args.steps):
         per.append(g.render_device(cam, depth, out.data_ptr()))
     torch.cuda.synchronize()
+    wall_single = time.perf_counter() - t_start
+    _progress(f"{args.steps} single frames: {wall_single * 1e3 / args.steps:.2f} ms/frame")
+    # the timed steps: the same frames with two in flight (eos_render_frames)
+    cams = np.stack([np.array(cam, copy=True)] * args.steps)
+    if args.warmup:
+        g.render_frames(cams[:args.warmup], [out.data_ptr()] * args.warmup, depth=depth)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    per_seq = g.render_frames(cams, [out.data_ptr()] * args.steps, depth=depth)
+    torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
-    _progress(f"{args.steps} timed frames: {wall * 1e3 / args.steps:.2f} ms/frame")
+    _progress(f"{args.steps} timed frames, two in flight: {wall * 1e3 / args.steps:.2f} ms/frame")
 
     st = per[-1]
     rays = st["primary_rays"] + st["shadow_rays"] + st["secondary_rays"]
+    assert all(p["primary_rays"] + p["shadow_rays"] + p["secondary_rays"] == rays for p in per_seq)
     value = rays * args.steps / wall / 1e6
     mean = lambda k: float(np.mean([p[k] for p in per]))  # noqa: E731
     t_first = mean("trace_first_ms")
@@ -362,6 +417,9 @@ def main() -> int:
                            "secondary": st["secondary_rays"], "hits": st["hits"]},
         "gpu_ms": {"render": round(mean("render_ms"), 4), "trace_closest": round(mean("trace_closest_ms"), 4),
                    "trace_shadow": round(mean("trace_shadow_ms"), 4), "shade": round(mean("shade_ms"), 4)},
+        "single_frame": {"ms_per_frame": round(wall_single / args.steps * 1e3, 4),
+                         "mrays_per_s": round(rays * args.steps / wall_single / 1e6, 3),
+                         "note": "one eos_render per frame (frame latency); value keeps two frames in flight"},
         "lbvh_build_ms": round(build["build_ms"], 3),
         "scene_gen_s": round(gen_s, 2),
         "frame_totals": {"node_visits": counted["node_visits"], "tri_tests": counted["tri_tests"]},
@@ -382,6 +440,12 @@ def main() -> int:
     if not args.skip_adjoint:
         result["adjoint"] = bench_adjoint(E, torch, reps=3, check=not args.no_cpu)
         _progress(f"adjoint: fwd+adj / fwd = {result['adjoint']['ratio']}")
+    if not args.skip_aa:
+        result["aa"] = bench_aa(E, torch, g, scene, reps=3)
+        _progress(f"aa 2x2: {result['aa']['mrays_per_s']} Mrays/s, {result['aa']['ms_per_frame']} ms")
+    if not args.skip_glass:
+        result["glass"] = bench_glass(E, torch, reps=3)
+        _progress(f"glass: {result['glass']['mrays_per_s']} Mrays/s, {result['glass']['ms_per_frame']} ms")
     if not args.skip_instanced:
         result["instanced"] = bench_instanced(E, torch, reps=5)
         _progress(f"instanced: {result['instanced']['transformed']['mrays_per_s']} Mrays/s, "

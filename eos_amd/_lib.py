@@ -22,7 +22,7 @@ EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_build
            "eos_update_lights", "eos_render", "eos_render_adjoint", "eos_render_batch",
            "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
            "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free", "eos_set_transforms",
-           "eos_set_ambient_refractive_index"]
+           "eos_set_ambient_refractive_index", "eos_render_frames"]
 
 
 class EosError(RuntimeError):
@@ -131,6 +131,7 @@ def lib():
         L.eos_mesh_free.argtypes = [vp]
         L.eos_set_transforms.argtypes = [vp, i32, vp, vp, i32]
         L.eos_set_ambient_refractive_index.argtypes = [vp, C.c_float]
+        L.eos_render_frames.argtypes = [vp, vp, i32, vp, vp, vp]
         for name in EXPORTS:
             if name != "eos_last_error" and name != "eos_abi_version":
                 getattr(L, name).restype = C.c_int
@@ -270,6 +271,18 @@ class Scene:
         st = RenderStats()
         check(lib().eos_render(self.h, cam.ctypes.data, C.byref(o), rgba_dev_ptr, C.byref(st)))
         return st.as_dict()
+
+    def render_frames(self, cameras, rgba_dev_ptrs, depth=None, count_visits=False) -> list:
+        """eos_render_frames: frame i from cameras[i] into the device buffer rgba_dev_ptrs[i]
+        (two frames in flight); per-frame stats."""
+        cams = np.ascontiguousarray(cameras)
+        n = len(cams)
+        depth = self.data.depth if depth is None else depth
+        o = RenderOpts(depth, EOS_MEM_DEVICE, None, None, 0, int(count_visits), 0)
+        ptrs = (C.c_void_p * n)(*rgba_dev_ptrs)
+        st = (RenderStats * n)()
+        check(lib().eos_render_frames(self.h, cams.ctypes.data, n, C.byref(o), ptrs, st))
+        return [s.as_dict() for s in st]
 
     def adjoint(self, dl_drgb, camera=None, depth=None, recompute=False):
         cam = np.array(self.data.camera if camera is None else camera, copy=True)

@@ -77,6 +77,16 @@ static void free_scene(eos_scene* s) {
     for (DevBuf* b : bufs) b->release();
     s->fb.surf.release();
     s->fb.vis.release();
+    for (eos_frame_buffers* fb : {&s->fb, &s->fb1})
+        for (DevBuf* b : {&fb->hit0, &fb->q[0], &fb->q[1], &fb->hitq, &fb->levcol, &fb->levels, &fb->counters,
+                          &fb->gstack, &fb->aa_acc, &fb->surf, &fb->vis, &fb->rayhit, &fb->aux, &fb->hitcol})
+            b->release();
+    for (hipEvent_t e : {s->sync1[0], s->sync1[1], s->resolved[0], s->resolved[1], s->seq_start, s->seq_ev[0],
+                         s->seq_ev[1]})
+        if (e) (void)hipEventDestroy(e);
+    if (s->stream4) (void)hipStreamDestroy(s->stream4);
+    if (s->stream3) (void)hipStreamDestroy(s->stream3);
+    if (s->host_ctr) (void)hipHostFree(s->host_ctr);
     for (hipEvent_t& e : s->ev)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t& e : s->sync)
@@ -338,6 +348,25 @@ eos_status eos_render(eos_scene* s, const eos_camera* cam, const eos_render_opts
         return fail(EOS_ERR_INVALID, "eos_render: keep_records needs chains of reflections (no transparent materials)");
     EOS_HIP(hipSetDevice(s->device));
     return render(s, cam, o, rgba, st);
+}
+
+eos_status eos_render_frames(eos_scene* s, const eos_camera* cams, int32_t n, const eos_render_opts* o,
+                             uint8_t* const* rgba, eos_render_stats* st) {
+    if (!s || !cams || !o || !rgba || n < 0) return fail(EOS_ERR_INVALID, "eos_render_frames: null argument");
+    if (!s->built) return fail(EOS_ERR_STATE, "eos_render_frames: call eos_build_lbvh first");
+    if (o->depth < 0 || o->depth > EOS_MAX_DEPTH)
+        return fail(EOS_ERR_INVALID, "eos_render_frames: depth must be in [0, 8]");
+    if (o->output_memory != EOS_MEM_DEVICE || o->rgb_out || o->hit_out || o->keep_records || o->aa_samples > 1)
+        return fail(EOS_ERR_INVALID, "eos_render_frames: RGBA8 device outputs only (no rgb_out, hit_out, "
+                                     "keep_records or anti-aliasing)");
+    for (int i = 0; i < n; ++i) {
+        if (!rgba[i]) return fail(EOS_ERR_INVALID, "eos_render_frames: null output");
+        if (cams[i].width <= 0 || cams[i].height <= 0 || (int64_t)cams[i].width * cams[i].height > (int64_t)1 << 30)
+            return fail(EOS_ERR_INVALID, "eos_render_frames: bad image size");
+    }
+    if (n == 0) return EOS_OK;
+    EOS_HIP(hipSetDevice(s->device));
+    return render_frames(s, cams, n, o, rgba, st);
 }
 
 eos_status eos_render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint_opts* o, const float* dl,

@@ -192,6 +192,14 @@ struct eos_scene {
     int num_cu = 256;
     eos_frame_buffers fb;
     hipEvent_t ev[96] = {};  // frame timing events
+    // Second frame slot of eos_render_frames (two frames in flight): frame
+    // buffers, two streams at default priority, fork/join events; the
+    // sequence's ordering and timing events; pinned counter copies.
+    eos_frame_buffers fb1;
+    hipStream_t stream3 = nullptr, stream4 = nullptr;
+    hipEvent_t sync1[2] = {}, resolved[2] = {}, seq_start = nullptr, seq_ev[2] = {};
+    void* host_ctr = nullptr;
+    size_t host_ctr_bytes = 0;
 };
 
 namespace eos {
@@ -204,6 +212,8 @@ eos_status render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint
                           double* gm, double* gl, eos_adjoint_stats* st);
 eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n, const eos_batch_opts* o,
                         uint8_t* const* out, eos_batch_stats* st);
+eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_render_opts* o, uint8_t* const* rgba,
+                         eos_render_stats* st);
 SceneView view_of(const eos_scene* s);
 // One anti-aliasing sample frame: offset inside the pixel and what the resolve
 // does with the sample's colour (src/RayTracer.cpp:214-242).

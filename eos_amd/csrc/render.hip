@@ -468,6 +468,8 @@ struct Frame {
     bool xf = false;       // some triangle has a transformation
     bool tree = false;     // some material is transparent: rays may have two children
     AaSample aa;           // anti-aliasing sample of this frame (mode AA_NONE: plain frame)
+    hipEvent_t resolve_after = nullptr;   // frames in flight: resolve after this event (the previous frame's)
+    hipEvent_t resolve_done = nullptr;    // and record this one after it
     bool count = false;
     LaunchCfg cfg;
     int num_cu = 256;
@@ -704,6 +706,7 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
     }
     ResolveArgs ra{(int)f.npix, fb.levels.as<uint8_t>(), fb.levcol.as<float4>(), f.rgba, f.rgb,
                    f.aa.acc, f.aa.mode, f.aa.inv_n2};
+    if (f.resolve_after) EOS_HIP(hipStreamWaitEvent(S, f.resolve_after, 0));
     F.resolve0 = rec(S);
     if (f.tree) {
         float4* hc = fb.hitcol.as<float4>();
@@ -717,12 +720,20 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
     } else {
         hipLaunchKernelGGL(k_resolve, dim3(blocks_for(f.npix, 256, shade_grid)), dim3(256), 0, S, ra);
     }
+    if (f.resolve_done) EOS_HIP(hipEventRecord(f.resolve_done, S));
     F.end = rec(S);
     EOS_HIP(hipGetLastError());
     return EOS_OK;
 }
 
 // Reads the frame's counters (synchronises the stream) into stats.
+// Counters of a frame as the host reads them: int counters, then the 64-bit
+// visit / test statistics (the device layout of the counter block's start).
+constexpr size_t kCounterBytes = 4 * CTR_COUNT + 8 * kStats;
+
+eos_status frame_stats(const Frame& f, const int* hctr, const unsigned long long* hstats, hipEvent_t* E,
+                       const FrameEvents* F, eos_render_stats* st);
+
 eos_status finish_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hipEvent_t* E, const FrameEvents* F,
                         eos_render_stats* st) {
     int hctr[CTR_COUNT];
@@ -732,6 +743,11 @@ eos_status finish_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hi
     EOS_HIP(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, S));
     EOS_HIP(hipMemcpyAsync(hstats, stats, sizeof hstats, hipMemcpyDeviceToHost, S));
     EOS_HIP(hipStreamSynchronize(S));
+    return frame_stats(f, hctr, hstats, E, F, st);
+}
+
+eos_status frame_stats(const Frame& f, const int* hctr, const unsigned long long* hstats, hipEvent_t* E,
+                       const FrameEvents* F, eos_render_stats* st) {
     if (hctr[CTR_ERR] & 1) return fail(EOS_ERR_LIMIT, "traversal stack overflow (tree deeper than 64)");
     if (hctr[CTR_ERR] & 2) return fail(EOS_ERR_STATE, "traversal exceeded its visit bound (corrupt BVH)");
     if (hctr[CTR_ERR] & 4)
@@ -782,8 +798,10 @@ eos_status finish_frame(eos_frame_buffers& fb, const Frame& f, hipStream_t S, hi
 }  // namespace
 
 // Frame driver shared by eos_render and the adjoint's re-trace.
-eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev,
-                     float* rgb_dev, int32_t* hits_dev, eos_render_stats* st, const AaSample& aa) {
+namespace {
+// A single-scene frame description (camera, scene view, launch shape).
+Frame make_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev, float* rgb_dev,
+                 int32_t* hits_dev, const AaSample& aa) {
     Frame f;
     f.aa = aa;
     f.cam = *reinterpret_cast<const DevCamera*>(camp);
@@ -808,12 +826,91 @@ eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count
     f.rgba = rgba_dev;
     f.rgb = rgb_dev;
     f.hits = hits_dev;
-    const int trace_grid = std::max(1, f.num_cu * f.cfg.waves_per_cu / (f.cfg.workgroup / 64));
-    EOS_TRY(ensure_buffers(s->fb, f, trace_grid));
+    return f;
+}
+
+int trace_grid_of(const Frame& f) { return std::max(1, f.num_cu * f.cfg.waves_per_cu / (f.cfg.workgroup / 64)); }
+}  // namespace
+
+eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev,
+                     float* rgb_dev, int32_t* hits_dev, eos_render_stats* st, const AaSample& aa) {
+    const Frame f = make_frame(s, camp, depth, count, rgba_dev, rgb_dev, hits_dev, aa);
+    EOS_TRY(ensure_buffers(s->fb, f, trace_grid_of(f)));
     FrameEvents fe;
     FrameStreams fs{s->stream, s->stream2, s->sync};
     EOS_TRY(enqueue_frame(s->fb, f, fs, s->ev, &fe));
     return finish_frame(s->fb, f, s->stream, s->ev, &fe, st);
+}
+
+// Frames in flight: frame i runs on slot i % 2 (its own frame buffers and pair
+// of streams), so frame i + 1's primary rays trace while frame i's shadow and
+// reflection launches -- a few rays each, bound by the longest ray's chain of
+// steps -- still run.  Resolves are ordered (frame i's after frame i - 1's), so
+// outputs that alias end with the last frame.  Counters are copied per frame
+// to pinned host memory and read after the last frame.
+eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_render_opts* o, uint8_t* const* rgba,
+                         eos_render_stats* st) {
+    if (!s->stream3) {
+        // the second slot's streams (default priority) and the sequence's events
+        hipError_t e = hipStreamCreateWithFlags(&s->stream3, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream4, hipStreamNonBlocking);
+        for (hipEvent_t* ev : {&s->sync1[0], &s->sync1[1], &s->resolved[0], &s->resolved[1]})
+            if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+        for (hipEvent_t* ev : {&s->seq_start, &s->seq_ev[0], &s->seq_ev[1]})
+            if (e == hipSuccess) e = hipEventCreate(ev);
+        if (e != hipSuccess) return fail(EOS_ERR_HIP, std::string("eos_render_frames: ") + hipGetErrorString(e));
+    }
+    const size_t need = kCounterBytes * (size_t)n;
+    if (s->host_ctr_bytes < need) {
+        if (s->host_ctr) (void)hipHostFree(s->host_ctr);
+        s->host_ctr = nullptr;
+        s->host_ctr_bytes = 0;
+        EOS_HIP(hipHostMalloc(&s->host_ctr, need, hipHostMallocDefault));
+        s->host_ctr_bytes = need;
+    }
+    eos_frame_buffers* fbs[2] = {&s->fb, &s->fb1};
+    const FrameStreams fs[2] = {{s->stream, s->stream2, s->sync}, {s->stream3, s->stream4, s->sync1}};
+    // size both slots for every frame first (buffers only grow): no buffer is
+    // reallocated while a frame in flight uses it
+    for (int i = 0; i < n; ++i) {
+        const Frame fm = make_frame(s, &cams[i], o->depth, o->count_visits != 0, nullptr, nullptr, nullptr, AaSample());
+        for (eos_frame_buffers* b : fbs) EOS_TRY(ensure_buffers(*b, fm, trace_grid_of(fm)));
+    }
+    s->fb.rec_valid = 0;
+    EOS_HIP(hipEventRecord(s->seq_start, s->stream));
+    EOS_HIP(hipStreamWaitEvent(s->stream3, s->seq_start, 0));
+    std::vector<Frame> frames;
+    frames.reserve((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const int k = i & 1;
+        Frame f = make_frame(s, &cams[i], o->depth, o->count_visits != 0, reinterpret_cast<uchar4*>(rgba[i]),
+                             nullptr, nullptr, AaSample());
+        EOS_TRY(ensure_buffers(*fbs[k], f, trace_grid_of(f)));
+        f.resolve_after = i > 0 ? s->resolved[k ^ 1] : nullptr;
+        f.resolve_done = s->resolved[k];
+        EOS_TRY(enqueue_frame(*fbs[k], f, fs[k], nullptr, nullptr));
+        EOS_HIP(hipMemcpyAsync(static_cast<char*>(s->host_ctr) + kCounterBytes * (size_t)i, fbs[k]->counters.ptr,
+                               kCounterBytes, hipMemcpyDeviceToHost, fs[k].S));
+        frames.push_back(f);
+    }
+    EOS_HIP(hipEventRecord(s->seq_ev[0], s->stream));
+    EOS_HIP(hipEventRecord(s->seq_ev[1], s->stream3));
+    EOS_HIP(hipStreamSynchronize(s->stream));
+    EOS_HIP(hipStreamSynchronize(s->stream3));
+    float ms0 = 0.f, ms1 = 0.f;
+    (void)hipEventElapsedTime(&ms0, s->seq_start, s->seq_ev[0]);
+    if (n > 1) (void)hipEventElapsedTime(&ms1, s->seq_start, s->seq_ev[1]);
+    const double total = std::max(ms0, ms1);
+    for (int i = 0; i < n; ++i) {
+        const char* base = static_cast<const char*>(s->host_ctr) + kCounterBytes * (size_t)i;
+        eos_render_stats one{};
+        EOS_TRY(frame_stats(frames[(size_t)i], reinterpret_cast<const int*>(base),
+                            reinterpret_cast<const unsigned long long*>(base + 4 * CTR_COUNT), nullptr, nullptr,
+                            &one));
+        one.render_ms = total / n;
+        if (st) st[i] = one;
+    }
+    return EOS_OK;
 }
 
 eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,

@@ -194,6 +194,17 @@ eos_status eos_update_lights(eos_scene* scene, const eos_light* lights, int32_t 
 eos_status eos_render(eos_scene* scene, const eos_camera* camera, const eos_render_opts* opts,
                       uint8_t* rgba_out, eos_render_stats* stats);
 
+/* n frames of one scene, frame i from cameras[i] into rgba_out[i] (device
+ * memory, [H][W][4]), with two frames in flight: frame i + 1's primary rays
+ * trace while frame i's shadow and reflection launches finish.  Each image is
+ * the one eos_render gives for that camera.  opts: depth, count_visits;
+ * output_memory must be EOS_MEM_DEVICE, no rgb_out / hit_out / keep_records /
+ * anti-aliasing.  stats: n entries (render_ms = the sequence's GPU time / n)
+ * or NULL.  No counterpart in the reference, whose driver renders one frame
+ * per call of its viewer loop (src/main.cpp:514-606).  Synchronous. */
+eos_status eos_render_frames(eos_scene* scene, const eos_camera* cameras, int32_t n, const eos_render_opts* opts,
+                             uint8_t* const* rgba_out, eos_render_stats* stats);
+
 /* Adjoint of eos_render's pre-quantisation image with hit sets held fixed
  * (no counterpart in the reference).  dl_drgb: [H][W][3] upstream gradient.
  * Gradients are ACCUMULATED into grad_materials [M][7] (kd rgb, ks rgb,
