@@ -94,3 +94,39 @@ def test_round_trip_of_a_written_mesh(tmp_path):
     scenes.write_obj(path, s.verts, s.normals)
     v, n = E.load_obj(path)
     assert np.array_equal(v, s.verts) and np.array_equal(n, s.normals)
+
+
+REF_MODELS = "/root/reference/objmodels"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_MODELS), reason="the reference's OBJ models are not present")
+@pytest.mark.parametrize("name", ["bunny.obj", "head.obj", "monkey.obj", "grid.obj", "grid_back.obj",
+                                  "triangle_grid.obj"])
+def test_reference_models_read_exactly(name):
+    """The OBJ models the reference ships (its driver loads bunny.obj,
+    src/main.cpp:386-403): every face becomes its fan of triangles, every
+    corner the file's position, and every generated normal is a unit vector."""
+    path = os.path.join(REF_MODELS, name)
+    pos, faces = [], []
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if not t:
+                continue
+            if t[0] == "v":
+                pos.append([float(x) for x in t[1:4]])
+            elif t[0] == "f":
+                faces.append([int(c.split("/")[0]) for c in t[1:]])
+    P = np.asarray(pos, np.float32)
+    want = []
+    for c in faces:
+        idx = [i - 1 if i > 0 else len(P) + i for i in c]
+        for k in range(1, len(idx) - 1):
+            want.append(P[[idx[0], idx[k], idx[k + 1]]])
+    v, n = E.load_obj(path)
+    assert v.shape == (len(want), 3, 3)
+    assert np.array_equal(v, np.asarray(want, np.float32).reshape(-1, 3, 3))
+    ln = np.linalg.norm(n.astype(np.float64), axis=-1)
+    assert np.all((np.abs(ln - 1.0) < 1e-6) | (ln == 0.0))
+    if name == "bunny.obj":
+        assert len(v) == 69666
