@@ -90,7 +90,8 @@ class Mesh(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("workgroup", C.c_int32), ("stack_lds", C.c_int32), ("waves_per_cu", C.c_int32),
-                ("refill", C.c_int32), ("batch", C.c_int32), ("primary_packets", C.c_int32), ("coop", C.c_int32)]
+                ("refill", C.c_int32), ("batch", C.c_int32), ("primary_packets", C.c_int32), ("coop", C.c_int32),
+                ("frames_waves_per_cu", C.c_int32), ("frames_refill", C.c_int32), ("frames_batch", C.c_int32)]
 
 
 _LIB = None
@@ -232,7 +233,8 @@ class Scene:
         check(lib().eos_update_lights(self.h, _ptr(self._lights), len(self._lights)))
 
     DEFAULT_LAUNCH = {"workgroup": 256, "stack_lds": 16, "waves_per_cu": 20, "refill": 16, "batch": 64,
-                      "primary_packets": 0, "coop": 3}
+                      "primary_packets": 0, "coop": 3, "frames_waves_per_cu": 14, "frames_refill": 8,
+                      "frames_batch": 128}
 
     def set_launch_config(self, *args, **kw):
         """Positional fields in eos_launch_config order, or keywords; fields not

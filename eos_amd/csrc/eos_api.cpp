@@ -405,6 +405,11 @@ eos_status eos_set_launch_config(eos_scene* s, const eos_launch_config* c) {
     if (c->primary_packets != 0 && c->primary_packets != 1)
         return fail(EOS_ERR_INVALID, "primary_packets must be 0 or 1");
     if (c->coop < 0 || c->coop > 4) return fail(EOS_ERR_INVALID, "coop must be in [0, 4]");
+    if (c->frames_waves_per_cu < 1 || c->frames_waves_per_cu > 64)
+        return fail(EOS_ERR_INVALID, "frames_waves_per_cu must be in [1, 64]");
+    if (c->frames_refill < 1 || c->frames_refill > 64) return fail(EOS_ERR_INVALID, "frames_refill must be in [1, 64]");
+    if (c->frames_batch < 1 || c->frames_batch > 4096)
+        return fail(EOS_ERR_INVALID, "frames_batch must be in [1, 4096]");
     s->cfg.workgroup = c->workgroup;
     s->cfg.stack_lds = c->stack_lds;
     s->cfg.waves_per_cu = c->waves_per_cu;
@@ -412,6 +417,9 @@ eos_status eos_set_launch_config(eos_scene* s, const eos_launch_config* c) {
     s->cfg.batch = c->batch;
     s->cfg.packets = c->primary_packets;
     s->cfg.coop = c->coop;
+    s->cfg.frames_waves_per_cu = c->frames_waves_per_cu;
+    s->cfg.frames_refill = c->frames_refill;
+    s->cfg.frames_batch = c->frames_batch;
     return EOS_OK;
 }
 
@@ -424,6 +432,9 @@ eos_status eos_get_launch_config(const eos_scene* s, eos_launch_config* c) {
     c->batch = s->cfg.batch;
     c->primary_packets = s->cfg.packets;
     c->coop = s->cfg.coop;
+    c->frames_waves_per_cu = s->cfg.frames_waves_per_cu;
+    c->frames_refill = s->cfg.frames_refill;
+    c->frames_batch = s->cfg.frames_batch;
     return EOS_OK;
 }
 

@@ -848,8 +848,11 @@ eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count
 // steps -- still run.  Resolves are ordered (frame i's after frame i - 1's), so
 // outputs that alias end with the last frame.  Counters are copied per frame
 // to pinned host memory and read after the last frame.
-eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_render_opts* o, uint8_t* const* rgba,
-                         eos_render_stats* st) {
+namespace {
+// Runs prepared frames two in flight (slot i % 2); per-frame stats (render_ms:
+// the sequence's GPU time / frames) into st when non-null.
+eos_status run_pipelined(eos_scene* s, std::vector<Frame>& frames, eos_render_stats* st) {
+    const int n = (int)frames.size();
     if (!s->stream3) {
         // the second slot's streams (default priority) and the sequence's events
         hipError_t e = hipStreamCreateWithFlags(&s->stream3, hipStreamNonBlocking);
@@ -858,7 +861,7 @@ eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_
             if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
         for (hipEvent_t* ev : {&s->seq_start, &s->seq_ev[0], &s->seq_ev[1]})
             if (e == hipSuccess) e = hipEventCreate(ev);
-        if (e != hipSuccess) return fail(EOS_ERR_HIP, std::string("eos_render_frames: ") + hipGetErrorString(e));
+        if (e != hipSuccess) return fail(EOS_ERR_HIP, std::string("frames in flight: ") + hipGetErrorString(e));
     }
     const size_t need = kCounterBytes * (size_t)n;
     if (s->host_ctr_bytes < need) {
@@ -870,28 +873,27 @@ eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_
     }
     eos_frame_buffers* fbs[2] = {&s->fb, &s->fb1};
     const FrameStreams fs[2] = {{s->stream, s->stream2, s->sync}, {s->stream3, s->stream4, s->sync1}};
+    for (Frame& f : frames) {
+        f.cfg.waves_per_cu = f.cfg.frames_waves_per_cu;
+        f.cfg.refill = f.cfg.frames_refill;
+        f.cfg.batch = f.cfg.frames_batch;
+    }
     // size both slots for every frame first (buffers only grow): no buffer is
     // reallocated while a frame in flight uses it
-    for (int i = 0; i < n; ++i) {
-        const Frame fm = make_frame(s, &cams[i], o->depth, o->count_visits != 0, nullptr, nullptr, nullptr, AaSample());
-        for (eos_frame_buffers* b : fbs) EOS_TRY(ensure_buffers(*b, fm, trace_grid_of(fm)));
-    }
+    for (const Frame& f : frames)
+        for (eos_frame_buffers* b : fbs) EOS_TRY(ensure_buffers(*b, f, trace_grid_of(f)));
     s->fb.rec_valid = 0;
     EOS_HIP(hipEventRecord(s->seq_start, s->stream));
     EOS_HIP(hipStreamWaitEvent(s->stream3, s->seq_start, 0));
-    std::vector<Frame> frames;
-    frames.reserve((size_t)n);
     for (int i = 0; i < n; ++i) {
         const int k = i & 1;
-        Frame f = make_frame(s, &cams[i], o->depth, o->count_visits != 0, reinterpret_cast<uchar4*>(rgba[i]),
-                             nullptr, nullptr, AaSample());
+        Frame& f = frames[(size_t)i];
         EOS_TRY(ensure_buffers(*fbs[k], f, trace_grid_of(f)));
         f.resolve_after = i > 0 ? s->resolved[k ^ 1] : nullptr;
         f.resolve_done = s->resolved[k];
         EOS_TRY(enqueue_frame(*fbs[k], f, fs[k], nullptr, nullptr));
         EOS_HIP(hipMemcpyAsync(static_cast<char*>(s->host_ctr) + kCounterBytes * (size_t)i, fbs[k]->counters.ptr,
                                kCounterBytes, hipMemcpyDeviceToHost, fs[k].S));
-        frames.push_back(f);
     }
     EOS_HIP(hipEventRecord(s->seq_ev[0], s->stream));
     EOS_HIP(hipEventRecord(s->seq_ev[1], s->stream3));
@@ -911,6 +913,17 @@ eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_
         if (st) st[i] = one;
     }
     return EOS_OK;
+}
+}  // namespace
+
+eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_render_opts* o, uint8_t* const* rgba,
+                         eos_render_stats* st) {
+    std::vector<Frame> frames;
+    frames.reserve((size_t)n);
+    for (int i = 0; i < n; ++i)
+        frames.push_back(make_frame(s, &cams[i], o->depth, o->count_visits != 0, reinterpret_cast<uchar4*>(rgba[i]),
+                                    nullptr, nullptr, AaSample()));
+    return run_pipelined(s, frames, st);
 }
 
 eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
@@ -948,38 +961,41 @@ eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o,
     if (n == 1) {
         EOS_TRY(run_frame(s, cam, o->depth, o->count_visits != 0, rgba_dev, rgb_dev, hit_dev, st));
     } else {
-        // one frame per sample, in the reference's (q, p) order, summed by the resolve
+        // one frame per sample, in the reference's (q, p) order, summed by the
+        // resolves (ordered across the two slots), two sample frames in flight
         EOS_TRY(fb.aa_acc.ensure(16 * npix));
         AaSample aa;
         aa.acc = fb.aa_acc.as<float4>();
         const float inv_n = (float)(1.0 / (double)(float)n);   // `float inv = 1.0 / (float) n`
         aa.inv_n2 = inv_n * inv_n;
-        eos_render_stats sum{}, one{};
+        std::vector<Frame> frames;
         for (int q = 0; q < n; ++q)
             for (int p = 0; p < n; ++p) {
                 const int k = q * n + p;
                 aa.ox = ((float)p + 0.0f) * inv_n;
                 aa.oy = ((float)q + 0.0f) * inv_n;
                 aa.mode = k == 0 ? AA_FIRST : (k == n * n - 1 ? AA_LAST : AA_NEXT);
-                EOS_TRY(run_frame(s, cam, o->depth, o->count_visits != 0, aa.mode == AA_LAST ? rgba_dev : nullptr,
-                                  aa.mode == AA_LAST ? rgb_dev : nullptr, k == 0 ? hit_dev : nullptr, &one, aa));
-                sum.primary_rays += one.primary_rays;
-                sum.shadow_rays += one.shadow_rays;
-                sum.secondary_rays += one.secondary_rays;
-                sum.hits += one.hits;
-                sum.node_visits += one.node_visits;
-                sum.tri_tests += one.tri_tests;
-                sum.render_ms += one.render_ms;
-                sum.trace_closest_ms += one.trace_closest_ms;
-                sum.trace_shadow_ms += one.trace_shadow_ms;
-                sum.shade_ms += one.shade_ms;
-                sum.trace_closest_launches += one.trace_closest_launches;
-                sum.trace_shadow_launches += one.trace_shadow_launches;
-                sum.closest_visits_first += one.closest_visits_first;
-                sum.closest_tests_first += one.closest_tests_first;
-                sum.trace_first_ms += one.trace_first_ms;
+                frames.push_back(make_frame(s, cam, o->depth, o->count_visits != 0,
+                                            aa.mode == AA_LAST ? rgba_dev : nullptr,
+                                            aa.mode == AA_LAST ? rgb_dev : nullptr, k == 0 ? hit_dev : nullptr, aa));
             }
-        if (st) *st = sum;
+        std::vector<eos_render_stats> per(frames.size());
+        EOS_TRY(run_pipelined(s, frames, per.data()));
+        eos_render_stats sum{};
+        for (const eos_render_stats& one : per) {
+            sum.primary_rays += one.primary_rays;
+            sum.shadow_rays += one.shadow_rays;
+            sum.secondary_rays += one.secondary_rays;
+            sum.hits += one.hits;
+            sum.node_visits += one.node_visits;
+            sum.tri_tests += one.tri_tests;
+            sum.render_ms += one.render_ms;
+            sum.trace_closest_launches += one.trace_closest_launches;
+            sum.trace_shadow_launches += one.trace_shadow_launches;
+            sum.closest_visits_first += one.closest_visits_first;
+            sum.closest_tests_first += one.closest_tests_first;
+        }
+        if (st) *st = sum;   // per-launch GPU times are not split out for overlapped frames
     }
     if (!dev_out) {
         if (rgba) EOS_HIP(hipMemcpyAsync(rgba, rgba_dev, 4 * npix, hipMemcpyDeviceToHost, s->stream));

@@ -276,6 +276,11 @@ typedef struct eos_launch_config {
                                  (pays off when triangles cover many pixels; 0, the default, walks per ray) */
     int32_t coop;             /* launches walked by lane quads (four lanes per ray, 4-wide tree): 0 never;
                                  1, 2, 3: launches with at most lanes/4, lanes/2, lanes items; 4 always */
+    /* the same three knobs for frames that run two in flight (eos_render_frames,
+       anti-aliasing samples): each frame's launches share the GPU with the other's */
+    int32_t frames_waves_per_cu;
+    int32_t frames_refill;
+    int32_t frames_batch;
 } eos_launch_config;
 
 eos_status eos_set_launch_config(eos_scene* scene, const eos_launch_config* cfg);
