@@ -259,6 +259,26 @@ void DeviceRenderer::renderCudaSceneToHostBufferWarpShuffleMegaKernel(cudaScene_
     renderCudaSceneToHostBufferMegaKernel(scene, image);
 }
 
+void RayTracer::render(const Scene& scene, const Camera& camera, unsigned char* outputImage) {
+    if (!outputImage) {
+        std::fprintf(stderr, "Eos Error: RayTracer::render: null image\n");
+        std::exit(1);
+    }
+    // the scene is read, not changed (createCudaScene takes the reference's non-const pointer)
+    cudaScene_t* d = createCudaScene(const_cast<Scene*>(&scene));
+    if (!d) {
+        std::fprintf(stderr, "Eos Error: RayTracer::render: the scene does not use a BVH (Scene::useBvh(true))\n");
+        std::exit(1);
+    }
+    const eos_camera cam = to_c(camera, camera.getWidth(), camera.getHeight());
+    eos_render_opts o{};
+    o.depth = m_TraceDepth;
+    o.output_memory = EOS_MEM_HOST;
+    o.aa_samples = m_AASamples;
+    check(eos_render(d->scene, &cam, &o, outputImage, nullptr), "RayTracer::render");
+    destroyCudaScene(d);
+}
+
 MultiDeviceRenderer::MultiDeviceRenderer(Scene* hostScene, Camera* /*hostCamera*/, int width, int height)
     : m_Scene(nullptr), m_Width(width), m_Height(height) {
     int n = 0;

@@ -15,6 +15,7 @@
 // float32 camera position, view, right, up (12 floats).
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "eos/eos.h"
@@ -132,6 +133,17 @@ int main(int argc, char** argv) {
     std::fclose(o);
     handler.freeDeviceCamera();
     destroyCudaScene(dscene);
+
+    // the reference's CPU renderer API (src/main.cpp:367-371), here on the GPU:
+    // EOS_AA sets its anti-aliasing samples, the image goes to <out_single>.rt
+    RayTracer rayTracer;
+    rayTracer.setTracedDepth(depth);
+    if (const char* aa = std::getenv("EOS_AA")) rayTracer.setAASamples(std::atoi(aa));
+    std::vector<unsigned char> image3(4 * (size_t)W * H);
+    rayTracer.render(scene, camera, image3.data());
+    o = std::fopen((std::string(argv[3]) + ".rt").c_str(), "wb");
+    std::fwrite(image3.data(), 1, image3.size(), o);
+    std::fclose(o);
     for (Triangle* t : tris) delete t;
     for (LightSource* l : ls) delete l;
     return 0;

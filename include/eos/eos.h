@@ -294,6 +294,29 @@ private:
     float m_LastMs = 0.0f;
 };
 
+/* src/RayTracer.h:29-67: the reference's CPU renderer API.  render() gives the
+ * image the reference's RayTracer::render computes (renderWithoutAA when the
+ * sample count is RAYTRACER_NO_AA, else renderWithAA), rendered on the GPU:
+ * the scene is uploaded and its LBVH built on each call, then eos_render runs
+ * with the traced depth and anti-aliasing samples set here. */
+#define RAYTRACER_AA_SAMPLES_8 8
+#define RAYTRACER_AA_SAMPLES_4 4
+#define RAYTRACER_AA_SAMPLES_2 2
+#define RAYTRACER_NO_AA 0
+class RayTracer {
+public:
+    RayTracer() : m_AASamples(RAYTRACER_NO_AA), m_TraceDepth(RAYTRACER_DEFAULT_TRACE_DEPTH) {}
+    void render(const Scene& scene, const Camera& camera, unsigned char* outputImage);
+    int getAASamples() const { return m_AASamples; }
+    void setAASamples(int samples) { if (samples > 0) m_AASamples = samples; }   // src/RayTracer.cpp:33-36
+    int getTracedDepth() const { return m_TraceDepth; }
+    void setTracedDepth(int depth) { if (depth > 0) m_TraceDepth = depth; }      // src/RayTracer.cpp:47-50
+
+private:
+    int m_AASamples;
+    int m_TraceDepth;
+};
+
 class MultiDeviceRenderer {
 public:
     MultiDeviceRenderer(Scene* hostScene, Camera* hostCamera, int width, int height);

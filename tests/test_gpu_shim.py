@@ -45,14 +45,21 @@ def test_cpp_api_renders_like_the_c_abi(tmp_path, name):
     _write_scene(sc, s)
     o1, o2 = tmp_path / "multi.rgba", tmp_path / "single.rgba"
     print(f"running {DEMO} on {name}", flush=True)
+    aa = 2 if name == "config_a" else 0           # the RayTracer route with renderWithAA
     r = subprocess.run([DEMO, str(sc), str(o1), str(o2)], capture_output=True, text=True, timeout=120,
-                       env=dict(os.environ, EOS_QUIET="1"))
+                       env=dict(os.environ, EOS_QUIET="1", EOS_AA=str(aa)))
     assert r.returncode == 0, r.stderr
     multi = np.fromfile(o1, np.uint8).reshape(s.height, s.width, 4)
     single = np.fromfile(o2, np.uint8).reshape(s.height, s.width, 4)
-    ref = E.Scene(s, leaf_max=4).render()["rgba"]
+    g = E.Scene(s, leaf_max=4)
+    ref = g.render()["rgba"]
     assert np.array_equal(multi, ref)
     assert np.array_equal(single, ref)
+    rt = np.fromfile(str(o2) + ".rt", np.uint8).reshape(s.height, s.width, 4)
+    assert np.array_equal(rt, g.render(aa_samples=aa)["rgba"])
+    if aa:
+        assert np.abs(rt.astype(int) - po.Oracle(s).render_aa(aa)["rgba"].astype(int)).max(-1).__le__(1).mean() \
+            >= 0.9999
     orc = po.Oracle(s).render()["rgba"]
     assert (np.abs(multi.astype(int) - orc.astype(int)).max(-1) <= 1).mean() >= 0.9999
 
