@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Headline benchmark: Whitted frames on one MI355X, Mrays/s.
 
-Workload (the north-star frame): displaced UV sphere, 4,194,304 triangles,
+Workload (the headline frame): displaced UV sphere, 4,194,304 triangles,
 3840x2160, 4 point lights, 16 materials, reflection depth 3, FP32.  A "step"
 is one full frame (primary rays, one shadow ray per hit and light, reflection
 rays up to depth 3, shading, quantisation) with the scene and its LBVH
