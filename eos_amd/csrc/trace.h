@@ -135,7 +135,7 @@ __device__ __forceinline__ int wave_append(int* ctr, bool pred) {
     if (m == 0) return -1;
     const int leader = __ffsll((unsigned long long)m) - 1;
     int base = 0;
-    if (__lane_id() == leader) base = atomicAdd(ctr, __popcll(m));
+    if ((int)__lane_id() == leader) base = atomicAdd(ctr, __popcll(m));
     base = __shfl(base, leader);
     return pred ? base + __popcll(m & lanemask_lt()) : -1;
 }
@@ -279,7 +279,7 @@ __device__ __forceinline__ int fetch_items(Fetch& f, const TraceArgs& a, int n, 
             size = size > a.batch ? a.batch : size;
             size = size < want ? want : size;
             int got = 0;
-            if (__lane_id() == leader) got = atomicAdd(a.cursor + r * kCursorStride, size);
+            if ((int)__lane_id() == leader) got = atomicAdd(a.cursor + r * kCursorStride, size);
             got = __shfl(got, leader);
             if (got < len) {
                 f.seen = got + size;
@@ -578,7 +578,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
                     item = -1;
                 }
             }
-            if (__popcll(__ballot(sub == 0 && item < 0)) >= need) break;
+            if ((int)__popcll(__ballot(sub == 0 && item < 0)) >= need) break;
         }
     }
 }
@@ -815,7 +815,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                         item = -1;
                     }
                 }
-                if (__popcll(__ballot(item < 0)) >= need) break;
+                if ((int)__popcll(__ballot(item < 0)) >= need) break;
             }
         }
     }
