@@ -22,7 +22,7 @@ EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_build
            "eos_update_lights", "eos_render", "eos_render_adjoint", "eos_render_batch",
            "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
            "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free", "eos_set_transforms",
-           "eos_set_ambient_refractive_index", "eos_render_frames"]
+           "eos_set_ambient_refractive_index", "eos_render_frames", "eos_render_rows"]
 
 
 class EosError(RuntimeError):
@@ -133,6 +133,7 @@ def lib():
         L.eos_set_transforms.argtypes = [vp, i32, vp, vp, i32]
         L.eos_set_ambient_refractive_index.argtypes = [vp, C.c_float]
         L.eos_render_frames.argtypes = [vp, vp, i32, vp, vp, vp]
+        L.eos_render_rows.argtypes = [vp, vp, i32, i32, vp, vp, vp]
         for name in EXPORTS:
             if name != "eos_last_error" and name != "eos_abi_version":
                 getattr(L, name).restype = C.c_int
@@ -273,6 +274,20 @@ class Scene:
         st = RenderStats()
         check(lib().eos_render(self.h, cam.ctypes.data, C.byref(o), rgba_dev_ptr, C.byref(st)))
         return st.as_dict()
+
+    def render_rows(self, row_begin, row_end, camera=None, depth=None, want_rgb=False, want_hits=False,
+                    aa_samples=0):
+        """eos_render_rows: rows [row_begin, row_end) of the frame (host outputs for the band)."""
+        cam = np.array(self.data.camera if camera is None else camera, copy=True)
+        depth = self.data.depth if depth is None else depth
+        W, Hb = int(cam["width"]), max(row_end - row_begin, 0)
+        rgba = np.zeros((Hb, W, 4), np.uint8)
+        rgb = np.zeros((Hb, W, 3), np.float32) if want_rgb else None
+        hits = np.zeros((Hb, W), np.int32) if want_hits else None
+        o = RenderOpts(depth, EOS_MEM_HOST, _ptr(rgb), _ptr(hits), 0, 0, int(aa_samples))
+        st = RenderStats()
+        check(lib().eos_render_rows(self.h, cam.ctypes.data, row_begin, row_end, C.byref(o), _ptr(rgba), C.byref(st)))
+        return {"rgba": rgba, "rgb": rgb, "hit": hits, "stats": st.as_dict()}
 
     def render_frames(self, cameras, rgba_dev_ptrs, depth=None, count_visits=False) -> list:
         """eos_render_frames: frame i from cameras[i] into the device buffer rgba_dev_ptrs[i]

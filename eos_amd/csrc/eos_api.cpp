@@ -350,6 +350,23 @@ eos_status eos_render(eos_scene* s, const eos_camera* cam, const eos_render_opts
     return render(s, cam, o, rgba, st);
 }
 
+eos_status eos_render_rows(eos_scene* s, const eos_camera* cam, int32_t row_begin, int32_t row_end,
+                           const eos_render_opts* o, uint8_t* rgba, eos_render_stats* st) {
+    if (!s || !cam || !o) return fail(EOS_ERR_INVALID, "eos_render_rows: null argument");
+    if (!s->built) return fail(EOS_ERR_STATE, "eos_render_rows: call eos_build_lbvh first");
+    if (cam->width <= 0 || cam->height <= 0 || (int64_t)cam->width * cam->height > (int64_t)1 << 30)
+        return fail(EOS_ERR_INVALID, "eos_render_rows: bad image size");
+    if (row_begin < 0 || row_end > cam->height || row_begin >= row_end)
+        return fail(EOS_ERR_INVALID, "eos_render_rows: rows must satisfy 0 <= row_begin < row_end <= height");
+    if (o->depth < 0 || o->depth > EOS_MAX_DEPTH) return fail(EOS_ERR_INVALID, "eos_render_rows: depth must be in [0, 8]");
+    if (o->aa_samples < 0 || o->aa_samples > EOS_MAX_AA)
+        return fail(EOS_ERR_INVALID, "eos_render_rows: aa_samples must be in [0, 16]");
+    if (o->keep_records)
+        return fail(EOS_ERR_INVALID, "eos_render_rows: keep_records needs the whole frame (eos_render)");
+    EOS_HIP(hipSetDevice(s->device));
+    return render(s, cam, o, rgba, st, row_begin, row_end);
+}
+
 eos_status eos_render_frames(eos_scene* s, const eos_camera* cams, int32_t n, const eos_render_opts* o,
                              uint8_t* const* rgba, eos_render_stats* st) {
     if (!s || !cams || !o || !rgba || n < 0) return fail(EOS_ERR_INVALID, "eos_render_frames: null argument");

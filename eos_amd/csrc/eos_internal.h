@@ -210,8 +210,9 @@ namespace eos {
 // Implemented in lbvh.hip
 eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st);
 // Implemented in render.hip
+// Rows [row0, row1) of the camera's frame (row1 < 0: to the last row).
 eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
-                  eos_render_stats* st);
+                  eos_render_stats* st, int row0 = 0, int row1 = -1);
 eos_status render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint_opts* o, const float* dl,
                           double* gm, double* gl, eos_adjoint_stats* st);
 eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n, const eos_batch_opts* o,
@@ -230,5 +231,6 @@ struct AaSample {
 };
 // One frame of the wavefront pipeline (render.hip); outputs may be null.
 eos_status run_frame(eos_scene* s, const eos_camera* cam, int depth, bool count, uchar4* rgba_dev, float* rgb_dev,
-                     int32_t* hits_dev, eos_render_stats* st, const AaSample& aa = AaSample());
+                     int32_t* hits_dev, eos_render_stats* st, const AaSample& aa = AaSample(), int row0 = 0,
+                     int row1 = -1);
 }  // namespace eos

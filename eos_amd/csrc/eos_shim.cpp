@@ -328,21 +328,24 @@ void rayTrace_WarpShuffle_MegaKernel(cudaScene_t* scene, Camera* camera, int wid
 }
 
 // Renders the band of rows [gpuID*H/numGPUS, (gpuID+1)*H/numGPUS) into image
-// (src/MultiDeviceRenderer.cu:122-163): the full frame is traced here and the
-// band copied out.
+// (src/MultiDeviceRenderer.cu:122-163): only that band's primary rays are
+// generated, so each device traces its own share of the frame.
 void rayTrace_MultiMegaKernel(cudaScene_t* scene, Camera* camera, int width, int height, uchar4_t* image,
                               int numGPUS, int gpuID, int*, int*) {
     if (numGPUS <= 1) {
         rayTrace_MegaKernel(scene, camera, width, height, image, nullptr, nullptr);
         return;
     }
-    const size_t frame = 4 * (size_t)width * height;
-    void* tmp = nullptr;
-    check_hip(hipMalloc(&tmp, frame), "band buffer");
-    rayTrace_MegaKernel(scene, camera, width, height, static_cast<uchar4_t*>(tmp), nullptr, nullptr);
-    const size_t band = 4 * (size_t)width * (height / numGPUS);
-    check_hip(hipMemcpy(image, static_cast<char*>(tmp) + band * gpuID, band, hipMemcpyDeviceToDevice), "band copy");
-    check_hip(hipFree(tmp), "band buffer");
+    // this GPU's band of rows only (src/kernelInvocations.cu:56: row j of the
+    // band is row j + (height / numGPUS) * gpuID of the frame)
+    const int rows = height / numGPUS;
+    const eos_camera cam = to_c(download_camera(camera), width, height);
+    eos_render_opts o{};
+    o.depth = RAYTRACER_DEFAULT_TRACE_DEPTH;
+    o.output_memory = EOS_MEM_DEVICE;
+    check(eos_render_rows(scene->scene, &cam, rows * gpuID, rows * (gpuID + 1), &o, reinterpret_cast<uint8_t*>(image),
+                          nullptr),
+          "rayTrace_MultiMegaKernel");
 }
 
 void resetDevice(void) { check_hip(hipDeviceReset(), "hipDeviceReset"); }

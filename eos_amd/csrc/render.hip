@@ -30,6 +30,7 @@ struct SetupArgs {
     SceneView sc;
     DevCamera cam;
     float aa_ox, aa_oy;       // anti-aliasing sample offset inside the pixel
+    int row0, band_h;         // level 0 covers camera rows [row0, row0 + band_h)
     int tiles_x;
     const BatchScene* table;
     int n_scenes;
@@ -103,10 +104,10 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
                     mats = B.sc.mats;
                 }
             } else {
-                pix_ok = valid && tile_pixel(i, a.tiles_x, a.cam.width, a.cam.height, x, y);
+                pix_ok = valid && tile_pixel(i, a.tiles_x, a.cam.width, a.band_h, x, y);
                 pixel = y * a.cam.width + x;
                 o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-                if (pix_ok) d = camera_dir(a.cam, x, y, a.aa_ox, a.aa_oy);
+                if (pix_ok) d = camera_dir(a.cam, x, y + a.row0, a.aa_ox, a.aa_oy);
             }
         } else if (valid) {
             const float4 q0 = a.q_in[2 * (size_t)i], q1 = a.q_in[2 * (size_t)i + 1];
@@ -468,6 +469,7 @@ struct Frame {
     bool xf = false;       // some triangle has a transformation
     bool tree = false;     // some material is transparent: rays may have two children
     AaSample aa;           // anti-aliasing sample of this frame (mode AA_NONE: plain frame)
+    int row0 = 0, band_h = 0;  // single scene: camera rows [row0, row0 + band_h) of the frame
     hipEvent_t resolve_after = nullptr;   // frames in flight: resolve after this event (the previous frame's)
     hipEvent_t resolve_done = nullptr;    // and record this one after it
     bool count = false;
@@ -621,6 +623,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         ta.cam = f.cam;
         ta.aa_ox = f.aa.ox;
         ta.aa_oy = f.aa.oy;
+        ta.row0 = f.row0;
+        ta.band_h = f.band_h;
         ta.tiles_x = f.tiles_x;
         ta.table = f.table;
         ta.n_scenes = f.n_scenes;
@@ -651,6 +655,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         sa.cam = f.cam;
         sa.aa_ox = f.aa.ox;
         sa.aa_oy = f.aa.oy;
+        sa.row0 = f.row0;
+        sa.band_h = f.band_h;
         sa.tiles_x = f.tiles_x;
         sa.table = f.table;
         sa.n_scenes = f.n_scenes;
@@ -801,14 +807,16 @@ eos_status frame_stats(const Frame& f, const int* hctr, const unsigned long long
 namespace {
 // A single-scene frame description (camera, scene view, launch shape).
 Frame make_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev, float* rgb_dev,
-                 int32_t* hits_dev, const AaSample& aa) {
+                 int32_t* hits_dev, const AaSample& aa, int row0 = 0, int row1 = -1) {
     Frame f;
     f.aa = aa;
     f.cam = *reinterpret_cast<const DevCamera*>(camp);
     f.sc = view_of(s);
+    f.row0 = row0;
+    f.band_h = (row1 < 0 ? f.cam.height : row1) - row0;
     f.tiles_x = (f.cam.width + 7) / 8;
-    const int tiles_y = (f.cam.height + 7) / 8;
-    f.npix = (size_t)f.cam.width * f.cam.height;
+    const int tiles_y = (f.band_h + 7) / 8;
+    f.npix = (size_t)f.cam.width * f.band_h;
     f.n_ids = (size_t)f.tiles_x * tiles_y * 64;
     f.depth = depth;
     f.lstride = s->n_light;
@@ -833,8 +841,8 @@ int trace_grid_of(const Frame& f) { return std::max(1, f.num_cu * f.cfg.waves_pe
 }  // namespace
 
 eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev,
-                     float* rgb_dev, int32_t* hits_dev, eos_render_stats* st, const AaSample& aa) {
-    const Frame f = make_frame(s, camp, depth, count, rgba_dev, rgb_dev, hits_dev, aa);
+                     float* rgb_dev, int32_t* hits_dev, eos_render_stats* st, const AaSample& aa, int row0, int row1) {
+    const Frame f = make_frame(s, camp, depth, count, rgba_dev, rgb_dev, hits_dev, aa, row0, row1);
     EOS_TRY(ensure_buffers(s->fb, f, trace_grid_of(f)));
     FrameEvents fe;
     FrameStreams fs{s->stream, s->stream2, s->sync};
@@ -927,9 +935,10 @@ eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_
 }
 
 eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
-                  eos_render_stats* st) {
+                  eos_render_stats* st, int row0, int row1) {
     eos_frame_buffers& fb = s->fb;
-    const size_t npix = (size_t)cam->width * cam->height;
+    if (row1 < 0) row1 = cam->height;
+    const size_t npix = (size_t)cam->width * (size_t)(row1 - row0);
     const bool dev_out = o->output_memory == EOS_MEM_DEVICE;
     static thread_local DevBuf stage_rgba, stage_rgb, stage_hit;
     uchar4* rgba_dev = nullptr;
@@ -959,7 +968,8 @@ eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o,
     fb.rec_valid = 0;
     const int n = o->aa_samples > 1 ? o->aa_samples : 1;
     if (n == 1) {
-        EOS_TRY(run_frame(s, cam, o->depth, o->count_visits != 0, rgba_dev, rgb_dev, hit_dev, st));
+        EOS_TRY(run_frame(s, cam, o->depth, o->count_visits != 0, rgba_dev, rgb_dev, hit_dev, st, AaSample(), row0,
+                          row1));
     } else {
         // one frame per sample, in the reference's (q, p) order, summed by the
         // resolves (ordered across the two slots), two sample frames in flight
@@ -977,7 +987,8 @@ eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o,
                 aa.mode = k == 0 ? AA_FIRST : (k == n * n - 1 ? AA_LAST : AA_NEXT);
                 frames.push_back(make_frame(s, cam, o->depth, o->count_visits != 0,
                                             aa.mode == AA_LAST ? rgba_dev : nullptr,
-                                            aa.mode == AA_LAST ? rgb_dev : nullptr, k == 0 ? hit_dev : nullptr, aa));
+                                            aa.mode == AA_LAST ? rgb_dev : nullptr, k == 0 ? hit_dev : nullptr, aa,
+                                            row0, row1));
             }
         std::vector<eos_render_stats> per(frames.size());
         EOS_TRY(run_pipelined(s, frames, per.data()));

@@ -202,6 +202,7 @@ struct TraceArgs {
     SceneView sc;
     DevCamera cam;
     float aa_ox, aa_oy;      // anti-aliasing sample offset inside the pixel (0: pixel corner)
+    int row0, band_h;        // primary rays cover camera rows [row0, row0 + band_h)
     int tiles_x;
     const BatchScene* table;  // batched frames
     int n_scenes;
@@ -347,9 +348,9 @@ __device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& 
             r.o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
             r.d = camera_dir(B.cam, x, y, 0.0f, 0.0f);
         } else {
-            r.valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
+            r.valid = tile_pixel(my, a.tiles_x, a.cam.width, a.band_h, x, y);
             r.o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-            r.d = camera_dir(a.cam, x, y, a.aa_ox, a.aa_oy);
+            r.d = camera_dir(a.cam, x, y + a.row0, a.aa_ox, a.aa_oy);
         }
     } else if (MODE == MODE_QUEUE) {
         const float4 q0 = a.q[2 * (size_t)my], q1 = a.q[2 * (size_t)my + 1];
@@ -886,9 +887,9 @@ __global__ __launch_bounds__(WG) void k_trace_packet(TraceArgs a) {
             o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
             d = camera_dir(B.cam, x, y, 0.0f, 0.0f);
         } else {
-            valid = tile_pixel(my, a.tiles_x, a.cam.width, a.cam.height, x, y);
+            valid = tile_pixel(my, a.tiles_x, a.cam.width, a.band_h, x, y);
             o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-            d = camera_dir(a.cam, x, y, a.aa_ox, a.aa_oy);
+            d = camera_dir(a.cam, x, y + a.row0, a.aa_ox, a.aa_oy);
         }
         const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
         const float px = o.x * ix, py = o.y * iy, pz = o.z * iz;

@@ -194,6 +194,15 @@ eos_status eos_update_lights(eos_scene* scene, const eos_light* lights, int32_t 
 eos_status eos_render(eos_scene* scene, const eos_camera* camera, const eos_render_opts* opts,
                       uint8_t* rgba_out, eos_render_stats* stats);
 
+/* Rows [row_begin, row_end) of the camera's frame (row 0 at the bottom), the
+ * pixels eos_render gives there, into outputs sized for the band
+ * ([row_end - row_begin][W]...).  One GPU's share of a frame split by rows:
+ * replaces the per-GPU band of rayTrace_MultiMegaKernel (src/kernelWrapper.h:33,
+ * src/kernelInvocations.cu:34-60,697-711; MultiDeviceRenderer gives GPU k the
+ * rows from k * (H / numGPUs)).  keep_records must be 0. */
+eos_status eos_render_rows(eos_scene* scene, const eos_camera* camera, int32_t row_begin, int32_t row_end,
+                           const eos_render_opts* opts, uint8_t* rgba_out, eos_render_stats* stats);
+
 /* n frames of one scene, frame i from cameras[i] into rgba_out[i] (device
  * memory, [H][W][4]), with two frames in flight: frame i + 1's primary rays
  * trace while frame i's shadow and reflection launches finish.  Each image is
