@@ -339,6 +339,7 @@ void rayTrace_MultiMegaKernel(cudaScene_t* scene, Camera* camera, int width, int
     // this GPU's band of rows only (src/kernelInvocations.cu:56: row j of the
     // band is row j + (height / numGPUS) * gpuID of the frame)
     const int rows = height / numGPUS;
+    if (rows <= 0 || gpuID < 0 || gpuID >= numGPUS) return;  // an empty band, as the reference's launch of 0 rows
     const eos_camera cam = to_c(download_camera(camera), width, height);
     eos_render_opts o{};
     o.depth = RAYTRACER_DEFAULT_TRACE_DEPTH;
