@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libeos_amd.so")
 
 EOS_OK, EOS_ERR_INVALID, EOS_ERR_HIP, EOS_ERR_NOMEM, EOS_ERR_STATE, EOS_ERR_LIMIT, EOS_ERR_NODEVICE = range(7)
+ABI_VERSION = 2   # EOS_ABI_VERSION of include/eos_c.h these structures follow
 EOS_MEM_HOST, EOS_MEM_DEVICE = 0, 1
 DBG = {"morton_sorted": 1, "perm": 2, "kchild": 3, "kparent": 4, "ibox": 5, "nodes": 6, "tris": 7,
        "morton": 8, "nodes4": 9, "xforms": 10}
@@ -137,6 +138,9 @@ def lib():
         for name in EXPORTS:
             if name != "eos_last_error" and name != "eos_abi_version":
                 getattr(L, name).restype = C.c_int
+        if L.eos_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH}: ABI version {L.eos_abi_version()}, these bindings need {ABI_VERSION}; "
+                               "rebuild the library")
         _LIB = L
     return _LIB
 
