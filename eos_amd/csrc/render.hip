@@ -17,7 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "eos_internal.h"
@@ -510,12 +512,27 @@ struct FrameStreams {
 // Largest queue of a ray tree level (1 << 26 rays, or a frame's pixels if more).
 constexpr size_t kTreeCap = (size_t)1 << 26;
 
+// Traversal launches number their items with 32-bit ints: primary ray ids, and
+// a level's shadow rays (its surfaces times the light slots).
+constexpr size_t kMaxItems = (size_t)INT_MAX;
+
+// Levels 1.. of a chain frame trace their shadow rays in launches of this many
+// consecutive levels (all of them unless their items could pass kMaxItems).
+int chain_shadow_span(const Frame& f) {
+    const size_t per_level = f.npix * (size_t)std::max(f.lstride, 1);
+    return (int)std::min<size_t>((size_t)std::max(f.depth, 1), std::max<size_t>(1, kMaxItems / per_level));
+}
+
 eos_status ensure_buffers(eos_frame_buffers& fb, const Frame& f, int trace_grid) {
+    if (f.n_ids > kMaxItems || f.npix * (size_t)std::max(f.lstride, 1) > kMaxItems)
+        return fail(EOS_ERR_LIMIT, "frame too large: pixels times light slots must stay below 2^31 (" +
+                                       std::to_string(f.npix) + " pixels, " + std::to_string(f.lstride) + " lights)");
     size_t cap = f.npix;
     fb.tree = f.tree;
     if (f.tree) {
-        // level l holds at most twice the rays of level l - 1 (two children per hit)
-        const size_t lim = std::max(f.npix, kTreeCap);
+        // level l holds at most twice the rays of level l - 1 (two children per hit),
+        // and its shadow rays (queue times light slots) stay within kMaxItems
+        const size_t lim = std::min(std::max(f.npix, kTreeCap), kMaxItems / (size_t)std::max(f.lstride, 1));
         fb.toff[0] = 0;
         for (int l = 0; l <= f.depth; ++l) {
             fb.tcap[l] = l == 0 ? f.npix : std::min(2 * fb.tcap[l - 1], lim);
@@ -705,7 +722,13 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         }
     }
     // deeper levels' shadow rays in one launch once the last level is set up
-    if (!f.tree && f.depth >= 1) EOS_TRY(shadow_and_phong(1, f.depth, S, 0, 1));
+    // (several when a frame's levels hold more shadow rays than 32-bit item ids
+    // number; launch slot l0 times the one that starts at level l0 >= 2)
+    if (!f.tree && f.depth >= 1) {
+        const int span = chain_shadow_span(f);
+        for (int l0 = 1; l0 <= f.depth; l0 += span)
+            EOS_TRY(shadow_and_phong(l0, std::min(f.depth, l0 + span - 1), S, 0, l0));
+    }
     if (two && !f.tree) {
         EOS_HIP(hipEventRecord(fs.sync[1], S2));
         EOS_HIP(hipStreamWaitEvent(S, fs.sync[1], 0));
@@ -779,7 +802,8 @@ eos_status frame_stats(const Frame& f, const int* hctr, const unsigned long long
             st->shade_ms += c + d2;
             if (l == 0) st->trace_first_ms = a;
         }
-        for (int k = 0; k < (f.tree ? f.depth + 1 : (f.depth >= 1 ? 2 : 1)); ++k) {
+        const int step = f.tree ? 1 : chain_shadow_span(f);
+        for (int k = 0; k <= f.depth; k = k == 0 ? 1 : k + step) {
             float b = 0.f;
             (void)hipEventElapsedTime(&b, E[F->s[2 * k]], E[F->s[2 * k + 1]]);
             st->trace_shadow_ms += b;
@@ -789,7 +813,8 @@ eos_status frame_stats(const Frame& f, const int* hctr, const unsigned long long
         st->shade_ms += r;
     }
     st->trace_closest_launches = f.depth + 1;
-    st->trace_shadow_launches = f.lstride > 0 ? (f.tree ? f.depth + 1 : (f.depth >= 1 ? 2 : 1)) : 0;
+    st->trace_shadow_launches =
+        f.lstride > 0 ? (f.tree ? f.depth + 1 : 1 + (f.depth + chain_shadow_span(f) - 1) / chain_shadow_span(f)) : 0;
     if (f.count) {
         for (int l = 0; l <= f.depth; ++l) {
             st->node_visits += (int64_t)(hstats[2 * l] + hstats[2 * (16 + l)]);
