@@ -20,6 +20,7 @@ ROOT = os.path.dirname(HERE)
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libeos_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.path.dirname(os.path.dirname(HIPCC))
 ARCH = os.environ.get("EOS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["eos_api.cpp", "eos_shim.cpp", "obj_mesh.cpp", "lbvh.hip", "render.hip", "adjoint.hip"]
@@ -53,7 +54,9 @@ def build(verbose: bool = True) -> str:
     # the C++ example that drives the reference-compatible API (include/eos/eos.h)
     demo = os.path.join(HERE, "examples", "render_scene")
     r = subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"),
+                        "-I", os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__",
                         os.path.join(HERE, "examples", "render_scene.cpp"), "-L", HERE, "-leos_amd",
+                        "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
                         "-Wl,-rpath,$ORIGIN/..", "-o", demo], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"example build failed:\n{r.stderr}")

@@ -13,10 +13,13 @@
 // scene.bin: int32 n_tri, n_mat, n_light, width, height, depth; float32
 // verts[n][9], normals[n][9]; int32 mat[n]; 64-byte materials; 32-byte lights;
 // float32 camera position, view, right, up (12 floats).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
+
+#include <hip/hip_runtime_api.h>
 
 #include "eos/eos.h"
 
@@ -144,6 +147,35 @@ int main(int argc, char** argv) {
     o = std::fopen((std::string(argv[3]) + ".rt").c_str(), "wb");
     std::fwrite(image3.data(), 1, image3.size(), o);
     std::fclose(o);
+
+    // the split-frame launcher (src/kernelWrapper.h:33) called once per band, as
+    // MultiDeviceRenderer's per-GPU threads do (src/MultiDeviceRenderer.cu:122-163):
+    // EOS_BANDS sets the band count; band k (H / n rows from row k * (H / n)) goes
+    // to a device buffer of its own size, the stitched rows to <out_single>.bands
+    if (const char* nb = std::getenv("EOS_BANDS")) {
+        const int n = std::atoi(nb), rows = H / std::max(n, 1);
+        cudaScene_t* bscene = createCudaScene(&scene);
+        DeviceCameraHandler bcam;
+        bcam.setDeviceCamera(&camera);
+        std::vector<unsigned char> image4(4 * (size_t)W * rows * n);
+        void* band = nullptr;
+        if (hipMalloc(&band, 4 * (size_t)W * rows + 4) != hipSuccess) return 1;
+        int blocks[2] = {(W + 15) / 16, (rows + 15) / 16}, threads[2] = {16, 16};
+        for (int k = 0; k < n; ++k) {
+            rayTrace_MultiMegaKernel(bscene, bcam.getDeviceCamera(), W, H, static_cast<uchar4_t*>(band), n, k, blocks,
+                                     threads);
+            gpuSynch();
+            if (hipMemcpy(image4.data() + 4 * (size_t)W * rows * k, band, 4 * (size_t)W * rows,
+                          hipMemcpyDeviceToHost) != hipSuccess)
+                return 1;
+        }
+        (void)hipFree(band);
+        bcam.freeDeviceCamera();
+        destroyCudaScene(bscene);
+        o = std::fopen((std::string(argv[3]) + ".bands").c_str(), "wb");
+        std::fwrite(image4.data(), 1, image4.size(), o);
+        std::fclose(o);
+    }
     for (Triangle* t : tris) delete t;
     for (LightSource* l : ls) delete l;
     return 0;

@@ -88,3 +88,48 @@ def test_glass_batch_and_rules():
     plain.ambient_index = ss[0].ambient_index
     assert np.array_equal(g.render()["rgba"], po.Oracle(plain).render()["rgba"]) or \
         np.abs(g.render()["rgba"].astype(int) - po.Oracle(plain).render()["rgba"]).max() <= 1
+
+
+def _glass_instances(width, height, depth=3):
+    """The glass scene's three spheres as one local-space icosphere placed by
+    per-triangle transformations (rotation, non-uniform scale, one mirrored),
+    over the world-space plane."""
+    import math
+    base = scenes.glass(width, height, level=2, depth=depth)
+    sv, sn = scenes.icosphere(2)
+    pv, pn = scenes.quad_grid(24, 4.0, -0.7)      # the glass scene's plane
+    xforms = np.stack([scenes.affine((0.3, 1.0, 0.2), 0.7, (0.7, 0.6, 0.75), (-1.2, 0.0, 0.0)),
+                       scenes.affine((1.0, 0.0, 0.4), 1.9, (0.7, 0.7, 0.55), (0.3, 0.0, -0.6)),
+                       scenes.affine((0.0, 0.0, 1.0), math.pi, (-0.5, 0.5, 0.5), (1.4, -0.2, 0.4))])
+    n = len(sv)
+    verts = np.concatenate([sv, sv, sv, pv])
+    normals = np.concatenate([sn, sn, sn, pn])
+    mat = np.concatenate([np.full(n, 0, np.int32), np.full(n, 1, np.int32), np.full(n, 2, np.int32),
+                          np.full(len(pv), 3, np.int32)])
+    xi = np.concatenate([np.repeat(np.arange(3, dtype=np.int32), n), np.full(len(pv), -1, np.int32)])
+    s = scenes.SceneData(verts, normals, mat, base.materials, base.lights, base.camera, depth,
+                         "glass instances", {}, xforms, xi)
+    s.ambient_index = base.ambient_index
+    return s
+
+
+@pytest.mark.parametrize("depth", [3, 5])
+def test_glass_with_transforms(depth):
+    """Ray trees over transformed triangles: the hit point through M, the normal
+    through transpose(inverse), both Fresnel children from the world-space hit."""
+    from eos_amd import _lib as E
+    s = _glass_instances(96, 72, depth)
+    print(f"glass with transforms depth={depth} n={s.n_tri}", flush=True)
+    g = E.Scene(s, leaf_max=4)
+    go = g.render(want_rgb=True, want_hits=True)
+    oo = po.Oracle(s).render()
+    agree, d = _compare(s, go, oo, "glass_xf")
+    st, oc = go["stats"], oo["counts"]
+    print(f"  agree={agree} maxdiff={d.max()} rays={st['primary_rays']}+{st['shadow_rays']}+{st['secondary_rays']}",
+          flush=True)
+    assert st["primary_rays"] == oc["primary"]
+    assert st["shadow_rays"] == oc["shadow"]
+    assert st["secondary_rays"] == oc["secondary"]
+    assert st["secondary_rays"] > 0
+    inst = s.xform_index[go["hit"][go["hit"] >= 0]]
+    assert set(np.unique(inst[inst >= 0])) == {0, 1, 2}

@@ -126,3 +126,25 @@ def test_cpp_api_translated_mesh(tmp_path):
     plain = E.Scene(scenes.SceneData(whole.verts, whole.normals, whole.mat, s.materials, s.lights, s.camera,
                                      s.depth, "unmoved"), leaf_max=4).render()["rgba"]
     assert not np.array_equal(plain, ref["rgba"])
+
+
+@pytest.mark.parametrize("n_bands", [2, 3, 7])
+def test_cpp_split_frame_launcher_renders_each_band(tmp_path, n_bands):
+    """rayTrace_MultiMegaKernel(…, numGPUS, gpuID, …) called once per band, as
+    the reference's MultiDeviceRenderer threads call it: each band is the
+    frame's rows [k·H/n, (k+1)·H/n) at the launcher's traced depth (4), byte for
+    byte; rows past n·(H/n) belong to no band, as in the reference."""
+    from eos_amd import _lib as E
+    s = scenes.config_b(72, 50, n=8000)
+    sc = tmp_path / "scene.bin"
+    _write_scene(sc, s)
+    o1, o2 = tmp_path / "multi.rgba", tmp_path / "single.rgba"
+    print(f"running the C++ example with {n_bands} bands", flush=True)
+    r = subprocess.run([DEMO, str(sc), str(o1), str(o2)], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, EOS_QUIET="1", EOS_BANDS=str(n_bands)))
+    assert r.returncode == 0, r.stderr
+    rows = s.height // n_bands
+    bands = np.fromfile(str(o2) + ".bands", np.uint8).reshape(rows * n_bands, s.width, 4)
+    g = E.Scene(s, leaf_max=4)
+    full = g.render(depth=4)["rgba"]
+    assert np.array_equal(bands, full[:rows * n_bands])
