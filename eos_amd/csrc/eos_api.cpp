@@ -77,15 +77,21 @@ static void free_scene(eos_scene* s) {
     for (DevBuf* b : bufs) b->release();
     s->fb.surf.release();
     s->fb.vis.release();
-    for (eos_frame_buffers* fb : {&s->fb, &s->fb1})
+    for (int k = 0; k < eos_scene::kFrameSlots; ++k) {
+        eos_frame_buffers* fb = k == 0 ? &s->fb : &s->fbx[k - 1];
         for (DevBuf* b : {&fb->hit0, &fb->q[0], &fb->q[1], &fb->hitq, &fb->levcol, &fb->levels, &fb->counters,
                           &fb->gstack, &fb->aa_acc, &fb->surf, &fb->vis, &fb->rayhit, &fb->aux, &fb->hitcol})
             b->release();
-    for (hipEvent_t e : {s->sync1[0], s->sync1[1], s->resolved[0], s->resolved[1], s->seq_start, s->seq_ev[0],
-                         s->seq_ev[1]})
-        if (e) (void)hipEventDestroy(e);
-    if (s->stream4) (void)hipStreamDestroy(s->stream4);
-    if (s->stream3) (void)hipStreamDestroy(s->stream3);
+        if (k > 0) {
+            for (hipEvent_t e : s->xsync[k - 1])
+                if (e) (void)hipEventDestroy(e);
+            for (hipStream_t st : s->xstream[k - 1])
+                if (st) (void)hipStreamDestroy(st);
+        }
+        if (s->resolved[k]) (void)hipEventDestroy(s->resolved[k]);
+        if (s->seq_ev[k]) (void)hipEventDestroy(s->seq_ev[k]);
+    }
+    if (s->seq_start) (void)hipEventDestroy(s->seq_start);
     if (s->host_ctr) (void)hipHostFree(s->host_ctr);
     for (hipEvent_t& e : s->ev)
         if (e) (void)hipEventDestroy(e);

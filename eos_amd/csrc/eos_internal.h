@@ -196,12 +196,15 @@ struct eos_scene {
     int num_cu = 256;
     eos_frame_buffers fb;
     hipEvent_t ev[96] = {};  // frame timing events
-    // Second frame slot of eos_render_frames (two frames in flight): frame
-    // buffers, two streams at default priority, fork/join events; the
-    // sequence's ordering and timing events; pinned counter copies.
-    eos_frame_buffers fb1;
-    hipStream_t stream3 = nullptr, stream4 = nullptr;
-    hipEvent_t sync1[2] = {}, resolved[2] = {}, seq_start = nullptr, seq_ev[2] = {};
+    // Further frame slots of eos_render_frames (kFrameSlots frames in flight;
+    // slot 0 is fb / stream / stream2 / sync): frame buffers, two streams at
+    // default priority and fork/join events each; the sequence's ordering and
+    // timing events; pinned counter copies.
+    static constexpr int kFrameSlots = 2;
+    eos_frame_buffers fbx[kFrameSlots - 1];
+    hipStream_t xstream[kFrameSlots - 1][2] = {};
+    hipEvent_t xsync[kFrameSlots - 1][2] = {};
+    hipEvent_t resolved[kFrameSlots] = {}, seq_start = nullptr, seq_ev[kFrameSlots] = {};
     void* host_ctr = nullptr;
     size_t host_ctr_bytes = 0;
 };

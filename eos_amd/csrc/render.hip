@@ -875,25 +875,33 @@ eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count
     return finish_frame(s->fb, f, s->stream, s->ev, &fe, st);
 }
 
-// Frames in flight: frame i runs on slot i % 2 (its own frame buffers and pair
-// of streams), so frame i + 1's primary rays trace while frame i's shadow and
-// reflection launches -- a few rays each, bound by the longest ray's chain of
-// steps -- still run.  Resolves are ordered (frame i's after frame i - 1's), so
-// outputs that alias end with the last frame.  Counters are copied per frame
-// to pinned host memory and read after the last frame.
+// Frames in flight: frame i runs on slot i % kFrameSlots (its own frame
+// buffers and pair of streams), so the next frames' primary rays trace while
+// frame i's shadow and reflection launches -- a few rays each, bound by the
+// longest ray's chain of steps -- still run.  Resolves are ordered (frame i's
+// after frame i - 1's), so outputs that alias end with the last frame.
+// Counters are copied per frame to pinned host memory and read after the last
+// frame.
 namespace {
-// Runs prepared frames two in flight (slot i % 2); per-frame stats (render_ms:
-// the sequence's GPU time / frames) into st when non-null.
+// Runs prepared frames kFrameSlots in flight; per-frame stats (render_ms: the
+// sequence's GPU time / frames) into st when non-null.
 eos_status run_pipelined(eos_scene* s, std::vector<Frame>& frames, eos_render_stats* st) {
+    constexpr int K = eos_scene::kFrameSlots;
     const int n = (int)frames.size();
-    if (!s->stream3) {
-        // the second slot's streams (default priority) and the sequence's events
-        hipError_t e = hipStreamCreateWithFlags(&s->stream3, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream4, hipStreamNonBlocking);
-        for (hipEvent_t* ev : {&s->sync1[0], &s->sync1[1], &s->resolved[0], &s->resolved[1]})
-            if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
-        for (hipEvent_t* ev : {&s->seq_start, &s->seq_ev[0], &s->seq_ev[1]})
-            if (e == hipSuccess) e = hipEventCreate(ev);
+    if (!s->seq_start) {
+        // the further slots' streams (default priority) and the sequence's events
+        hipError_t e = hipSuccess;
+        for (int k = 1; k < K; ++k) {
+            for (hipStream_t& x : s->xstream[k - 1])
+                if (e == hipSuccess) e = hipStreamCreateWithFlags(&x, hipStreamNonBlocking);
+            for (hipEvent_t& x : s->xsync[k - 1])
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+        }
+        for (int k = 0; k < K; ++k) {
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&s->resolved[k], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreate(&s->seq_ev[k]);
+        }
+        if (e == hipSuccess) e = hipEventCreate(&s->seq_start);
         if (e != hipSuccess) return fail(EOS_ERR_HIP, std::string("frames in flight: ") + hipGetErrorString(e));
     }
     const size_t need = kCounterBytes * (size_t)n;
@@ -904,38 +912,44 @@ eos_status run_pipelined(eos_scene* s, std::vector<Frame>& frames, eos_render_st
         EOS_HIP(hipHostMalloc(&s->host_ctr, need, hipHostMallocDefault));
         s->host_ctr_bytes = need;
     }
-    eos_frame_buffers* fbs[2] = {&s->fb, &s->fb1};
-    const FrameStreams fs[2] = {{s->stream, s->stream2, s->sync}, {s->stream3, s->stream4, s->sync1}};
+    eos_frame_buffers* fbs[K];
+    FrameStreams fs[K];
+    for (int k = 0; k < K; ++k) {
+        fbs[k] = k == 0 ? &s->fb : &s->fbx[k - 1];
+        fs[k] = k == 0 ? FrameStreams{s->stream, s->stream2, s->sync}
+                       : FrameStreams{s->xstream[k - 1][0], s->xstream[k - 1][1], s->xsync[k - 1]};
+    }
     for (Frame& f : frames) {
         f.cfg.waves_per_cu = f.cfg.frames_waves_per_cu;
         f.cfg.refill = f.cfg.frames_refill;
         f.cfg.batch = f.cfg.frames_batch;
     }
-    // size both slots for every frame first (buffers only grow): no buffer is
-    // reallocated while a frame in flight uses it
+    const int slots = std::min(K, std::max(n, 1));
+    // size every slot in use for every frame first (buffers only grow): no
+    // buffer is reallocated while a frame in flight uses it
     for (const Frame& f : frames)
-        for (eos_frame_buffers* b : fbs) EOS_TRY(ensure_buffers(*b, f, trace_grid_of(f)));
+        for (int k = 0; k < slots; ++k) EOS_TRY(ensure_buffers(*fbs[k], f, trace_grid_of(f)));
     s->fb.rec_valid = 0;
     EOS_HIP(hipEventRecord(s->seq_start, s->stream));
-    EOS_HIP(hipStreamWaitEvent(s->stream3, s->seq_start, 0));
+    for (int k = 1; k < slots; ++k) EOS_HIP(hipStreamWaitEvent(fs[k].S, s->seq_start, 0));
     for (int i = 0; i < n; ++i) {
-        const int k = i & 1;
+        const int k = i % slots;
         Frame& f = frames[(size_t)i];
         EOS_TRY(ensure_buffers(*fbs[k], f, trace_grid_of(f)));
-        f.resolve_after = i > 0 ? s->resolved[k ^ 1] : nullptr;
+        f.resolve_after = i > 0 ? s->resolved[(i - 1) % slots] : nullptr;
         f.resolve_done = s->resolved[k];
         EOS_TRY(enqueue_frame(*fbs[k], f, fs[k], nullptr, nullptr));
         EOS_HIP(hipMemcpyAsync(static_cast<char*>(s->host_ctr) + kCounterBytes * (size_t)i, fbs[k]->counters.ptr,
                                kCounterBytes, hipMemcpyDeviceToHost, fs[k].S));
     }
-    EOS_HIP(hipEventRecord(s->seq_ev[0], s->stream));
-    EOS_HIP(hipEventRecord(s->seq_ev[1], s->stream3));
-    EOS_HIP(hipStreamSynchronize(s->stream));
-    EOS_HIP(hipStreamSynchronize(s->stream3));
-    float ms0 = 0.f, ms1 = 0.f;
-    (void)hipEventElapsedTime(&ms0, s->seq_start, s->seq_ev[0]);
-    if (n > 1) (void)hipEventElapsedTime(&ms1, s->seq_start, s->seq_ev[1]);
-    const double total = std::max(ms0, ms1);
+    double total = 0.0;
+    for (int k = 0; k < slots; ++k) EOS_HIP(hipEventRecord(s->seq_ev[k], fs[k].S));
+    for (int k = 0; k < slots; ++k) {
+        EOS_HIP(hipStreamSynchronize(fs[k].S));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, s->seq_start, s->seq_ev[k]);
+        total = std::max(total, (double)ms);
+    }
     for (int i = 0; i < n; ++i) {
         const char* base = static_cast<const char*>(s->host_ctr) + kCounterBytes * (size_t)i;
         eos_render_stats one{};
