@@ -412,7 +412,12 @@ def main() -> int:
                      "kernel": kernel, "launch_ms": round(t_first, 4),
                      "alg_bytes_per_launch": int(alg_bytes),
                      "node_visits": int(counted["closest_visits_first"]),
-                     "tri_tests": int(counted["closest_tests_first"])},
+                     "tri_tests": int(counted["closest_tests_first"]),
+                     # the measured HBM bytes of the launch at its measured time: node and
+                     # triangle records are mostly re-read out of L2 and MALL, so the walk is
+                     # bound by its dependent steps, not by HBM bandwidth
+                     "hbm_achieved": round(traffic / (t_first * 1e-3) / 1e9, 2) if traffic else None,
+                     "hbm_frac": round(traffic / (t_first * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None},
         "rays_per_frame": {"primary": st["primary_rays"], "shadow": st["shadow_rays"],
                            "secondary": st["secondary_rays"], "hits": st["hits"]},
         "gpu_ms": {"render": round(mean("render_ms"), 4), "trace_closest": round(mean("trace_closest_ms"), 4),
