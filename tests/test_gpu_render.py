@@ -1,6 +1,6 @@
 """GPU render path vs the CPU oracle (reference RayTracer semantics).
 
-Bars (north_star): primary hit triangle IDs identical on >= 99.99% of rays,
+Bars: primary hit triangle IDs identical on >= 99.99% of rays,
 every mismatch logged; RGBA within 1/255 per channel after quantisation;
 ray counts (primary, shadow, secondary) equal to the oracle's.  The GPU's
 node-visit and triangle-test counters equal the oracle's restatement of the
