@@ -65,6 +65,9 @@ def test_null_arguments_are_rejected():
     assert L.eos_destroy_scene(None) == E.EOS_OK
     assert L.eos_abi_version() == E.ABI_VERSION == 2
     assert L.eos_set_transforms(None, 0, None, None, 0) == E.EOS_ERR_INVALID
+    assert L.eos_render_rows(None, None, 0, 1, None, None, None) == E.EOS_ERR_INVALID
+    assert L.eos_render_frames(None, None, 1, None, None, None) == E.EOS_ERR_INVALID
+    assert L.eos_set_ambient_refractive_index(None, 1.0) == E.EOS_ERR_INVALID
 
 
 def test_no_device_is_reported():
