@@ -92,3 +92,34 @@ def test_random_scene_matches_oracle(seed):
     assert st["primary_rays"] == oc["primary"]
     assert st["shadow_rays"] == oc["shadow"]
     assert st["secondary_rays"] == oc["secondary"]
+
+
+@pytest.mark.parametrize("group", range(4))
+def test_random_batch_equals_single_frames(group):
+    """Eight random scenes (built at one tree width) in one batched frame, with
+    and without graph replay: every image equals that scene's own frame."""
+    from eos_amd import _lib as E
+    ss = [_random_scene(100 + 8 * group + k)[0] for k in range(8)]
+    depth = 1 + group % 3
+    print(f"random batch group={group} depth={depth} sizes={[s.n_tri for s in ss]}", flush=True)
+    gs = [E.Scene(s, leaf_max=4, width=4) for s in ss]
+    singles = [g.render(depth=depth)["rgba"] for g in gs]
+    for use_graph in (False, True):
+        outs, st = E.render_batch(gs, depth=depth, use_graph=use_graph)
+        for k, (o, want) in enumerate(zip(outs, singles)):
+            assert np.array_equal(o, want), (group, k, use_graph)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_bands_stitch(seed):
+    """A random scene rendered in random row bands equals its whole frame."""
+    from eos_amd import _lib as E
+    s, o = _random_scene(200 + seed)
+    rng = np.random.default_rng(seed)
+    H = s.height
+    cuts = np.unique(np.concatenate([[0, H], rng.integers(1, H, int(rng.integers(1, 5)))]))
+    print(f"random bands seed={seed} {s.width}x{H} cuts={cuts.tolist()} aa={o['aa']}", flush=True)
+    g = E.Scene(s, leaf_max=o["leaf_max"], width=o["width"])
+    full = g.render(aa_samples=o["aa"])["rgba"]
+    bands = np.concatenate([g.render_rows(int(a), int(b), aa_samples=o["aa"])["rgba"] for a, b in zip(cuts, cuts[1:])])
+    assert np.array_equal(bands, full)
