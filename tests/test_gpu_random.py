@@ -123,3 +123,37 @@ def test_random_bands_stitch(seed):
     full = g.render(aa_samples=o["aa"])["rgba"]
     bands = np.concatenate([g.render_rows(int(a), int(b), aa_samples=o["aa"])["rgba"] for a, b in zip(cuts, cuts[1:])])
     assert np.array_equal(bands, full)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_adjoint_matches_fp64_forward_mode(seed):
+    """Gradients of the L2 image loss on random opaque scenes (transformations
+    on most seeds) against the oracle's FP64 forward mode over the same hits."""
+    from eos_amd import _lib as E
+    s, o = _random_scene(300 + seed)
+    s.materials["transparent"] = 0
+    if s.depth == 0:
+        s.depth = 1
+    print(f"random adjoint seed={seed} n={s.n_tri} {s.width}x{s.height} depth={s.depth} "
+          f"xf={s.xforms is not None}", flush=True)
+    g = E.Scene(s, leaf_max=o["leaf_max"], width=o["width"])
+    tm, tl = scenes.perturb_params(s, 77 + seed)
+    g.set_materials(tm)
+    g.set_lights(tl)
+    target = g.render(want_rgb=True)["rgb"]
+    g.set_materials(s.materials)
+    g.set_lights(s.lights)
+    fwd = g.render(want_rgb=True, want_hits=True, keep_records=True)
+    gm, gl, _ = g.adjoint(2.0 * (fwd["rgb"] - target))
+    out = po.Oracle(s).render(want_recs=True)
+    assert np.array_equal(out["hit"], fwd["hit"])
+    prm = po.params_vector(s.materials, s.lights)
+    ref = po.grad_fwd64(out["recs"], s.camera["position"], len(s.materials), len(s.lights), prm, target)
+    got = np.concatenate([gm.ravel(), gl.ravel()])
+    scale = np.abs(ref).max()
+    if scale == 0.0:
+        assert np.abs(got).max() == 0.0
+        return
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-3 * scale)
+    print(f"  max rel err {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-3
