@@ -157,3 +157,25 @@ def test_random_adjoint_matches_fp64_forward_mode(seed):
     err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-3 * scale)
     print(f"  max rel err {err.max():.2e}", flush=True)
     assert err.max() <= 1e-3
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_frames_in_flight(seed):
+    """Five cameras around a random scene rendered as frames in flight equal
+    five single renders, each with its own camera."""
+    import torch
+    from eos_amd import _lib as E
+    s, o = _random_scene(400 + seed)
+    rng = np.random.default_rng(seed)
+    cams = []
+    for k in range(5):
+        eye = rng.normal(size=3)
+        cams.append(scenes.make_camera(3.0 * eye / np.linalg.norm(eye), (0.0, 0.0, 0.0), 55.0, s.width, s.height))
+    cams = np.stack(cams)
+    print(f"random frames seed={seed} n={s.n_tri} {s.width}x{s.height} depth={s.depth}", flush=True)
+    g = E.Scene(s, leaf_max=o["leaf_max"], width=o["width"])
+    outs = [torch.empty(s.width * s.height * 4, dtype=torch.uint8, device="cuda:0") for _ in cams]
+    g.render_frames(cams, [t.data_ptr() for t in outs])
+    for k, c in enumerate(cams):
+        want = g.render(camera=c)["rgba"]
+        assert np.array_equal(outs[k].cpu().numpy().reshape(s.height, s.width, 4), want), k
