@@ -631,6 +631,10 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         int cur = kDone, sp = 0;
         int lpos = -1;   // next triangle of the current leaf (-1: not in a leaf)
         int budget = 0;  // node visits left for this ray: a tree visits each internal node at most once
+        // XF: the ray in the local space of transformation lxi, kept while the
+        // walk stays among that transformation's triangles (-1: none yet)
+        int lxi = -1;
+        F3 lo_o = {0, 0, 0}, lo_d = {0, 0, 0};
         const int thresh = a.refill < 1 ? 1 : (a.refill > 64 ? 64 : a.refill);
 
         for (;;) {
@@ -662,6 +666,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     bu = bv = 0.0f;
                     sp = 0;
                     lpos = -1;
+                    lxi = -1;
                     budget = r.max_visits;
                     cur = r.valid ? r.root : kDone;
                 }
@@ -771,8 +776,29 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                             const float4 b0 = geo[3 * (size_t)s1], b1 = geo[3 * (size_t)s1 + 1], b2 = geo[3 * (size_t)s1 + 2];
                             float ta, ua, va, tb, ub, vb;
                             F3 oa = o, da = d, ob = o, db = d;
-                            to_local<XF>(xf, __float_as_int(a2.z), oa, da);
-                            to_local<XF>(xf, __float_as_int(b2.z), ob, db);
+                            if (XF) {
+                                const int xa = __float_as_int(a2.z), xb = __float_as_int(b2.z);
+                                if (xa >= 0 && xa != lxi) {
+                                    lo_o = o;
+                                    lo_d = d;
+                                    to_local<XF>(xf, xa, lo_o, lo_d);
+                                    lxi = xa;
+                                }
+                                if (xa >= 0) {
+                                    oa = lo_o;
+                                    da = lo_d;
+                                }
+                                if (xb >= 0 && xb != lxi) {
+                                    lo_o = o;
+                                    lo_d = d;
+                                    to_local<XF>(xf, xb, lo_o, lo_d);
+                                    lxi = xb;
+                                }
+                                if (xb >= 0) {
+                                    ob = lo_o;
+                                    db = lo_d;
+                                }
+                            }
                             const bool ha = moller_trumbore(oa, da, f3(a0.x, a0.y, a0.z), f3(a0.w, a1.x, a1.y),
                                                             f3(a1.z, a1.w, a2.x), ta, ua, va);
                             const bool hb = moller_trumbore(ob, db, f3(b0.x, b0.y, b0.z), f3(b0.w, b1.x, b1.y),
