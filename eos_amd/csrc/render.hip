@@ -76,10 +76,12 @@ __device__ __forceinline__ float fresnel(F3 incident, F3 normal, float n1, float
 template <bool L0, bool BATCH>
 __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
     const int n = L0 ? a.n_ids : *a.count_ptr;
-    const int lane = __lane_id();
     const int stride = gridDim.x * blockDim.x;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
-        const int i = base + lane;
+    // every wave of the workgroup runs every round (the round's slots are
+    // claimed once per workgroup, below), so the trip count is the workgroup's
+    __shared__ int s_cnt[3][4], s_base[3][4];
+    for (int bbase = blockIdx.x * blockDim.x; bbase < n; bbase += stride) {
+        const int i = bbase + (int)threadIdx.x;
         const bool valid = i < n;
         float4 h = valid ? a.hit_in[i] : make_float4(INFINITY, __int_as_float(-1), 0.f, 0.f);
         const int slot = __float_as_int(h.y);
@@ -156,13 +158,43 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
         } else if (!L0 && hit) {
             a.levels[pixel] = (uint8_t)(a.level + 1);
         }
-        const int j = wave_append(a.hits_ctr, hit);
-        wave_sum_add(a.shadow_ctr, hit ? (BATCH ? a.table[scene].sc.n_light : a.sc.n_light) : 0);
+        // slots of this round, claimed once per workgroup: hit surfaces, queued
+        // rays (reflection, then the Fresnel ray of a transparent material) and
+        // the shadow-ray count
+        const bool transp = a.rayhit && hit && mats[mat].transparent != 0;
+        const bool child = transp && a.level < a.depth;
+        const uint64_t lt = lanemask_lt();
+        const uint64_t mh = __ballot(hit), mr = __ballot(refl), mc = __ballot(child);
+        int nl = hit ? (BATCH ? a.table[scene].sc.n_light : a.sc.n_light) : 0;
+        for (int off = 32; off > 0; off >>= 1) nl += __shfl_xor(nl, off);
+        const int w = (int)threadIdx.x >> 6;
+        if (__lane_id() == 0) {
+            s_cnt[0][w] = __popcll(mh);
+            s_cnt[1][w] = __popcll(mr) + __popcll(mc);
+            s_cnt[2][w] = nl;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int* const ctrs[3] = {a.hits_ctr, a.q_out_ctr, a.shadow_ctr};
+            for (int c = 0; c < 3; ++c) {
+                int tot = 0;
+                for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+                    s_base[c][k] = tot;
+                    tot += s_cnt[c][k];
+                }
+                const int b0 = tot > 0 ? atomicAdd(ctrs[c], tot) : 0;
+                for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s_base[c][k] += b0;
+            }
+        }
+        __syncthreads();
+        const int j = hit ? s_base[0][w] + __popcll(mh & lt) : -1;
+        const int qb = s_base[1][w];
+        const int k = refl ? qb + __popcll(mr & lt) : -1;
+        const int kt = child ? qb + __popcll(mr) + __popcll(mc & lt) : -1;
         if (hit) {
             a.surf_out[2 * (size_t)j] = make_float4(P.x, P.y, P.z, __int_as_float(pixel));
             a.surf_out[2 * (size_t)j + 1] = make_float4(N.x, N.y, N.z, __int_as_float(pack_mat(mat, scene)));
         }
-        const int k = wave_append(a.q_out_ctr, refl);
         if (refl && k < a.q_cap) {
             // src/RayTracer.cpp:350-355
             const F3 R = normalize3(reflect3(d, N));
@@ -181,7 +213,6 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
             } else if (valid) {
                 a.rayhit[i] = hit ? j : -1;
             }
-            const bool transp = hit && mats[mat].transparent != 0;
             float K = 0.0f;
             F3 fo = {0, 0, 0}, fd = {0, 0, 0};
             if (transp) {
@@ -200,8 +231,6 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
                 fd = normalize3(reflect3(d, nn));
                 fo = add(P, scale(1e-3f, nn));
             }
-            const bool child = transp && a.level < a.depth;
-            const int kt = wave_append(a.q_out_ctr, child);
             if (child && kt < a.q_cap) {
                 a.q_out[2 * (size_t)kt] = make_float4(fo.x, fo.y, fo.z, __int_as_float(pixel));
                 a.q_out[2 * (size_t)kt + 1] = make_float4(fd.x, fd.y, fd.z, __int_as_float(scene));
