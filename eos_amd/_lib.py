@@ -23,7 +23,13 @@ EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_build
            "eos_update_lights", "eos_render", "eos_render_adjoint", "eos_render_batch",
            "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
            "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free", "eos_set_transforms",
-           "eos_set_ambient_refractive_index", "eos_render_frames", "eos_render_rows"]
+           "eos_set_ambient_refractive_index", "eos_render_frames", "eos_render_rows", "eos_trace_rays"]
+
+# eos_trace_rays query modes and the eos_ray_hit record (48 bytes)
+QUERY = {"nearest": 0, "occluded": 1, "linear": 2}
+RAY_HIT = np.dtype([("point", np.float32, 3), ("hit", np.int32), ("normal", np.float32, 3), ("material", np.int32),
+                    ("t", np.float32), ("u", np.float32), ("v", np.float32), ("triangle", np.int32)])
+assert RAY_HIT.itemsize == 48
 
 
 class EosError(RuntimeError):
@@ -78,6 +84,10 @@ class AdjointStats(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("depth", C.c_int32), ("output_memory", C.c_int32), ("use_graph", C.c_int32),
                 ("pad", C.c_int32)]
+
+
+class TraceOpts(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("memory", C.c_int32)]
 
 
 class BatchStats(C.Structure):
@@ -135,6 +145,7 @@ def lib():
         L.eos_set_ambient_refractive_index.argtypes = [vp, C.c_float]
         L.eos_render_frames.argtypes = [vp, vp, i32, vp, vp, vp]
         L.eos_render_rows.argtypes = [vp, vp, i32, i32, vp, vp, vp]
+        L.eos_trace_rays.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         for name in EXPORTS:
             if name != "eos_last_error" and name != "eos_abi_version":
                 getattr(L, name).restype = C.c_int
@@ -190,6 +201,8 @@ class Scene:
         check(lib().eos_create_scene(C.byref(d), C.byref(h)))
         self.h = h
         self.build_stats = None
+        # bumped by every call that changes the scene or replaces its kept hit records
+        self.generation = 0
         xf = getattr(scene, "xforms", None)
         if xf is not None:
             self.set_transforms(xf, scene.xform_index)
@@ -206,12 +219,14 @@ class Scene:
     def set_transforms(self, xforms, xform_index):
         """eos_set_transforms: [T][16] column-major float32 matrices and a per-triangle
         index (-1 = world space).  The scene must be built again before rendering."""
+        self.generation += 1
         self._xf = np.ascontiguousarray(xforms, np.float32).reshape(-1, 16)
         self._xi = np.ascontiguousarray(xform_index, np.int32)
         check(lib().eos_set_transforms(self.h, len(self._xf), _ptr(self._xf), _ptr(self._xi), EOS_MEM_HOST))
         self.n_xforms = len(self._xf)
 
     def build(self, leaf_max: int = 4, width: int = 4) -> dict:
+        self.generation += 1
         st = BuildStats()
         o = BuildOpts(leaf_max, width)
         check(lib().eos_build_lbvh_ex(self.h, C.byref(o), C.byref(st)))
@@ -230,10 +245,12 @@ class Scene:
             pass
 
     def set_materials(self, mats):
+        self.generation += 1
         self._mats = np.ascontiguousarray(mats)
         check(lib().eos_update_materials(self.h, _ptr(self._mats), len(self._mats)))
 
     def set_lights(self, lights):
+        self.generation += 1
         self._lights = np.ascontiguousarray(lights)
         check(lib().eos_update_lights(self.h, _ptr(self._lights), len(self._lights)))
 
@@ -257,6 +274,7 @@ class Scene:
 
     def render(self, camera=None, depth=None, want_rgb=False, want_hits=False, keep_records=False,
                count_visits=False, want_rgba=True, aa_samples=0):
+        self.generation += 1
         cam = np.array(self.data.camera if camera is None else camera, copy=True)
         depth = self.data.depth if depth is None else depth
         W, H = int(cam["width"]), int(cam["height"])
@@ -272,6 +290,7 @@ class Scene:
     def render_device(self, camera, depth, rgba_dev_ptr, rgb_dev_ptr=None, hit_dev_ptr=None,
                       keep_records=False, count_visits=False, aa_samples=0) -> dict:
         """Render into device buffers (raw pointers, e.g. torch tensors' data_ptr())."""
+        self.generation += 1
         cam = np.array(camera, copy=True)
         o = RenderOpts(depth, EOS_MEM_DEVICE, rgb_dev_ptr, hit_dev_ptr, int(keep_records), int(count_visits),
                        int(aa_samples))
@@ -282,6 +301,7 @@ class Scene:
     def render_rows(self, row_begin, row_end, camera=None, depth=None, want_rgb=False, want_hits=False,
                     aa_samples=0):
         """eos_render_rows: rows [row_begin, row_end) of the frame (host outputs for the band)."""
+        self.generation += 1
         cam = np.array(self.data.camera if camera is None else camera, copy=True)
         depth = self.data.depth if depth is None else depth
         W, Hb = int(cam["width"]), max(row_end - row_begin, 0)
@@ -296,6 +316,7 @@ class Scene:
     def render_frames(self, cameras, rgba_dev_ptrs, depth=None, count_visits=False) -> list:
         """eos_render_frames: frame i from cameras[i] into the device buffer rgba_dev_ptrs[i]
         (two frames in flight); per-frame stats."""
+        self.generation += 1
         cams = np.ascontiguousarray(cameras)
         n = len(cams)
         depth = self.data.depth if depth is None else depth
@@ -304,6 +325,31 @@ class Scene:
         st = (RenderStats * n)()
         check(lib().eos_render_frames(self.h, cams.ctypes.data, n, C.byref(o), ptrs, st))
         return [s.as_dict() for s in st]
+
+    def trace_rays(self, origins, directions, mode: str = "nearest"):
+        """eos_trace_rays on host arrays [n][3]: for "nearest" (Scene::findMinDistanceIntersectionBVH)
+        and "linear" (Scene::findMinDistanceIntersectionLinear) a RAY_HIT record per ray, for
+        "occluded" (Scene::shadowRayVisibilityBVH) a bool per ray."""
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        if len(o) != len(d):
+            raise ValueError("origins and directions differ in length")
+        n = len(o)
+        opts = TraceOpts(QUERY[mode], EOS_MEM_HOST)
+        if mode == "occluded":
+            occ = np.zeros(n, np.uint8)
+            check(lib().eos_trace_rays(self.h, n, _ptr(o), _ptr(d), C.byref(opts), None, _ptr(occ)))
+            return occ.astype(bool)
+        hits = np.zeros(n, RAY_HIT)
+        check(lib().eos_trace_rays(self.h, n, _ptr(o), _ptr(d), C.byref(opts), _ptr(hits), None))
+        return hits
+
+    def trace_rays_device(self, n, origins_ptr, directions_ptr, mode: str = "nearest", hits_ptr=None,
+                          occluded_ptr=None):
+        """eos_trace_rays with device buffers (raw pointers): origins / directions [n][3] float32,
+        hits [n] RAY_HIT records or occluded [n] uint8."""
+        opts = TraceOpts(QUERY[mode], EOS_MEM_DEVICE)
+        check(lib().eos_trace_rays(self.h, n, origins_ptr, directions_ptr, C.byref(opts), hits_ptr, occluded_ptr))
 
     def adjoint(self, dl_drgb, camera=None, depth=None, recompute=False):
         cam = np.array(self.data.camera if camera is None else camera, copy=True)

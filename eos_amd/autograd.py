@@ -8,6 +8,12 @@ Parameters are torch tensors on any device:
 The output is the pre-quantisation image [H, W, 3] (float32, on cuda:0).
 Gradients come from the hand-written HIP adjoint with hit sets held fixed
 (visibility, hit points and reflection paths of the forward pass).
+
+Each forward keeps its own materials and lights.  The scene holds one set of
+hit records, so a backward whose scene was rendered or changed after its
+forward (two renders in one loss, a parameter update in between) uploads its
+forward's parameters again and re-traces the records instead of replaying
+another call's, then puts the scene's parameters back.
 """
 from __future__ import annotations
 
@@ -36,13 +42,17 @@ def _lights_from(base, pos, col):
 class RenderFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, kd, ks, shininess, light_pos, light_color, scene: "E.Scene", camera, depth: int):
-        scene.set_materials(_materials_from(scene.data.materials, kd, ks, shininess))
-        scene.set_lights(_lights_from(scene.data.lights, light_pos, light_color))
+        mats = _materials_from(scene.data.materials, kd, ks, shininess)
+        lights = _lights_from(scene.data.lights, light_pos, light_color)
+        scene.set_materials(mats)
+        scene.set_lights(lights)
         W, H = int(camera["width"]), int(camera["height"])
         rgb = torch.empty((H, W, 3), dtype=torch.float32, device="cuda:0")
         torch.cuda.synchronize()
         scene.render_device(camera, depth, None, rgb_dev_ptr=rgb.data_ptr(), keep_records=True)
         ctx.scene, ctx.camera, ctx.depth = scene, camera, depth
+        ctx.mats, ctx.lights = mats, lights
+        ctx.generation = scene.generation   # the records this forward left on the scene
         ctx.shapes = (kd.shape[0], light_pos.shape[0])
         ctx.devices = (kd.device, light_pos.device)
         return rgb
@@ -55,7 +65,17 @@ class RenderFunction(torch.autograd.Function):
         gm = torch.zeros((M, 7), dtype=torch.float64, device="cuda:0")
         gl = torch.zeros((max(L, 1), 6), dtype=torch.float64, device="cuda:0")
         torch.cuda.synchronize()
-        scene.adjoint_device(camera, depth, g.data_ptr(), gm.data_ptr(), gl.data_ptr())
+        stale = scene.generation != ctx.generation
+        if stale:
+            current = (scene._mats.copy(), scene._lights.copy())
+            scene.set_materials(ctx.mats)
+            scene.set_lights(ctx.lights)
+        try:
+            scene.adjoint_device(camera, depth, g.data_ptr(), gm.data_ptr(), gl.data_ptr(), recompute=stale)
+        finally:
+            if stale:
+                scene.set_materials(current[0])
+                scene.set_lights(current[1])
         gm = gm.to(ctx.devices[0], torch.float32)
         gl = gl[:L].to(ctx.devices[1], torch.float32)
         return gm[:, 0:3], gm[:, 3:6], gm[:, 6], gl[:, 0:3], gl[:, 3:6], None, None, None

@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ROCM = os.path.dirname(os.path.dirname(HIPCC))
 ARCH = os.environ.get("EOS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["eos_api.cpp", "eos_shim.cpp", "obj_mesh.cpp", "lbvh.hip", "render.hip", "adjoint.hip"]
+SOURCES = ["eos_api.cpp", "eos_shim.cpp", "obj_mesh.cpp", "lbvh.hip", "render.hip", "adjoint.hip", "query.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function", "-Wno-unused-result",

@@ -71,7 +71,7 @@ static void free_scene(eos_scene* s) {
                       &s->shade, &s->codes, &s->codes_sorted, &s->perm, &s->kchild, &s->kparent,
                       &s->first_last, &s->ibox, &s->lbox, &s->flags, &s->cbounds, &s->sort_tmp_k,
                       &s->sort_tmp_v, &s->sort_hist, &s->bounds_part, &s->nodes4, &s->flags4, &s->scan_tmp,
-                      &s->xf, &s->xf_index, &s->wverts,
+                      &s->xf, &s->xf_index, &s->wverts, &s->q_rays, &s->q_out, &s->q_rec, &s->q_ctr, &s->q_gstack,
                       &s->fb.hit0, &s->fb.q[0], &s->fb.q[1],
                       &s->fb.hitq, &s->fb.levcol, &s->fb.levels, &s->fb.counters, &s->fb.gstack, &s->fb.aa_acc};
     for (DevBuf* b : bufs) b->release();
@@ -193,10 +193,12 @@ eos_status eos_create_scene(const eos_scene_desc* d, eos_scene** out) {
             if (d->material_index[i] < 0 || d->material_index[i] >= d->num_materials)
                 return fail(EOS_ERR_INVALID, "eos_create_scene: material index out of range at triangle " +
                                                  std::to_string(i));
-    eos_scene* s = new eos_scene();
-    EOS_HIP(hipGetDevice(&s->device));
+    int device = 0;
+    EOS_HIP(hipGetDevice(&device));
     hipDeviceProp_t p;
-    EOS_HIP(hipGetDeviceProperties(&p, s->device));
+    EOS_HIP(hipGetDeviceProperties(&p, device));
+    eos_scene* s = new eos_scene();
+    s->device = device;
     s->num_cu = p.multiProcessorCount;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking);
@@ -302,6 +304,7 @@ eos_status eos_update_materials(eos_scene* s, const eos_material* m, int32_t n) 
     EOS_HIP(hipMemcpyAsync(s->mats.ptr, m, sizeof(eos_material) * n, hipMemcpyHostToDevice, s->stream));
     EOS_HIP(hipStreamSynchronize(s->stream));
     s->fb.rec_valid = 0;  // reflection flags and weights shape the stored records
+    ++s->mat_generation;  // cached batched frames were shaped by the old flags
     s->any_transparent = false;
     for (int i = 0; i < n; ++i) s->any_transparent |= m[i].transparent != 0;
     return EOS_OK;
@@ -401,6 +404,25 @@ eos_status eos_render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adj
     if (o->depth < 0 || o->depth > EOS_MAX_DEPTH) return fail(EOS_ERR_INVALID, "eos_render_adjoint: bad depth");
     EOS_HIP(hipSetDevice(s->device));
     return render_adjoint(s, cam, o, dl, gm, gl, st);
+}
+
+eos_status eos_trace_rays(eos_scene* s, int32_t n, const float* o, const float* d, const eos_trace_opts* opts,
+                          eos_ray_hit* hits, uint8_t* occ) {
+    if (!s || !opts) return fail(EOS_ERR_INVALID, "eos_trace_rays: null argument");
+    if (n < 0) return fail(EOS_ERR_INVALID, "eos_trace_rays: negative ray count");
+    if (opts->mode != EOS_QUERY_NEAREST && opts->mode != EOS_QUERY_OCCLUDED && opts->mode != EOS_QUERY_NEAREST_LINEAR)
+        return fail(EOS_ERR_INVALID, "eos_trace_rays: unknown query mode");
+    if (opts->memory != EOS_MEM_HOST && opts->memory != EOS_MEM_DEVICE)
+        return fail(EOS_ERR_INVALID, "eos_trace_rays: memory must be EOS_MEM_HOST or EOS_MEM_DEVICE");
+    if (n > 0 && (!o || !d)) return fail(EOS_ERR_INVALID, "eos_trace_rays: null ray arrays");
+    if (n > 0 && opts->mode == EOS_QUERY_OCCLUDED && !occ)
+        return fail(EOS_ERR_INVALID, "eos_trace_rays: EOS_QUERY_OCCLUDED writes `occluded`, which is null");
+    if (n > 0 && opts->mode != EOS_QUERY_OCCLUDED && !hits)
+        return fail(EOS_ERR_INVALID, "eos_trace_rays: nearest-hit queries write `hits`, which is null");
+    if (!s->built) return fail(EOS_ERR_STATE, "eos_trace_rays: call eos_build_lbvh first");
+    if (n == 0) return EOS_OK;
+    EOS_HIP(hipSetDevice(s->device));
+    return trace_rays(s, n, o, d, opts, hits, occ);
 }
 
 eos_status eos_render_batch(eos_scene* const* scenes, const eos_camera* cams, int32_t n, const eos_batch_opts* o,

@@ -186,6 +186,7 @@ struct eos_scene {
     int n_xf = 0;
     float ambient_index = 1.0f;   // Scene::setAmbientRefractiveIndex
     bool any_transparent = false; // some material is transparent: frames trace ray trees
+    uint64_t mat_generation = 0;  // bumped by every eos_update_materials
     int root = 0;
     int root4 = 0;
     int n_nodes4 = 0;
@@ -207,6 +208,9 @@ struct eos_scene {
     hipEvent_t resolved[kFrameSlots] = {}, seq_start = nullptr, seq_ev[kFrameSlots] = {};
     void* host_ctr = nullptr;
     size_t host_ctr_bytes = 0;
+    // Ray queries (eos_trace_rays): staged rays and outputs for host memory,
+    // hit records, fetch cursors and error flag, overflow stacks.
+    eos::DevBuf q_rays, q_out, q_rec, q_ctr, q_gstack;
 };
 
 namespace eos {
@@ -222,6 +226,9 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
                         uint8_t* const* out, eos_batch_stats* st);
 eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_render_opts* o, uint8_t* const* rgba,
                          eos_render_stats* st);
+// Implemented in query.hip
+eos_status trace_rays(eos_scene* s, int n, const float* o, const float* d, const eos_trace_opts* opts,
+                      eos_ray_hit* hits, uint8_t* occ);
 SceneView view_of(const eos_scene* s);
 // One anti-aliasing sample frame: offset inside the pixel and what the resolve
 // does with the sample's colour (src/RayTracer.cpp:214-242).

@@ -122,17 +122,98 @@ int Scene::addTransformation(const eos::mat4& t) {
     return (int)m_Transformations.size() - 1;
 }
 
+namespace {
+cudaScene_t* make_device_scene(const Scene* h);
+}
+
 void Scene::flush() {
     for (Surface* s : m_Surfaces)
         if (!dynamic_cast<Triangle*>(s)) {
             std::fprintf(stderr, "Eos Error: only triangle surfaces are supported on the GPU path\n");
             std::exit(1);
         }
+    m_Device.reset();   // the queries copy the flushed scene to the device when first asked
+}
+
+cudaScene_t* Scene::device() const {
+    if (!m_Device) m_Device.reset(make_device_scene(this), destroyCudaScene);
+    return m_Device.get();
+}
+
+// One eos_trace_rays call for n rays; results into the reference's types.
+void Scene::query(int mode, const Ray* rays, int n, RayIntersection* out, bool* flags) const {
+    if (n <= 0) return;
+    if (!rays || !flags || (mode != EOS_QUERY_OCCLUDED && !out)) {
+        std::fprintf(stderr, "Eos Error: Scene ray query: null argument\n");
+        std::exit(1);
+    }
+    std::vector<float> o(3 * (size_t)n), d(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const eos::vec3 &ro = rays[i].getOrigin(), &rd = rays[i].getDirection();
+        o[3 * (size_t)i] = ro.x; o[3 * (size_t)i + 1] = ro.y; o[3 * (size_t)i + 2] = ro.z;
+        d[3 * (size_t)i] = rd.x; d[3 * (size_t)i + 1] = rd.y; d[3 * (size_t)i + 2] = rd.z;
+    }
+    eos_scene* es = eosSceneOf(device());
+    const eos_trace_opts opts{mode, EOS_MEM_HOST};
+    if (mode == EOS_QUERY_OCCLUDED) {
+        std::vector<uint8_t> occ((size_t)n);
+        check(eos_trace_rays(es, n, o.data(), d.data(), &opts, nullptr, occ.data()), "Scene::shadowRayVisibilityBVH");
+        for (int i = 0; i < n; ++i) flags[i] = occ[(size_t)i] != 0;
+        return;
+    }
+    std::vector<eos_ray_hit> hits((size_t)n);
+    check(eos_trace_rays(es, n, o.data(), d.data(), &opts, hits.data(), nullptr), "Scene ray query");
+    for (int i = 0; i < n; ++i) {
+        const eos_ray_hit& h = hits[(size_t)i];
+        flags[i] = h.hit != 0;
+        if (h.triangle < 0) continue;   // the reference leaves the intersection as it was
+        out[i].setPoint(eos::vec3(h.point[0], h.point[1], h.point[2]));
+        out[i].setNormal(eos::vec3(h.normal[0], h.normal[1], h.normal[2]));
+        out[i].setMaterial(m_Materials[(size_t)h.material]);
+        out[i].setMaterialIndex(h.material);
+    }
+}
+
+bool Scene::findMinDistanceIntersectionBVH(const Ray& ray, RayIntersection& x) const {
+    bool f = false;
+    query(EOS_QUERY_NEAREST, &ray, 1, &x, &f);
+    return f;
+}
+
+bool Scene::shadowRayVisibilityBVH(const Ray& ray) const {
+    bool f = false;
+    query(EOS_QUERY_OCCLUDED, &ray, 1, nullptr, &f);
+    return f;
+}
+
+bool Scene::findMinDistanceIntersectionLinear(const Ray& ray, RayIntersection& x) const {
+    bool f = false;
+    query(EOS_QUERY_NEAREST_LINEAR, &ray, 1, &x, &f);
+    return f;
+}
+
+void Scene::findMinDistanceIntersectionsBVH(const Ray* rays, int n, RayIntersection* x, bool* found) const {
+    query(EOS_QUERY_NEAREST, rays, n, x, found);
+}
+
+void Scene::shadowRayVisibilitiesBVH(const Ray* rays, int n, bool* occluded) const {
+    query(EOS_QUERY_OCCLUDED, rays, n, nullptr, occluded);
+}
+
+void Scene::findMinDistanceIntersectionsLinear(const Ray* rays, int n, RayIntersection* x, bool* found) const {
+    query(EOS_QUERY_NEAREST_LINEAR, rays, n, x, found);
 }
 
 cudaScene_t* createCudaScene(Scene* h) {
     if (!h) return nullptr;
     if (!h->isUsingBvh()) return nullptr;   // src/cudaStructures.cu:500-505
+    return make_device_scene(h);
+}
+
+namespace {
+// The scene's triangles, materials, lights and transformations copied to the
+// device (eos_create_scene, eos_set_transforms) and its LBVH built.
+cudaScene_t* make_device_scene(const Scene* h) {
     const int n = h->getNumSurfaces();
     std::vector<float> verts(9 * (size_t)n), normals(9 * (size_t)n);
     std::vector<int32_t> mat((size_t)n), xform((size_t)n, -1);
@@ -210,6 +291,7 @@ cudaScene_t* createCudaScene(Scene* h) {
     check(eos_build_lbvh(out->scene, 4, nullptr), "createCudaScene (LBVH build)");
     return out;
 }
+}  // namespace
 
 void destroyCudaScene(cudaScene_t* s) {
     if (!s) return;

@@ -47,12 +47,14 @@ bool parse_corner(const char* tok, int nv, int nn, int& vi, int& ni) {
             ++p;
             long c = std::strtol(p, &end, 10);
             if (end == p) return false;
+            // range-checked as long, before the narrowing (1-based, or -1 for the last)
+            if (c == 0 || c > (long)nn || c < -(long)nn) return false;
             ni = c > 0 ? (int)(c - 1) : (int)(nn + c);
-            if (ni < 0 || ni >= nn) return false;
         }
     }
+    if (a == 0 || a > (long)nv || a < -(long)nv) return false;
     vi = a > 0 ? (int)(a - 1) : (int)(nv + a);
-    return vi >= 0 && vi < nv;
+    return true;
 }
 
 }  // namespace

@@ -68,11 +68,9 @@ __device__ __forceinline__ float fresnel(F3 incident, F3 normal, float n1, float
     return (rOrth * rOrth + rPar * rPar) / 2.0f;
 }
 
-// src/Triangle.cpp:41-47 and src/BVH.cpp:362-381: point o + t*d and normal
-// normalize(n1*(1-u-v) + n2*u + n3*v) in the triangle's local space (the ray
-// through the inverse), then the point through M and the normal through
-// transpose(inverse), normalised again.  Without a transformation the point is
-// the world one and the second normalisation repeats the first.
+// Hit set-up of one level: the hit surface (hit_surface, the reference's
+// point and normal), the pixel's level count, the reflection ray and the
+// slots of the next stages.
 template <bool L0, bool BATCH>
 __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
     const int n = L0 ? a.n_ids : *a.count_ptr;
@@ -133,22 +131,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
         bool refl = false;
         if (hit) {
             const float4 s0 = shade[3 * (size_t)slot], s1 = shade[3 * (size_t)slot + 1], s2 = shade[3 * (size_t)slot + 2];
-            const int xi = __float_as_int(s1.w);
-            const float4* X = xi >= 0 ? xf + kXfRows * (size_t)xi : nullptr;
-            const F3 lo = X ? xform3(X + kXfInv, o, 1.0f) : o;
-            const F3 ld = X ? xform3(X + kXfInv, d, 0.0f) : d;
-            P = add(lo, scale(h.x, ld));
-            const float u = h.z, v = h.w;
-            const float w = (1.0f - u) - v;
-            const F3 na = {s0.x * w, s0.y * w, s0.z * w};
-            const F3 nb = {s1.x * u, s1.y * u, s1.z * u};
-            const F3 nc = {s2.x * v, s2.y * v, s2.z * v};
-            N = normalize3(add(add(na, nb), nc));
-            if (X) {
-                P = xform3(X + kXfM, P, 1.0f);
-                N = xform3(X + kXfInvT, N, 0.0f);
-            }
-            N = normalize3(N);
+            hit_surface(s0, s1, s2, xf, o, d, h.x, h.z, h.w, true, P, N);
             mat = __float_as_int(s0.w);
             refl = mats[mat].reflective != 0 && a.level < a.depth;
         }
@@ -721,7 +704,7 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             sa.rayhit = fb.rayhit.as<int32_t>() + fb.toff[l];
             sa.aux = fb.aux.as<float4>() + fb.toff[l];
         }
-        sa.shadow_ctr = ctr + CTR_SHADOW;
+        sa.shadow_ctr = ctr + CTR_SHADOW + l;
         sa.levels = fb.levels.as<uint8_t>();
         sa.hit_ids = l == 0 ? f.hits : nullptr;
         F.u[2 * l] = rec(S);
@@ -817,7 +800,7 @@ eos_status frame_stats(const Frame& f, const int* hctr, const unsigned long long
         st->hits += hctr[CTR_HITS + l];
         if (l >= 1) st->secondary_rays += hctr[CTR_Q + l];
     }
-    st->shadow_rays = hctr[CTR_SHADOW];
+    for (int l = 0; l <= f.depth; ++l) st->shadow_rays += hctr[CTR_SHADOW + l];
     if (E && F) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, E[F->start], E[F->end]);
@@ -1099,6 +1082,7 @@ struct BatchState {
     DevBuf table, rgba;
     std::vector<BatchScene> host_table;
     std::vector<eos_scene*> scenes;
+    std::vector<uint64_t> mat_gens;   // each scene's material generation when the frame was shaped
     std::vector<eos_camera> cams;
     int depth = -1;
     LaunchCfg cfg;
@@ -1118,6 +1102,9 @@ bool same_batch(eos_scene* const* scenes, const eos_camera* cams, int n, int dep
         return false;
     for (int i = 0; i < n; ++i) {
         if (b.scenes[i] != scenes[i]) return false;
+        // material flags (reflective, transparent) shape the frame: a ray-tree or
+        // chain schedule and its buffers, captured into the graph
+        if (b.mat_gens[i] != scenes[i]->mat_generation) return false;
         if (std::memcmp(&b.cams[i], &cams[i], sizeof(eos_camera)) != 0) return false;
         const SceneView v = view_of(scenes[i]);   // rebuilt BVH or changed light count
         if (std::memcmp(&v, &b.host_table[i].sc, sizeof(SceneView)) != 0) return false;
@@ -1145,6 +1132,7 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
             (void)hipGraphExecDestroy(b.exec);
             b.exec = nullptr;
         }
+        b.scenes.clear();   // a failure below leaves nothing that a later call could take as the same batch
         b.host_table.assign(n, BatchScene{});
         size_t ids = 0, pix = 0;
         int lmax = 0;
@@ -1193,6 +1181,8 @@ eos_status render_batch(eos_scene* const* scenes, const eos_camera* cams, int n,
         const int trace_grid = std::max(1, f.num_cu * f.cfg.waves_per_cu / (f.cfg.workgroup / 64));
         EOS_TRY(ensure_buffers(b.fb, f, trace_grid));
         b.scenes.assign(scenes, scenes + n);
+        b.mat_gens.resize(n);
+        for (int i = 0; i < n; ++i) b.mat_gens[i] = scenes[i]->mat_generation;
         b.cams.assign(cams, cams + n);
         b.depth = o->depth;
         b.cfg = cfg;

@@ -104,6 +104,33 @@ __device__ __forceinline__ void to_local(const float4* xf, int xi, F3& o, F3& d)
     }
 }
 
+// src/Triangle.cpp:41-47 and src/BVH.cpp:362-381: point o + t*d and normal
+// normalize(n1*(1-u-v) + n2*u + n3*v) in the triangle's local space (the ray
+// through the inverse), then the point through M and the normal through
+// transpose(inverse), normalised again (renorm) as the BVH query does;
+// Scene::findMinDistanceIntersectionLinear (src/Scene.cpp:159-201) leaves the
+// transformed normal as it is (renorm false).  Without a transformation the
+// point is the world one and the second normalisation repeats the first.
+// s0..s2: the triangle's shading record (n1.xyz mat | n2.xyz xf | n3.xyz).
+__device__ __forceinline__ void hit_surface(float4 s0, float4 s1, float4 s2, const float4* xf, F3 o, F3 d, float t,
+                                            float u, float v, bool renorm, F3& P, F3& N) {
+    const int xi = __float_as_int(s1.w);
+    const float4* X = xi >= 0 ? xf + kXfRows * (size_t)xi : nullptr;
+    const F3 lo = X ? xform3(X + kXfInv, o, 1.0f) : o;
+    const F3 ld = X ? xform3(X + kXfInv, d, 0.0f) : d;
+    P = add(lo, scale(t, ld));
+    const float w = (1.0f - u) - v;
+    const F3 na = {s0.x * w, s0.y * w, s0.z * w};
+    const F3 nb = {s1.x * u, s1.y * u, s1.z * u};
+    const F3 nc = {s2.x * v, s2.y * v, s2.z * v};
+    N = normalize3(add(add(na, nb), nc));
+    if (X) {
+        P = xform3(X + kXfM, P, 1.0f);
+        N = xform3(X + kXfInvT, N, 0.0f);
+    }
+    if (renorm) N = normalize3(N);
+}
+
 __device__ __forceinline__ float safe_inv(float d) {
     float x = fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
     return 1.0f / x;
@@ -151,9 +178,13 @@ enum : int {
     CTR_Q = 0,         // + level: rays queued for level (level >= 1)
     CTR_HITS = 16,     // + level: hits at level
     CTR_ERR = 64,      // stack overflow / invalid state
-    CTR_SHADOW = 65,   // shadow rays (sum over hits of the hit scene's light count)
+    CTR_SHADOW = 65,   // + level: shadow rays of level (sum over its hits of the hit scene's light count);
+                       // one level's count stays below 2^31, the frame's total is summed on the host in 64 bits
     CTR_COUNT = 128
 };
+static_assert(CTR_Q + EOS_MAX_DEPTH + 1 < CTR_HITS && CTR_HITS + EOS_MAX_DEPTH < CTR_ERR &&
+                  CTR_SHADOW + EOS_MAX_DEPTH < CTR_COUNT,
+              "per-level counters overlap");
 // int64 statistics (count_visits): [2*launch] visits, [2*launch+1] tests;
 // launch = level (closest) or 16 + level (any hit).
 constexpr int kStats = 64;
@@ -169,7 +200,12 @@ constexpr int kCursorBase = CTR_COUNT + 2 * kStats;       // int offset of the c
 constexpr int kCursorInts = 32 * kRanges * kCursorStride; // 32 launches
 constexpr int kCounterInts = kCursorBase + kCursorInts;
 
-enum TraceMode : int { MODE_PRIMARY = 0, MODE_QUEUE = 1, MODE_SHADOW = 2 };
+// MODE_RAYS / MODE_RAYS_ANY: caller-given rays (origins and directions, [n][3]
+// floats each), nearest hit or occlusion, for the ray queries.
+enum TraceMode : int { MODE_PRIMARY = 0, MODE_QUEUE = 1, MODE_SHADOW = 2, MODE_RAYS = 3, MODE_RAYS_ANY = 4 };
+
+// Any-hit modes: a ray ends at its first hit, whatever its distance.
+__host__ __device__ constexpr bool any_hit_mode(int mode) { return mode == MODE_SHADOW || mode == MODE_RAYS_ANY; }
 
 // One scene of a batched frame (device table, one entry per scene).
 struct BatchScene {
@@ -211,9 +247,11 @@ struct TraceArgs {
     int count_mul;           // items per counted entry (light slots for shadow rays)
     int* cursor;             // fetch cursors of the launch: range r at cursor[r * kCursorStride]
     const float4* q;         // MODE_QUEUE: ray queue
+    const float* ray_o;      // MODE_RAYS(_ANY): origins [n][3]
+    const float* ray_d;      // MODE_RAYS(_ANY): directions [n][3]
     float4* hit_out;         // closest: (t, slot, u, v)
     const float4* surf;      // MODE_SHADOW: hit surfaces (P.xyz pixel | N.xyz mat) of the first level
-    uint8_t* vis_out;        // MODE_SHADOW: 1 = light visible, first level
+    uint8_t* vis_out;        // MODE_SHADOW: 1 = light visible, first level; MODE_RAYS_ANY: 1 = occluded
     int nlev;                // MODE_SHADOW: consecutive levels in this launch (count_ptr[0..nlev))
     long long surf_stride;   // MODE_SHADOW: float4 between levels' surfaces
     long long vis_stride;    // MODE_SHADOW: bytes between levels' visibility
@@ -318,7 +356,7 @@ struct RayItem {
     const Node4* nodes4;
     const float4* geo;
     const float4* xf;        // transformations of the scene (null: none)
-    size_t out_idx;          // MODE_SHADOW: visibility byte of the item
+    size_t out_idx;          // any hit: result byte of the item
 };
 
 template <int MODE, bool BATCH, int WIDTH>
@@ -357,6 +395,12 @@ __device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& 
         r.o = f3(q0.x, q0.y, q0.z);
         r.d = f3(q1.x, q1.y, q1.z);
         if (BATCH) use_scene(a.table[__float_as_int(q1.w)]);
+    } else if (MODE == MODE_RAYS || MODE == MODE_RAYS_ANY) {
+        const float* po = a.ray_o + 3 * (size_t)my;
+        const float* pd = a.ray_d + 3 * (size_t)my;
+        r.o = f3(po[0], po[1], po[2]);
+        r.d = f3(pd[0], pd[1], pd[2]);
+        r.out_idx = (size_t)my;
     } else {
         // item -> (level, surface j, light slot li)
         int rem = my, lev = 0;
@@ -417,7 +461,7 @@ __device__ __forceinline__ bool use_coop(int mode, long long n, long long lanes)
 template <int MODE, int WG, bool COUNT, bool BATCH, bool XF>
 __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* wave_stack, unsigned long long& visits,
                                            unsigned long long& tests) {
-    constexpr bool ANY = MODE == MODE_SHADOW;
+    constexpr bool ANY = any_hit_mode(MODE);
     const int lane = __lane_id();
     const int sub = lane & 3, grp = lane >> 2;
     const int gbit = 4 * grp;
@@ -573,7 +617,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
                 }
                 if (cur == kDone) {
                     if (sub == 0) {
-                        if (ANY) a.vis_out[out_idx] = best_slot >= 0 ? 0 : 1;
+                        if (ANY) a.vis_out[out_idx] = (best_slot >= 0) == (MODE == MODE_RAYS_ANY) ? 1 : 0;
                         else a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
                     }
                     item = -1;
@@ -599,7 +643,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
 // triangle's local space.
 template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH, int WIDTH, bool XF>
 __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
-    constexpr bool ANY = MODE == MODE_SHADOW;
+    constexpr bool ANY = any_hit_mode(MODE);
     constexpr int MAXS = max_stack<WIDTH>();
     constexpr int LDS_LANE = STACK_LDS * WG, LDS_COOP = WIDTH == 4 ? (WG / 4) * kMaxStack4 : 0;
     __shared__ int32_t lds[LDS_LANE > LDS_COOP ? LDS_LANE : LDS_COOP];
@@ -835,7 +879,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     }
                     if (cur == kDone) {
                         if (ANY) {
-                            a.vis_out[out_idx] = best_slot >= 0 ? 0 : 1;
+                            a.vis_out[out_idx] = (best_slot >= 0) == (MODE == MODE_RAYS_ANY) ? 1 : 0;
                         } else {
                             a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
                         }

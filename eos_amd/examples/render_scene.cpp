@@ -5,6 +5,7 @@
 // route (createCudaScene + DeviceCameraHandler + DeviceRenderer).
 //
 // usage: render_scene <scene.bin> <out_multi.rgba> <out_single.rgba> [mesh.obj mesh_material [tx ty tz]]
+// (environment: EOS_AA, EOS_BANDS and EOS_QUERIES add the routes described below)
 // With a mesh, a TriangleMesh read from the OBJ file is added after the
 // scene's own triangles, the way the reference's driver adds its OBJ model
 // (src/main.cpp:386-403): a translation added with Scene::addTransformation
@@ -15,6 +16,7 @@
 // float32 camera position, view, right, up (12 floats).
 #include <algorithm>
 #include <cstdio>
+#include <memory>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -175,6 +177,58 @@ int main(int argc, char** argv) {
         o = std::fopen((std::string(argv[3]) + ".bands").c_str(), "wb");
         std::fwrite(image4.data(), 1, image4.size(), o);
         std::fclose(o);
+    }
+    // the Scene's ray queries (src/Scene.h:69-71), one ray per call as the
+    // reference's renderer makes them, then the batched forms: EOS_QUERIES names
+    // a file of int32 n, float origins[n][3], directions[n][3]; per ray,
+    // <file>.out gets int32 found, float point[3], normal[3], int32 material
+    // for the BVH query, int32 occluded, then the same four for the linear query,
+    // and <file>.batch the same from the batched calls
+    if (const char* qf = std::getenv("EOS_QUERIES")) {
+        FILE* q = std::fopen(qf, "rb");
+        if (!q) return 2;
+        int32_t nq = 0;
+        rd(q, &nq, 1);
+        std::vector<float> qo(3 * (size_t)nq), qd(3 * (size_t)nq);
+        rd(q, qo.data(), qo.size());
+        rd(q, qd.data(), qd.size());
+        std::fclose(q);
+        std::vector<Ray> rays;
+        for (int i = 0; i < nq; ++i)
+            rays.emplace_back(eos::vec3(qo[3 * i], qo[3 * i + 1], qo[3 * i + 2]),
+                              eos::vec3(qd[3 * i], qd[3 * i + 1], qd[3 * i + 2]));
+        auto put = [](FILE* f, bool found, const RayIntersection& x) {
+            const int32_t fl = found ? 1 : 0, m = found ? x.getMaterialIndex() : -1;
+            const float pn[6] = {x.getPoint().x, x.getPoint().y, x.getPoint().z,
+                                 x.getNormal().x, x.getNormal().y, x.getNormal().z};
+            std::fwrite(&fl, 4, 1, f);
+            std::fwrite(pn, 4, 6, f);
+            std::fwrite(&m, 4, 1, f);
+        };
+        FILE* out = std::fopen((std::string(qf) + ".out").c_str(), "wb");
+        for (const Ray& r : rays) {
+            RayIntersection x, y;
+            const bool f1 = scene.findMinDistanceIntersectionBVH(r, x);
+            put(out, f1, x);
+            const int32_t occ = scene.shadowRayVisibilityBVH(r) ? 1 : 0;
+            std::fwrite(&occ, 4, 1, out);
+            const bool f2 = scene.findMinDistanceIntersectionLinear(r, y);
+            put(out, f2, y);
+        }
+        std::fclose(out);
+        std::vector<RayIntersection> xb((size_t)nq), yb((size_t)nq);
+        std::unique_ptr<bool[]> fb(new bool[(size_t)nq]), ob(new bool[(size_t)nq]), lb(new bool[(size_t)nq]);
+        scene.findMinDistanceIntersectionsBVH(rays.data(), nq, xb.data(), fb.get());
+        scene.shadowRayVisibilitiesBVH(rays.data(), nq, ob.get());
+        scene.findMinDistanceIntersectionsLinear(rays.data(), nq, yb.data(), lb.get());
+        out = std::fopen((std::string(qf) + ".batch").c_str(), "wb");
+        for (int i = 0; i < nq; ++i) {
+            put(out, fb[i], xb[(size_t)i]);
+            const int32_t occ = ob[i] ? 1 : 0;
+            std::fwrite(&occ, 4, 1, out);
+            put(out, lb[i], yb[(size_t)i]);
+        }
+        std::fclose(out);
     }
     for (Triangle* t : tris) delete t;
     for (LightSource* l : ls) delete l;

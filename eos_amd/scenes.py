@@ -577,6 +577,49 @@ def glass(width: int = 512, height: int = 384, level: int = 3, depth: int = 3) -
     return s
 
 
+def translation(x: float, y: float, z: float) -> np.ndarray:
+    """glm::translate(glm::mat4(1.0f), vec3(x, y, z)) as a column-major float32 [16]."""
+    m = np.eye(4, dtype=np.float32)
+    m[3, :3] = (x, y, z)          # column 3 of the column-major matrix
+    return m.ravel()
+
+
+def driver_scene(verts: np.ndarray, normals: np.ndarray, width: int = 1280, height: int = 720) -> SceneData:
+    """The reference driver's scene (src/main.cpp:330-411, rendered at :454-481):
+    one OBJ mesh (the driver loads objmodels/bunny.obj) placed by
+    glm::translate(2, -2, -2) through Scene::addTransformation; material
+    kd (0.87, 0.3, 0.5, 0), ks 1, shininess 120, neither reflective nor
+    transparent; lights at (20, 10, 10) and (-20, 10, 10), colour 1; camera at
+    (2, 0, 5) looking down -z with unit right and up vectors (no field of view:
+    90 degrees both ways), 1280 x 720; air outside (1.0002); the trace depth
+    RayTracer and the renderers default to (4, src/RayTracer.h:36)."""
+    n = len(verts)
+    mat = make_material((0.87, 0.3, 0.5, 0.0), (1.0, 1.0, 1.0, 1.0), 120, 0.4, False, False, 1.33)
+    lights = make_lights([(20.0, 10.0, 10.0), (-20.0, 10.0, 10.0)])
+    cam = np.zeros((), CAMERA_DTYPE)
+    cam["position"] = (2.0, 0.0, 5.0)
+    cam["view"] = (0.0, 0.0, -1.0)
+    cam["right"] = (1.0, 0.0, 0.0)
+    cam["up"] = (0.0, 1.0, 0.0)
+    cam["width"] = width
+    cam["height"] = height
+    s = SceneData(np.ascontiguousarray(verts, np.float32), np.ascontiguousarray(normals, np.float32),
+                  np.zeros(n, np.int32), mat[None], lights, cam, 4, "reference driver scene (bunny)",
+                  {}, translation(2.0, -2.0, -2.0)[None], np.zeros(n, np.int32))
+    s.ambient_index = 1.0002
+    return s
+
+
+def write_indexed_obj(path: str, positions: np.ndarray, faces: np.ndarray) -> None:
+    """Writes `v` positions (9 significant digits: float32 values read back
+    exactly) and 1-based triangle `f` lines, no normals."""
+    p = np.asarray(positions, np.float32).reshape(-1, 3)
+    f = np.asarray(faces, np.int64).reshape(-1, 3) + 1
+    with open(path, "w") as out:
+        out.writelines(f"v {x:.9g} {y:.9g} {z:.9g}\n" for x, y, z in p)
+        out.writelines(f"f {a} {b} {c}\n" for a, b, c in f)
+
+
 def perturb_params(scene: SceneData, seed: int, rel: float = 0.10):
     """Materials and lights with every parameter scaled by a seeded factor in [1-rel, 1+rel]."""
     rng = Rng(seed)

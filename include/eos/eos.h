@@ -8,7 +8,9 @@
  *   LightSource          src/LightSource.h:27-39
  *   Surface, Triangle    src/surface.h:31-47, src/Triangle.h:32-99
  *   TriangleMesh         src/triangleMesh.h:30-80 (native OBJ reader instead of assimp)
- *   Scene                src/Scene.h:42-118
+ *   Scene                src/Scene.h:42-118 (with its ray queries, src/Scene.h:69-71)
+ *   Ray                  src/Ray.h:31-65
+ *   RayIntersection      src/RayIntersection.h:29-56
  *   cudaScene_t, createCudaScene            src/cudaStructures.h:87-98
  *   DeviceCameraHandler  src/DeviceCameraHandler.h:29-40
  *   DeviceRenderer       src/DeviceRenderer.h:33-56
@@ -28,6 +30,7 @@
 
 #include <stdint.h>
 
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -206,6 +209,62 @@ private:
     int m_SceneIndexStart = 0, m_SceneIndexEnd = 0;
 };
 
+/* src/Ray.h:31-65: origin, direction, the inverse direction (double-precision
+ * reciprocals in the constructor, float for x and y in setDirection) and the
+ * signs of the inverse.  The GPU queries use the origin and the direction. */
+class Ray {
+public:
+    Ray() : m_InvDirection(0.0f) {}
+    Ray(const eos::vec3& origin, const eos::vec3& direction) : m_Origin(origin), m_Direction(direction) {
+        m_InvDirection = eos::vec3((float)(1.0 / direction.x), (float)(1.0 / direction.y), (float)(1.0 / direction.z));
+        set_signs();
+    }
+    void setOrigin(const eos::vec3& origin) { m_Origin = origin; }
+    void setDirection(const eos::vec3& direction) {
+        m_Direction = direction;
+        m_InvDirection = eos::vec3(1.0f / direction.x, 1.0f / direction.y, (float)(1.0 / direction.z));
+        set_signs();
+    }
+    const eos::vec3& getOrigin() const { return m_Origin; }
+    const eos::vec3& getDirection() const { return m_Direction; }
+    const eos::vec3& getInvDirection() const { return m_InvDirection; }
+
+    int m_sign[3] = {0, 0, 0};
+
+private:
+    void set_signs() {
+        m_sign[0] = m_InvDirection.x < 0.0f;
+        m_sign[1] = m_InvDirection.y < 0.0f;
+        m_sign[2] = m_InvDirection.z < 0.0f;
+    }
+    eos::vec3 m_Origin, m_Direction, m_InvDirection;
+};
+
+/* src/RayIntersection.h:29-56: the nearest hit's world-space point and normal,
+ * its material and material index. */
+class RayIntersection {
+public:
+    RayIntersection() : m_Point(0.0f), m_Normal(0.0f) {}
+    RayIntersection(const eos::vec3& point, const eos::vec3& normal) : m_Point(point), m_Normal(normal) {}
+    RayIntersection(const eos::vec3& point, const eos::vec3& normal, const Material& material)
+        : m_Point(point), m_Normal(normal), m_Material(material) {}
+    void setPoint(const eos::vec3& point) { m_Point = point; }
+    void setNormal(const eos::vec3& normal) { m_Normal = normal; }
+    void setMaterial(const Material& material) { m_Material = material; }
+    void setMaterialIndex(int idx) { m_MaterialIndex = idx; }
+    const eos::vec3& getPoint() const { return m_Point; }
+    const eos::vec3& getNormal() const { return m_Normal; }
+    const Material& getMaterial() const { return m_Material; }
+    int getMaterialIndex() const { return m_MaterialIndex; }
+
+private:
+    eos::vec3 m_Point, m_Normal;
+    Material m_Material;
+    int m_MaterialIndex = -1;
+};
+
+struct cudaScene_t;
+
 class Scene {
 public:
     bool addSurface(Surface* surface);
@@ -230,11 +289,32 @@ public:
     int getAASamples() { return m_AASamples; }
     void useBvh(bool use) { m_UsingBvh = use; }
     bool isUsingBvh() const { return m_UsingBvh; }
-    /* The reference builds its CPU BVH here (src/Scene.cpp:204-209); the GPU path
-     * builds its LBVH on the device in createCudaScene, so this only validates. */
+    /* The reference builds its CPU BVH here (src/Scene.cpp:204-209).  Here the
+     * surfaces are checked and the ray queries' device copy is dropped;
+     * createCudaScene makes its own copy for rendering. */
     void flush();
 
+    /* The reference's per-ray queries (src/Scene.h:69-71, src/Scene.cpp:145-201),
+     * answered on the GPU (eos_trace_rays) by a device copy of the scene with its
+     * LBVH, made at the first query after flush().  Each call is one launch and
+     * one synchronisation; the batched forms below take many rays at once. */
+    bool findMinDistanceIntersectionBVH(const Ray& ray, RayIntersection& intersection) const;
+    bool shadowRayVisibilityBVH(const Ray& ray) const;
+    /* Every triangle in the order they were added (the reference walks its
+     * surfaces in the order its BVH build left them, which differs only when
+     * two triangles give the same least distance). */
+    bool findMinDistanceIntersectionLinear(const Ray& ray, RayIntersection& intersection) const;
+    /* Not in the reference: n rays in one launch; found[i] / occluded[i] are the
+     * per-ray queries' return values. */
+    void findMinDistanceIntersectionsBVH(const Ray* rays, int n, RayIntersection* intersections, bool* found) const;
+    void shadowRayVisibilitiesBVH(const Ray* rays, int n, bool* occluded) const;
+    void findMinDistanceIntersectionsLinear(const Ray* rays, int n, RayIntersection* intersections,
+                                            bool* found) const;
+
 private:
+    void query(int mode, const Ray* rays, int n, RayIntersection* intersections, bool* flags) const;
+    cudaScene_t* device() const;
+
     std::vector<Surface*> m_Surfaces;
     std::vector<LightSource*> m_Lights;
     std::vector<Material> m_Materials;
@@ -243,11 +323,11 @@ private:
     float m_AmbientRefractiveIndex = 1.0f;
     bool m_UsingBvh = false;
     int m_AASamples = 1;
+    mutable std::shared_ptr<cudaScene_t> m_Device;   // the queries' device copy (copies of the Scene share it)
 };
 
 /* Opaque device scene (the reference returns a device pointer that must not be
  * dereferenced on the host; this one is a host handle with the same rule). */
-struct cudaScene_t;
 cudaScene_t* createCudaScene(Scene* h_scene);
 /* Not in the reference (it never frees the device scene). */
 void destroyCudaScene(cudaScene_t* scene);

@@ -256,6 +256,46 @@ typedef struct eos_batch_stats {
 eos_status eos_render_batch(eos_scene* const* scenes, const eos_camera* cameras, int32_t n,
                             const eos_batch_opts* opts, uint8_t* const* rgba_out, eos_batch_stats* stats);
 
+/* ---- ray queries ----------------------------------------------------------- */
+
+/* Which query eos_trace_rays answers (src/Scene.h:69-71). */
+enum {
+    /* Scene::findMinDistanceIntersectionBVH (src/Scene.cpp:145-153, BVH::intersectStackNearest
+     * src/BVH.cpp:609-659): the nearest hit at t >= 0 */
+    EOS_QUERY_NEAREST = 0,
+    /* Scene::shadowRayVisibilityBVH (src/Scene.cpp:155-158, BVH::intersectStackVisibility
+     * src/BVH.cpp:661-716): 1 if any triangle is hit at t >= 0, whatever its distance */
+    EOS_QUERY_OCCLUDED = 1,
+    /* Scene::findMinDistanceIntersectionLinear (src/Scene.cpp:159-201): every triangle in index
+     * order, the first with the least t in (0, 999999); its normal is not normalised again after
+     * the transformation */
+    EOS_QUERY_NEAREST_LINEAR = 2
+};
+
+/* One ray's nearest hit: the fields of RayIntersection (src/RayIntersection.h:29-56) in world
+ * space, plus what the reference keeps to itself.  48 bytes. */
+typedef struct eos_ray_hit {
+    float point[3];           /* RayIntersection::getPoint */
+    int32_t hit;              /* the query's return value: 1 found, 0 not */
+    float normal[3];          /* RayIntersection::getNormal */
+    int32_t material;         /* RayIntersection::getMaterialIndex (-1: no hit) */
+    float t, u, v;            /* ray parameter and barycentric coordinates of the hit */
+    int32_t triangle;         /* original index of the hit triangle (-1: none) */
+} eos_ray_hit;
+
+typedef struct eos_trace_opts {
+    int32_t mode;             /* EOS_QUERY_* */
+    int32_t memory;           /* EOS_MEM_HOST / EOS_MEM_DEVICE for origins, directions and outputs */
+} eos_trace_opts;
+
+/* n arbitrary rays against the scene: origins and directions [n][3] (any length; t is in units of
+ * the direction).  Nearest modes write hits[n], EOS_QUERY_OCCLUDED writes occluded[n] (1 =
+ * occluded); the other output may be NULL.  Replaces the per-ray Scene queries above
+ * (src/Scene.h:69-71) with one GPU launch for all rays.  Needs eos_build_lbvh (all modes).
+ * Synchronous. */
+eos_status eos_trace_rays(eos_scene* scene, int32_t n, const float* origins, const float* directions,
+                          const eos_trace_opts* opts, eos_ray_hit* hits, uint8_t* occluded);
+
 /* ---- mesh files ------------------------------------------------------------ */
 
 /* Triangles read from a Wavefront OBJ file, in the layout of eos_scene_desc

@@ -83,6 +83,13 @@ void orc_nearest(const orc_scene* s, const void* accel, int32_t n, const float* 
                  int32_t any_hit, int32_t* hit_tri, float* hit_t, float* hit_u, float* hit_v, float* point,
                  float* normal);
 
+/* Scene::findMinDistanceIntersectionLinear (src/Scene.cpp:159-201) for n rays:
+ * found flag (its return value), kept triangle (original index or -1), t, the
+ * world-space point and the transformed (not renormalised) normal.  Outputs
+ * other than found / hit_tri may be NULL. */
+void orc_nearest_linear(const orc_scene* s, int32_t n, const float* orig, const float* dirs, int32_t* found,
+                        int32_t* hit_tri, float* hit_t, float* point, float* normal);
+
 /* Reference-semantics render of rows row_start, row_start+row_step, ...
  * Any output pointer may be NULL.  rgba: [H*W*4]; rgb: [H*W*3] pre-quantisation;
  * hit_ids: [H*W] primary hit (original index or -1); recs: [(depth+1)*H*W]. */

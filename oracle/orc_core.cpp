@@ -301,7 +301,7 @@ bool anyhit(const Accel& A, const orc_scene& s, V3 o, V3 d) {
 // normalised again.  Without a transformation these are the identity (the
 // point and the first normalisation pass through unchanged).
 inline void hit_geometry(const orc_scene& s, int tri, V3 o, V3 d, float t, float u, float v,
-                         V3& P, V3& N) {
+                         V3& P, V3& N, bool renorm = true) {
     const int xi = xform_of(s, tri);
     float inv[16], invT[16];
     if (xi >= 0) {
@@ -322,7 +322,7 @@ inline void hit_geometry(const orc_scene& s, int tri, V3 o, V3 d, float t, float
         P = xform(s.xforms + 16 * (size_t)xi, P, 1.0f);
         N = xform(invT, N, 0.0f);
     }
-    N = normalize(N);
+    if (renorm) N = normalize(N);
 }
 
 // World-space vertices of the triangles (through M, w = 1) for bounding
@@ -651,6 +651,51 @@ void orc_nearest(const orc_scene* s, const void* accel, int32_t n, const float* 
             point[3 * (size_t)i] = P.x; point[3 * (size_t)i + 1] = P.y; point[3 * (size_t)i + 2] = P.z;
             normal[3 * (size_t)i] = N.x; normal[3 * (size_t)i + 1] = N.y; normal[3 * (size_t)i + 2] = N.z;
         }
+    }
+}
+
+// src/Scene.cpp:159-201, Scene::findMinDistanceIntersectionLinear: every
+// triangle in index order, tested in its local space through Triangle::hit
+// (src/Triangle.cpp:26-55) with `distance` = minTemp, so a hit farther than the
+// last accepted one is rejected (the first is accepted: its start value is
+// uninitialised, read as no bound); an accepted hit is kept when
+// 0 < t < minDist (999999 at first).  found = some hit was accepted.  The kept
+// hit's point goes through M and its normal through transpose(inverse) without
+// a second normalisation.
+void orc_nearest_linear(const orc_scene* s, int32_t n, const float* orig, const float* dirs, int32_t* found,
+                        int32_t* hit_tri, float* hit_t, float* point, float* normal) {
+    // glm::inverse of each transformation, once (the reference recomputes the same values per test)
+    std::vector<float> inv(16 * (size_t)(s->xform_index ? s->n_xform : 0));
+    for (size_t x = 0; x < inv.size() / 16; ++x) inverse_glm(s->xforms + 16 * x, &inv[16 * x]);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; ++i) {
+        const V3 o = ld3(orig + 3 * (size_t)i), d = ld3(dirs + 3 * (size_t)i);
+        float min_temp = INFINITY, min_dist = 999999.0f, bu = 0.0f, bv = 0.0f;
+        int best = -1;
+        bool any = false;
+        for (int k = 0; k < s->n_tri; ++k) {
+            float t, u, v;
+            const int xi = xform_of(*s, k);
+            const bool hit = xi < 0 ? mt(o, d, s->verts + 9 * (size_t)k, t, u, v)
+                                    : mt(xform(&inv[16 * (size_t)xi], o, 1.0f), xform(&inv[16 * (size_t)xi], d, 0.0f),
+                                         s->verts + 9 * (size_t)k, t, u, v);
+            if (!hit || t > min_temp) continue;
+            any = true;
+            min_temp = t;
+            if (t < min_dist && t > 0.0f) {
+                min_dist = t;
+                best = k;
+                bu = u;
+                bv = v;
+            }
+        }
+        found[i] = any ? 1 : 0;
+        hit_tri[i] = best;
+        if (hit_t) hit_t[i] = best >= 0 ? min_dist : INFINITY;
+        V3 P = {0, 0, 0}, N = {0, 0, 0};
+        if (best >= 0) hit_geometry(*s, best, o, d, min_dist, bu, bv, P, N, false);
+        if (point) { point[3 * (size_t)i] = P.x; point[3 * (size_t)i + 1] = P.y; point[3 * (size_t)i + 2] = P.z; }
+        if (normal) { normal[3 * (size_t)i] = N.x; normal[3 * (size_t)i + 1] = N.y; normal[3 * (size_t)i + 2] = N.z; }
     }
 }
 

@@ -79,6 +79,7 @@ def lib():
         L.orc_accel_free.argtypes = [vp]
         L.orc_nearest.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp]
         L.orc_render.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.orc_nearest_linear.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp]
         L.orc_render_lbvh.argtypes = [vp, vp, i32, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, i32]
         L.orc_mt.argtypes = [vp, vp, vp, vp]
         L.orc_render_aa.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp]
@@ -147,6 +148,19 @@ class Oracle:
         if any_hit:
             return tri.astype(bool)
         return {"tri": tri, "t": t, "u": u, "v": v, "point": P, "normal": N}
+
+    def nearest_linear(self, origins, dirs):
+        """Scene::findMinDistanceIntersectionLinear: found flag, kept triangle, t, point, normal."""
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[0]
+        found = np.zeros(n, np.int32)
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        P = np.zeros((n, 3), np.float32)
+        N = np.zeros((n, 3), np.float32)
+        lib().orc_nearest_linear(C.byref(self._s), n, _p(o), _p(d), _p(found), _p(tri), _p(t), _p(P), _p(N))
+        return {"found": found.astype(bool), "tri": tri, "t": t, "point": P, "normal": N}
 
     def render_aa(self, n, camera=None, depth=None):
         """renderWithAA: n x n samples per pixel."""

@@ -56,10 +56,13 @@ REF_API void ref_mat4_inverse(const float* m, float* out) {
 // xform_index [n_tri] (-1 = identity), each given to its triangles the way
 // TriangleMesh::setTransformation does (src/triangleMesh.cpp:246-250: the
 // inverse, and the transpose of the inverse).
-REF_API void* ref_scene_create_xf(int32_t n_tri, const float* verts, const float* normals,
-                                  const int32_t* mat, int32_t n_mat, const void* mats_raw,
-                                  int32_t n_light, const void* lights_raw, int32_t build_threads,
-                                  int32_t n_xform, const float* xforms, const int32_t* xform_index) {
+// use_bvh = 0: Scene::useBvh(false) and no flush, so RayTracer and the
+// queries take the linear search (src/RayTracer.cpp:281-285,334-337).
+REF_API void* ref_scene_create_opts(int32_t n_tri, const float* verts, const float* normals,
+                                    const int32_t* mat, int32_t n_mat, const void* mats_raw,
+                                    int32_t n_light, const void* lights_raw, int32_t build_threads,
+                                    int32_t n_xform, const float* xforms, const int32_t* xform_index,
+                                    int32_t use_bvh) {
     RefScene* R = new RefScene();
     R->xf.resize(n_xform > 0 ? n_xform : 0);
     R->xf_inv.resize(R->xf.size());
@@ -71,7 +74,7 @@ REF_API void* ref_scene_create_xf(int32_t n_tri, const float* verts, const float
     }
     R->scene.setMaxTracedDepth(5);
     R->scene.setAmbientRefractiveIndex(1.0f);
-    R->scene.useBvh(true);
+    R->scene.useBvh(use_bvh != 0);
     R->scene.setAASamples(1);
     const char* mb = static_cast<const char*>(mats_raw);
     for (int i = 0; i < n_mat; ++i) {
@@ -118,11 +121,21 @@ REF_API void* ref_scene_create_xf(int32_t n_tri, const float* verts, const float
         R->lights.push_back(l);
         R->scene.addLightSource(l);
     }
-    int saved = omp_get_max_threads();
-    if (build_threads > 0) omp_set_num_threads(build_threads);  // the SAH sweep races (no determinism otherwise)
-    R->scene.flush();
-    omp_set_num_threads(saved);
+    if (use_bvh) {
+        int saved = omp_get_max_threads();
+        if (build_threads > 0) omp_set_num_threads(build_threads);  // the SAH sweep races (no determinism otherwise)
+        R->scene.flush();
+        omp_set_num_threads(saved);
+    }
     return R;
+}
+
+REF_API void* ref_scene_create_xf(int32_t n_tri, const float* verts, const float* normals,
+                                  const int32_t* mat, int32_t n_mat, const void* mats_raw,
+                                  int32_t n_light, const void* lights_raw, int32_t build_threads,
+                                  int32_t n_xform, const float* xforms, const int32_t* xform_index) {
+    return ref_scene_create_opts(n_tri, verts, normals, mat, n_mat, mats_raw, n_light, lights_raw, build_threads,
+                                 n_xform, xforms, xform_index, 1);
 }
 
 REF_API void* ref_scene_create(int32_t n_tri, const float* verts, const float* normals, const int32_t* mat,
@@ -197,6 +210,27 @@ REF_API void ref_occluded(void* h, int32_t n, const float* orig, const float* di
         r.setOrigin(glm::vec3(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]));
         r.setDirection(glm::vec3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
         occ[i] = R->scene.shadowRayVisibilityBVH(r) ? 1 : 0;
+    }
+}
+
+// Scene::findMinDistanceIntersectionLinear for n rays: found flag, point,
+// normal and the diffuse colour of the material it returns (the linear search
+// sets RayIntersection's material, not its material index).
+REF_API void ref_nearest_linear(void* h, int32_t n, const float* orig, const float* dirs, int32_t* hit,
+                                float* point, float* normal, float* kd) {
+    RefScene* R = static_cast<RefScene*>(h);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; ++i) {
+        Ray r;
+        r.setOrigin(glm::vec3(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]));
+        r.setDirection(glm::vec3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        RayIntersection x;
+        bool got = R->scene.findMinDistanceIntersectionLinear(r, x);
+        hit[i] = got ? 1 : 0;
+        if (got) {
+            for (int k = 0; k < 3; ++k) { point[3 * i + k] = x.getPoint()[k]; normal[3 * i + k] = x.getNormal()[k]; }
+            for (int k = 0; k < 4; ++k) kd[4 * i + k] = x.getMaterial().getDiffuseColor()[k];
+        }
     }
 }
 

@@ -32,14 +32,17 @@ def test_struct_layouts_match_header(tmp_path):
                "eos_render_opts": E.RenderOpts,
                "eos_render_stats": E.RenderStats, "eos_adjoint_opts": E.AdjointOpts,
                "eos_adjoint_stats": E.AdjointStats, "eos_batch_opts": E.BatchOpts,
-               "eos_batch_stats": E.BatchStats, "eos_launch_config": E.LaunchConfig}
+               "eos_batch_stats": E.BatchStats, "eos_launch_config": E.LaunchConfig, "eos_trace_opts": E.TraceOpts}
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in structs.items():
         src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
             src.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
     src += ['printf("eos_camera %zu\\n", sizeof(eos_camera));', 'printf("eos_material %zu\\n", sizeof(eos_material));',
-            'printf("eos_light %zu\\n", sizeof(eos_light));', "return 0;}"]
+            'printf("eos_light %zu\\n", sizeof(eos_light));', 'printf("eos_ray_hit %zu\\n", sizeof(eos_ray_hit));']
+    for f in E.RAY_HIT.names:
+        src.append(f'printf("eos_ray_hit.{f} %zu\\n", offsetof(eos_ray_hit, {f}));')
+    src.append("return 0;}")
     c = tmp_path / "probe.c"
     c.write_text("\n".join(src))
     exe = tmp_path / "probe"
@@ -50,6 +53,9 @@ def test_struct_layouts_match_header(tmp_path):
         for f, _ in py._fields_:
             assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"
     assert int(got["eos_camera"]) == 56 and int(got["eos_material"]) == 64 and int(got["eos_light"]) == 32
+    assert int(got["eos_ray_hit"]) == E.RAY_HIT.itemsize == 48
+    for f in E.RAY_HIT.names:
+        assert int(got[f"eos_ray_hit.{f}"]) == E.RAY_HIT.fields[f][1], f
 
 
 def test_null_arguments_are_rejected():
@@ -68,6 +74,7 @@ def test_null_arguments_are_rejected():
     assert L.eos_render_rows(None, None, 0, 1, None, None, None) == E.EOS_ERR_INVALID
     assert L.eos_render_frames(None, None, 1, None, None, None) == E.EOS_ERR_INVALID
     assert L.eos_set_ambient_refractive_index(None, 1.0) == E.EOS_ERR_INVALID
+    assert L.eos_trace_rays(None, 1, None, None, None, None, None) == E.EOS_ERR_INVALID
 
 
 def test_no_device_is_reported():

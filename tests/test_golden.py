@@ -121,3 +121,75 @@ def test_render_aa_matches_reference(gold, name, n):
     assert excl.mean() < 0.05
     # and it is not the plain image: the samples moved
     assert not np.array_equal(ref, gold[f"{name}/rgba"])
+
+
+@pytest.mark.parametrize("name", sorted(golden_scenes()))
+def test_linear_search_matches_reference(gold, name):
+    """Scene::findMinDistanceIntersectionLinear, run by the reference on a scene
+    without a BVH (surfaces in the order they were added): the restatement gives
+    the same return value, point, normal and material on every probe ray (no
+    box test is involved, so no ray is left out)."""
+    s = golden_scenes()[name]()
+    o, d = gold[f"{name}/ray_o"], gold[f"{name}/ray_d"]
+    r = po.Oracle(s).nearest_linear(o, d)
+    assert np.array_equal(r["found"], gold[f"{name}/lin_hit"].astype(bool))
+    f = r["found"]
+    assert np.array_equal(r["point"][f], gold[f"{name}/lin_point"][f])
+    assert np.array_equal(r["normal"][f], gold[f"{name}/lin_normal"][f])
+    k = r["tri"] >= 0
+    assert np.array_equal(s.materials["kd"][s.mat[r["tri"][k]]], gold[f"{name}/lin_kd"][k])
+    # and it is the nearest hit the BVH query finds, away from the box-test exclusions
+    b = po.Oracle(s).nearest(o, d)
+    keep = ~(d == 0).any(1) & ~((b["tri"] >= 0) & (b["t"] < 1e-3))
+    assert np.array_equal((r["tri"] >= 0)[keep], (b["tri"] >= 0)[keep])
+
+
+def bunny_scene():
+    """The reference driver's scene from the committed bunny mesh, read back
+    through the library's OBJ reader (as TriangleMesh::loadFromFile does)."""
+    import tempfile
+    from eos_amd import _lib as E
+    from eos_amd import scenes
+    m = np.load(os.path.join(os.path.dirname(__file__), "golden", "bunny_mesh.npz"), allow_pickle=False)
+    with tempfile.TemporaryDirectory() as t:
+        path = os.path.join(t, "bunny.obj")
+        scenes.write_indexed_obj(path, m["positions"], m["faces"])
+        v, n = E.load_obj(path)
+    return scenes.driver_scene(v, n)
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/objmodels"), reason="the reference's OBJ models are absent")
+def test_bunny_fixture_reads_like_the_reference_file():
+    """tests/golden/bunny_mesh.npz written back as OBJ reads into exactly the
+    triangles and normals objmodels/bunny.obj reads into."""
+    from eos_amd import _lib as E
+    s = bunny_scene()
+    v, n = E.load_obj("/root/reference/objmodels/bunny.obj")
+    assert np.array_equal(s.verts, v) and np.array_equal(s.normals, n)
+
+
+def test_bunny_linear_rays_match_reference(gold):
+    """The reference's linear query on its driver's scene (src/main.cpp:330-411;
+    its BVH build and its linear render cannot run there, see make_golden.py):
+    primary rays through every 8th pixel and the shadow rays of their hits."""
+    s = bunny_scene()
+    assert bytes(gold["bunny/hash"]).decode() == scene_hash(s)
+    orc = po.Oracle(s)
+    o, d = gold["bunny/ray_o"], gold["bunny/ray_d"]
+    hit = gold["bunny/lin_hit"].astype(bool)
+    rng = np.random.default_rng(5)
+    idx = np.sort(np.concatenate([np.flatnonzero(hit), rng.choice(np.flatnonzero(~hit), 400, replace=False)]))
+    r = orc.nearest_linear(o[idx], d[idx])
+    assert np.array_equal(r["found"], hit[idx])
+    f = r["found"]
+    assert np.array_equal(r["point"][f], gold["bunny/lin_point"][idx][f])
+    assert np.array_equal(r["normal"][f], gold["bunny/lin_normal"][idx][f])
+    assert (gold["bunny/lin_kd"][hit] == s.materials["kd"][0]).all()
+    occ = orc.nearest_linear(gold["bunny/shadow_o"], gold["bunny/shadow_d"])["found"]
+    assert np.array_equal(occ, gold["bunny/shadow_hit"].astype(bool))
+    # the BVH query: the same hits and points; its normal is normalised once more
+    b = orc.nearest(o[idx], d[idx])
+    assert np.array_equal(b["tri"] >= 0, f)
+    assert np.array_equal(b["point"][f], r["point"][f])
+    assert np.abs(b["normal"][f] - r["normal"][f]).max() <= 2.0 ** -22
+    assert np.array_equal(orc.nearest(gold["bunny/shadow_o"], gold["bunny/shadow_d"], any_hit=True), occ)

@@ -148,3 +148,66 @@ def test_cpp_split_frame_launcher_renders_each_band(tmp_path, n_bands):
     g = E.Scene(s, leaf_max=4)
     full = g.render(depth=4)["rgba"]
     assert np.array_equal(bands, full[:rows * n_bands])
+
+
+def test_cpp_api_reference_driver_scene(tmp_path):
+    """The reference driver's own scene built the way its driver builds it
+    (src/main.cpp:330-481): the bunny read through TriangleMesh::loadFromFile
+    (the committed mesh written back as OBJ), glm::translate(2, -2, -2) through
+    Scene::addTransformation and the mesh's transformation index, its material
+    and two lights, then MultiDeviceRenderer at 1280 x 720 and the single-GPU
+    route.  The image equals the C ABI's and agrees with the oracle; the Scene's
+    ray queries (per ray and batched) answer the reference's golden rays as the
+    reference did."""
+    from eos_amd import _lib as E
+    from tests.test_golden import GOLD, bunny_scene
+    gold = np.load(GOLD, allow_pickle=False)
+    s = bunny_scene()
+    m = np.load(os.path.join(ROOT, "tests", "golden", "bunny_mesh.npz"), allow_pickle=False)
+    empty = scenes.SceneData(np.zeros((0, 3, 3), np.float32), np.zeros((0, 3, 3), np.float32), np.zeros(0, np.int32),
+                             s.materials, s.lights, s.camera, s.depth, "driver scene without its mesh")
+    sc, obj = tmp_path / "driver.bin", tmp_path / "bunny.obj"
+    _write_scene(sc, empty)
+    scenes.write_indexed_obj(str(obj), m["positions"], m["faces"])
+    hit = gold["bunny/lin_hit"].astype(bool)
+    rng = np.random.default_rng(7)
+    sel = np.sort(np.concatenate([np.flatnonzero(hit), rng.choice(np.flatnonzero(~hit), 200, replace=False)]))
+    qo = np.concatenate([gold["bunny/ray_o"][sel], gold["bunny/shadow_o"]]).astype(np.float32)
+    qd = np.concatenate([gold["bunny/ray_d"][sel], gold["bunny/shadow_d"]]).astype(np.float32)
+    qf = tmp_path / "rays.bin"
+    with open(qf, "wb") as f:
+        np.array([len(qo)], np.int32).tofile(f)
+        qo.tofile(f)
+        qd.tofile(f)
+    o1, o2 = tmp_path / "multi.rgba", tmp_path / "single.rgba"
+    print(f"running the C++ example on the driver scene ({s.n_tri} triangles, {len(qo)} query rays)", flush=True)
+    r = subprocess.run([DEMO, str(sc), str(o1), str(o2), str(obj), "0", "2", "-2", "-2"], capture_output=True,
+                       text=True, timeout=300, env=dict(os.environ, EOS_QUIET="1", EOS_QUERIES=str(qf)))
+    assert r.returncode == 0, r.stderr
+    assert "has 69666 Triangles" in r.stdout
+    W, H = 1280, 720
+    multi = np.fromfile(o1, np.uint8).reshape(H, W, 4)
+    single = np.fromfile(o2, np.uint8).reshape(H, W, 4)
+    ref = E.Scene(s, leaf_max=4).render(want_hits=True)
+    assert np.array_equal(multi, ref["rgba"]) and np.array_equal(single, ref["rgba"])
+    oo = po.Oracle(s).render()
+    agree = (ref["hit"] == oo["hit"]).mean()
+    assert agree >= 0.9999, agree
+    diff = np.abs(ref["rgba"].astype(int) - oo["rgba"].astype(int)).max(-1)
+    assert (diff[ref["hit"] == oo["hit"]] <= 1).all()
+    assert (ref["hit"] >= 0).sum() > 10000                   # the bunny is in the frame
+    rec = np.dtype([("found", np.int32), ("point", np.float32, 3), ("normal", np.float32, 3), ("mat", np.int32),
+                    ("occ", np.int32), ("lfound", np.int32), ("lpoint", np.float32, 3), ("lnormal", np.float32, 3),
+                    ("lmat", np.int32)])
+    per_ray = np.fromfile(str(qf) + ".out", rec)
+    batched = np.fromfile(str(qf) + ".batch", rec)
+    assert len(per_ray) == len(qo) and np.array_equal(per_ray.view(np.uint8), batched.view(np.uint8))
+    n1 = len(sel)
+    want_hit = np.concatenate([hit[sel], gold["bunny/shadow_hit"].astype(bool)])
+    assert np.array_equal(per_ray["lfound"] == 1, want_hit)
+    assert np.array_equal(per_ray["found"] == 1, want_hit) and np.array_equal(per_ray["occ"] == 1, want_hit)
+    f = per_ray["lfound"][:n1] == 1
+    assert np.array_equal(per_ray["lpoint"][:n1][f], gold["bunny/lin_point"][sel][f])
+    assert np.array_equal(per_ray["lnormal"][:n1][f], gold["bunny/lin_normal"][sel][f])
+    assert np.array_equal(per_ray["point"][:n1][f], gold["bunny/lin_point"][sel][f])
+    assert (per_ray["mat"][per_ray["found"] == 1] == 0).all()
