@@ -12,8 +12,11 @@
 // records the forward pass stored (surfaces and visibility per level, level
 // colours with their weights); they are replayed, or re-traced first when the
 // caller asks for it or none are stored.  One thread per hit record, FP32
-// arithmetic, per-workgroup LDS accumulators (ds_add_f32), flushed once per
-// workgroup into FP64 global accumulators.
+// arithmetic.  Sums are exact: every FP32 contribution becomes a fixed-point
+// integer (resolution 2^-64) added into 64-bit integer limbs, per workgroup in
+// LDS, then into global limbs; integer addition does not depend on order, so
+// the gradients are the same bits whatever the record order (replay, re-trace,
+// workgroup scheduling).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +24,7 @@
 #include <vector>
 
 #include "eos_internal.h"
+#include "exact_sum.h"
 #include "trace.h"
 
 namespace eos {
@@ -35,22 +39,32 @@ struct AdjArgs {
     const uint8_t* vis;
     const float4* levcol;          // weights of the levels above
     const float* dl;               // [npix][3]
-    double* gout;                  // [M*7 + L*6]
+    unsigned long long* gout;      // [3][M*7 + L*6] limbs of the exact sums
+    int* err;                      // bit 8: a contribution out of range (non-finite or >= 2^31)
     int lds_acc;                   // accumulate in LDS (else straight to global)
 };
 
-__device__ __forceinline__ void acc_add(float* lds, double* g, int idx, float v, bool use_lds) {
-    if (use_lds) atomicAdd(&lds[idx], v);
-    else atomicAdd(&g[idx], (double)v);
+__device__ __forceinline__ void acc_add(unsigned long long* lds, unsigned long long* g, int nacc, int idx, float v,
+                                        bool use_lds, int* err) {
+    if (v == 0.0f) return;
+    long long l0, l1, l2;
+    if (!to_limbs(v, l0, l1, l2)) {
+        atomicOr(err, 8);
+        return;
+    }
+    unsigned long long* base = use_lds ? lds : g;
+    if (l0) atomicAdd(base + idx, (unsigned long long)l0);
+    if (l1) atomicAdd(base + nacc + idx, (unsigned long long)l1);
+    if (l2) atomicAdd(base + 2 * nacc + idx, (unsigned long long)l2);
 }
 
 __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
-    extern __shared__ float acc[];
+    extern __shared__ unsigned long long acc[];
     const int M = a.sc.n_mat, L = a.sc.n_light;
     const int nacc = 7 * M + 6 * L;
     const bool use_lds = a.lds_acc != 0;
     if (use_lds) {
-        for (int i = threadIdx.x; i < nacc; i += blockDim.x) acc[i] = 0.0f;
+        for (int i = threadIdx.x; i < 3 * nacc; i += blockDim.x) acc[i] = 0ull;
         __syncthreads();
     }
     const int n = *a.count_ptr;
@@ -111,45 +125,47 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
             const float gLL = dot3(gL, Ld);
             const F3 gD = scale(1.0f / dlen, sub(gL, scale(gLL, Ld)));
             const int lb = 7 * M + 6 * i;
-            acc_add(acc, a.gout, lb + 0, gD.x, use_lds);
-            acc_add(acc, a.gout, lb + 1, gD.y, use_lds);
-            acc_add(acc, a.gout, lb + 2, gD.z, use_lds);
-            acc_add(acc, a.gout, lb + 3, glc[0], use_lds);
-            acc_add(acc, a.gout, lb + 4, glc[1], use_lds);
-            acc_add(acc, a.gout, lb + 5, glc[2], use_lds);
+            acc_add(acc, a.gout, nacc, lb + 0, gD.x, use_lds, a.err);
+            acc_add(acc, a.gout, nacc, lb + 1, gD.y, use_lds, a.err);
+            acc_add(acc, a.gout, nacc, lb + 2, gD.z, use_lds, a.err);
+            acc_add(acc, a.gout, nacc, lb + 3, glc[0], use_lds, a.err);
+            acc_add(acc, a.gout, nacc, lb + 4, glc[1], use_lds, a.err);
+            acc_add(acc, a.gout, nacc, lb + 5, glc[2], use_lds, a.err);
         }
         if (any) {
             const int mb = 7 * mi;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                acc_add(acc, a.gout, mb + k, gkd[k], use_lds);
-                acc_add(acc, a.gout, mb + 3 + k, gks[k], use_lds);
+                acc_add(acc, a.gout, nacc, mb + k, gkd[k], use_lds, a.err);
+                acc_add(acc, a.gout, nacc, mb + 3 + k, gks[k], use_lds, a.err);
             }
-            acc_add(acc, a.gout, mb + 6, gs, use_lds);
+            acc_add(acc, a.gout, nacc, mb + 6, gs, use_lds, a.err);
         }
     }
     if (use_lds) {
         __syncthreads();
-        for (int i = threadIdx.x; i < nacc; i += blockDim.x) {
-            const float v = acc[i];
-            if (v != 0.0f) atomicAdd(&a.gout[i], (double)v);
+        for (int i = threadIdx.x; i < 3 * nacc; i += blockDim.x) {
+            const unsigned long long v = acc[i];
+            if (v != 0ull) atomicAdd(&a.gout[i], v);
         }
     }
 }
 
-__global__ void k_zero_f64(double* p, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
-// dst_m[M][7] += g[0:7M], dst_l[L][6] += g[7M:]
-__global__ void k_accumulate_out(const double* g, int M, int L, double* dm, double* dlg) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 7 * M + 6 * L; i += gridDim.x * blockDim.x) {
-        if (i < 7 * M) dm[i] += g[i];
-        else dlg[i - 7 * M] += g[i];
+// dst_m[M][7] += value of limbs i < 7M, dst_l[L][6] += the rest
+__global__ void k_accumulate_out(const unsigned long long* g, int M, int L, double* dm, double* dlg) {
+    const int nacc = 7 * M + 6 * L;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nacc; i += gridDim.x * blockDim.x) {
+        const double v = limb_value(g, nacc, i);
+        if (i < 7 * M) dm[i] += v;
+        else dlg[i - 7 * M] += v;
     }
 }
 
-constexpr int kLdsAccFloats = 12288;   // 48 KiB of accumulators per workgroup
+constexpr int kLdsAccParams = 2048;   // 48 KiB of limbs (3 x int64 per parameter) per workgroup
 
 }  // namespace
 
@@ -177,13 +193,16 @@ eos_status render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint
         EOS_HIP(hipMemcpyAsync(dl_dev.ptr, dl, 12 * npix, hipMemcpyHostToDevice, S));
         dl_ptr = dl_dev.as<float>();
     }
-    EOS_TRY(g_dev.ensure(8 * (size_t)std::max(nacc, 1)));
-    hipLaunchKernelGGL(k_zero_f64, dim3(std::max(1, std::min(64, (nacc + 255) / 256))), dim3(256), 0, S,
-                       g_dev.as<double>(), nacc);
+    // limbs [3][nacc], then one word for the range flag
+    const int words = 3 * nacc + 1;
+    EOS_TRY(g_dev.ensure(8 * (size_t)words));
+    hipLaunchKernelGGL(k_zero_u64, dim3(std::max(1, std::min(64, (words + 255) / 256))), dim3(256), 0, S,
+                       g_dev.as<unsigned long long>(), words);
+    int* err = reinterpret_cast<int*>(g_dev.as<unsigned long long>() + 3 * (size_t)nacc);
     int* ctr = fb.counters.as<int>();
     const SceneView sc = view_of(s);
-    const bool lds = nacc <= kLdsAccFloats;
-    const size_t shmem = lds ? sizeof(float) * (size_t)nacc : 0;
+    const bool lds = nacc <= kLdsAccParams;
+    const size_t shmem = lds ? 3 * sizeof(unsigned long long) * (size_t)nacc : 0;
     const unsigned grid = (unsigned)s->num_cu * 8;
     EOS_HIP(hipEventRecord(s->ev[0], S));
     for (int l = 0; l <= o->depth && L > 0; ++l) {
@@ -197,23 +216,29 @@ eos_status render_adjoint(eos_scene* s, const eos_camera* cam, const eos_adjoint
         a.vis = fb.vis_at(l);
         a.levcol = fb.levcol.as<float4>();
         a.dl = dl_ptr;
-        a.gout = g_dev.as<double>();
+        a.gout = g_dev.as<unsigned long long>();
+        a.err = err;
         a.lds_acc = lds ? 1 : 0;
         hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(256), shmem, S, a);
     }
     EOS_HIP(hipGetLastError());
     EOS_HIP(hipEventRecord(s->ev[1], S));
+    int herr = 0;
+    EOS_HIP(hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, S));
     if (o->memory == EOS_MEM_DEVICE) {
         hipLaunchKernelGGL(k_accumulate_out, dim3(std::max(1, std::min(64, (nacc + 255) / 256))), dim3(256), 0, S,
-                           g_dev.as<const double>(), M, L, gm, gl);
+                           g_dev.as<const unsigned long long>(), M, L, gm, gl);
         EOS_HIP(hipStreamSynchronize(S));
     } else {
-        std::vector<double> h((size_t)std::max(nacc, 1));
-        EOS_HIP(hipMemcpyAsync(h.data(), g_dev.ptr, 8 * (size_t)nacc, hipMemcpyDeviceToHost, S));
+        std::vector<unsigned long long> h((size_t)std::max(3 * nacc, 1));
+        EOS_HIP(hipMemcpyAsync(h.data(), g_dev.ptr, 8 * 3 * (size_t)nacc, hipMemcpyDeviceToHost, S));
         EOS_HIP(hipStreamSynchronize(S));
-        for (int i = 0; i < 7 * M; ++i) gm[i] += h[i];
-        for (int i = 0; i < 6 * L; ++i) gl[i] += h[7 * (size_t)M + i];
+        for (int i = 0; i < 7 * M; ++i) gm[i] += limb_value(h.data(), nacc, i);
+        for (int i = 0; i < 6 * L; ++i) gl[i] += limb_value(h.data(), nacc, 7 * M + i);
     }
+    if (herr & 8)
+        return fail(EOS_ERR_LIMIT, "eos_render_adjoint: a gradient contribution is not finite or exceeds 2^31 "
+                                   "(a light on a surface?); the gradients are incomplete");
     float ms = 0.0f;
     EOS_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
     int hits[EOS_MAX_DEPTH + 1] = {0};

@@ -60,3 +60,30 @@ def test_batch_with_launch_configs(batch):
         for a, b in zip(outs, ref):
             assert np.array_equal(a, b)
     sc[0].set_launch_config()
+
+
+def test_batch_follows_material_changes():
+    """A material turned transparent (or back) between two batched frames: the
+    batch is reshaped (ray trees instead of chains) and every image still equals
+    the scene's single render, with and without graph replay."""
+    from eos_amd import _lib as E
+    data = scenes.config_d(n_scenes=4, scale=0.02)
+    sc = [E.Scene(d, leaf_max=4) for d in data]
+    for use_graph in (False, True):
+        base, _ = E.render_batch(sc, depth=2, use_graph=use_graph)
+        for d, s, o in zip(data, sc, base):
+            assert np.array_equal(o, s.render(depth=2)["rgba"]), d.name
+        m = data[1].materials.copy()
+        m["transparent"][0] = 1
+        m["refractive_index"][0] = 1.4
+        sc[1].set_materials(m)
+        outs, _ = E.render_batch(sc, depth=2, use_graph=use_graph)
+        glass = sc[1].render(depth=2)["rgba"]
+        assert not np.array_equal(glass, base[1])
+        print(f"use_graph={use_graph}: scene 1 changed on {int((glass != base[1]).any(-1).sum())} pixels", flush=True)
+        for d, s, o in zip(data, sc, outs):
+            assert np.array_equal(o, s.render(depth=2)["rgba"]), d.name
+        sc[1].set_materials(data[1].materials)
+        outs, _ = E.render_batch(sc, depth=2, use_graph=use_graph)
+        for o, b in zip(outs, base):
+            assert np.array_equal(o, b)

@@ -38,6 +38,7 @@ def _compare(s, g_out, o_out, name):
 
 
 RENDER_CASES = {
+    "config_a_512": lambda: scenes.config_a(512, 512),   # config A at its stated size
     "config_a_128": lambda: scenes.config_a(128, 128),
     "config_a_odd": lambda: scenes.config_a(37, 23),
     "soup_50k_d3": lambda: scenes.config_b(160, 90, n=50000),
