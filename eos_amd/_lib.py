@@ -23,7 +23,9 @@ EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_build
            "eos_update_lights", "eos_render", "eos_render_adjoint", "eos_render_batch",
            "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
            "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free", "eos_set_transforms",
-           "eos_set_ambient_refractive_index", "eos_render_frames", "eos_render_rows", "eos_trace_rays"]
+           "eos_set_ambient_refractive_index", "eos_render_frames", "eos_render_rows", "eos_trace_rays",
+           "eos_trace_rays_static"]
+STATIC_MAX_RAYS = 1 << 20   # EOS_STATIC_MAX_RAYS
 
 # eos_trace_rays query modes and the eos_ray_hit record (48 bytes)
 QUERY = {"nearest": 0, "occluded": 1, "linear": 2}
@@ -146,6 +148,7 @@ def lib():
         L.eos_render_frames.argtypes = [vp, vp, i32, vp, vp, vp]
         L.eos_render_rows.argtypes = [vp, vp, i32, i32, vp, vp, vp]
         L.eos_trace_rays.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+        L.eos_trace_rays_static.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         for name in EXPORTS:
             if name != "eos_last_error" and name != "eos_abi_version":
                 getattr(L, name).restype = C.c_int

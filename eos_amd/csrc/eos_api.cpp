@@ -425,6 +425,28 @@ eos_status eos_trace_rays(eos_scene* s, int32_t n, const float* o, const float* 
     return trace_rays(s, n, o, d, opts, hits, occ);
 }
 
+eos_status eos_trace_rays_static(eos_scene* s, int32_t n, const float* o, const float* d, const eos_trace_opts* opts,
+                                 eos_ray_hit* hits, uint8_t* occ) {
+    if (!s || !opts) return fail(EOS_ERR_INVALID, "eos_trace_rays_static: null argument");
+    if (n < 0) return fail(EOS_ERR_INVALID, "eos_trace_rays_static: negative ray count");
+    if (n > EOS_STATIC_MAX_RAYS)
+        return fail(EOS_ERR_LIMIT, "eos_trace_rays_static: " + std::to_string(n) + " rays; at most " +
+                                       std::to_string(EOS_STATIC_MAX_RAYS) + " per call (the fixed scratch array)");
+    if (opts->mode != EOS_QUERY_NEAREST && opts->mode != EOS_QUERY_OCCLUDED && opts->mode != EOS_QUERY_NEAREST_LINEAR)
+        return fail(EOS_ERR_INVALID, "eos_trace_rays_static: unknown query mode");
+    if (opts->memory != EOS_MEM_DEVICE)
+        return fail(EOS_ERR_INVALID, "eos_trace_rays_static: rays and outputs must be device memory (EOS_MEM_DEVICE)");
+    if (n > 0 && (!o || !d)) return fail(EOS_ERR_INVALID, "eos_trace_rays_static: null ray arrays");
+    if (n > 0 && opts->mode == EOS_QUERY_OCCLUDED && !occ)
+        return fail(EOS_ERR_INVALID, "eos_trace_rays_static: EOS_QUERY_OCCLUDED writes `occluded`, which is null");
+    if (n > 0 && opts->mode != EOS_QUERY_OCCLUDED && !hits)
+        return fail(EOS_ERR_INVALID, "eos_trace_rays_static: nearest-hit queries write `hits`, which is null");
+    if (!s->built) return fail(EOS_ERR_STATE, "eos_trace_rays_static: call eos_build_lbvh first");
+    if (n == 0) return EOS_OK;
+    EOS_HIP(hipSetDevice(s->device));
+    return trace_rays_static(s, n, o, d, opts, hits, occ);
+}
+
 eos_status eos_render_batch(eos_scene* const* scenes, const eos_camera* cams, int32_t n, const eos_batch_opts* o,
                             uint8_t* const* out, eos_batch_stats* st) {
     if (!scenes || !cams || !o || !out || n <= 0) return fail(EOS_ERR_INVALID, "eos_render_batch: bad argument");

@@ -229,6 +229,8 @@ eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_
 // Implemented in query.hip
 eos_status trace_rays(eos_scene* s, int n, const float* o, const float* d, const eos_trace_opts* opts,
                       eos_ray_hit* hits, uint8_t* occ);
+eos_status trace_rays_static(eos_scene* s, int n, const float* o, const float* d, const eos_trace_opts* opts,
+                             eos_ray_hit* hits, uint8_t* occ);
 SceneView view_of(const eos_scene* s);
 // One anti-aliasing sample frame: offset inside the pixel and what the resolve
 // does with the sample's colour (src/RayTracer.cpp:214-242).

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "eos_internal.h"
@@ -160,7 +161,99 @@ __global__ void k_zero_i32_query(int* p, int n) {
     for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) p[i] = 0;
 }
 
+// ---- the allocation-free variant (eos_trace_rays_static) --------------------
+// Its scratch is this fixed array (one copy per GPU, held until the process
+// ends): the fetch cursors and the error flag in the first kStaticHeader bytes,
+// then the nearest-hit records (16 bytes per ray).  Host code reaches it only
+// through the address hipGetSymbolAddress gives on each device.
+constexpr size_t kStaticHeader = 4096;
+static_assert(4 * (size_t)kQueryCtrInts <= kStaticHeader, "cursors and flag fit the header");
+constexpr size_t kStaticBytes = kStaticHeader + 16 * (size_t)EOS_STATIC_MAX_RAYS;
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+
+// The device address of `scratch` on the current device, looked up once per
+// device (null when the lookup fails).
+void* static_scratch(int device) {
+    static std::mutex mu;
+    static void* ptr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        ptr[device] = p;
+    }
+    return ptr[device];
+}
+
 }  // namespace
+
+eos_status trace_rays_static(eos_scene* s, int n, const float* o, const float* d, const eos_trace_opts* opts,
+                             eos_ray_hit* hits, uint8_t* occ) {
+    const hipStream_t S = s->stream;
+    void* base = static_scratch(s->device);
+    if (!base) return fail(EOS_ERR_HIP, "eos_trace_rays_static: hipGetSymbolAddress of the scratch array failed");
+    int* ctr = static_cast<int*>(base);
+    float4* rec = reinterpret_cast<float4*>(static_cast<char*>(base) + kStaticHeader);
+    float4* out = reinterpret_cast<float4*>(hits);
+    const unsigned blocks = (unsigned)(((size_t)n + 255) / 256);
+    if (opts->mode == EOS_QUERY_NEAREST_LINEAR) {
+        hipLaunchKernelGGL(k_query_linear, dim3(blocks), dim3(kLinTile), 0, S, n, o, d, s->n_tri,
+                           s->verts.as<const float>(), s->normals.as<const float>(), s->matidx.as<const int32_t>(),
+                           s->n_xf > 0 ? s->xf_index.as<const int32_t>() : nullptr,
+                           s->n_xf > 0 ? s->xf.as<const float4>() : nullptr, out);
+        EOS_HIP(hipGetLastError());
+        EOS_HIP(hipStreamSynchronize(S));
+        return EOS_OK;
+    }
+    // every lane's stack lives in LDS: lane quads walk the 4-wide tree (96
+    // entries per quad), single lanes walk the binary tree with all of its 64
+    // entries in LDS, so no overflow stack is needed
+    const int trace_grid = std::max(1, s->num_cu * std::min(s->cfg.waves_per_cu, 8) / 4);
+    const int grid = (int)std::min<size_t>((size_t)trace_grid, std::max<size_t>(1, (4 * (size_t)n + 255) / 256));
+    hipLaunchKernelGGL(k_zero_i32_query, dim3(1), dim3(256), 0, S, ctr, kQueryCtrInts);
+    TraceArgs a{};
+    a.sc = view_of(s);
+    a.n_static = n;
+    a.count_mul = 1;
+    a.cursor = ctr;
+    a.err = ctr + kRanges * kCursorStride;
+    a.gstack = ctr;          // never dereferenced: no lane's stack leaves LDS
+    a.refill = s->cfg.refill;
+    a.batch = s->cfg.batch;
+    a.coop = 4;
+    a.ray_o = o;
+    a.ray_d = d;
+    const bool xf = s->n_xf > 0;
+    if (opts->mode == EOS_QUERY_OCCLUDED) {
+        a.vis_out = occ;
+        if (s->width == 2) {
+            if (xf) hipLaunchKernelGGL((k_trace<MODE_RAYS_ANY, 256, kMaxStack, false, false, 2, true>), dim3(grid), dim3(256), 0, S, a);
+            else hipLaunchKernelGGL((k_trace<MODE_RAYS_ANY, 256, kMaxStack, false, false, 2, false>), dim3(grid), dim3(256), 0, S, a);
+        } else {
+            launch_rays<MODE_RAYS_ANY>(4, xf, grid, S, a);
+        }
+    } else {
+        a.hit_out = rec;
+        if (s->width == 2) {
+            if (xf) hipLaunchKernelGGL((k_trace<MODE_RAYS, 256, kMaxStack, false, false, 2, true>), dim3(grid), dim3(256), 0, S, a);
+            else hipLaunchKernelGGL((k_trace<MODE_RAYS, 256, kMaxStack, false, false, 2, false>), dim3(grid), dim3(256), 0, S, a);
+        } else {
+            launch_rays<MODE_RAYS>(4, xf, grid, S, a);
+        }
+        hipLaunchKernelGGL(k_query_hits, dim3(blocks), dim3(256), 0, S, n, o, d, rec, s->shade.as<const float4>(),
+                           s->geo.as<const float4>(), xf ? s->xf.as<const float4>() : nullptr, out);
+    }
+    EOS_HIP(hipGetLastError());
+    int err = 0;
+    EOS_HIP(hipMemcpyAsync(&err, a.err, sizeof err, hipMemcpyDeviceToHost, S));
+    EOS_HIP(hipStreamSynchronize(S));
+    if (err & 1) return fail(EOS_ERR_LIMIT, "eos_trace_rays_static: traversal stack overflow");
+    if (err & 2) return fail(EOS_ERR_STATE, "eos_trace_rays_static: traversal exceeded its visit bound (corrupt BVH)");
+    return EOS_OK;
+}
 
 eos_status trace_rays(eos_scene* s, int n, const float* o, const float* d, const eos_trace_opts* opts,
                       eos_ray_hit* hits, uint8_t* occ) {

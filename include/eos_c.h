@@ -296,6 +296,21 @@ typedef struct eos_trace_opts {
 eos_status eos_trace_rays(eos_scene* scene, int32_t n, const float* origins, const float* directions,
                           const eos_trace_opts* opts, eos_ray_hit* hits, uint8_t* occluded);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): eos_trace_rays with no allocation.  Its scratch (fetch
+ * cursors, error flag and, for EOS_QUERY_NEAREST, 16 bytes of hit record per
+ * ray) is one fixed device array of the library, 4 KiB + 16 B x
+ * EOS_STATIC_MAX_RAYS (16 MiB), held by every GPU the library runs on until the
+ * process ends.  n above EOS_STATIC_MAX_RAYS fails with EOS_ERR_LIMIT; rays and
+ * outputs must be device memory (opts->memory = EOS_MEM_DEVICE).  Calls on one
+ * GPU share the array: calls that overlap on one GPU race.  The scene must have
+ * been built (eos_build_lbvh), which is where its own memory is allocated.
+ * Results equal eos_trace_rays's.  Synchronous. */
+#define EOS_STATIC_MAX_RAYS (1 << 20)
+eos_status eos_trace_rays_static(eos_scene* scene, int32_t n, const float* origins, const float* directions,
+                                 const eos_trace_opts* opts, eos_ray_hit* hits, uint8_t* occluded);
+
 /* ---- mesh files ------------------------------------------------------------ */
 
 /* Triangles read from a Wavefront OBJ file, in the layout of eos_scene_desc
