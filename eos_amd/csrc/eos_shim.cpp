@@ -396,7 +396,7 @@ void MultiDeviceRenderer::renderSceneToHostBuffer(void* image, Camera* hostCamer
 // ---------------------------------------------------------------- C launchers
 extern "C" {
 
-void rayTrace_MegaKernel(cudaScene_t* scene, Camera* camera, int width, int height, uchar4_t* image, int*, int*) {
+void rayTrace_MegaKernel(cudaScene_t* scene, Camera* camera, int width, int height, uchar4* image, int*, int*) {
     const eos_camera cam = to_c(download_camera(camera), width, height);
     eos_render_opts o{};
     o.depth = RAYTRACER_DEFAULT_TRACE_DEPTH;
@@ -404,7 +404,7 @@ void rayTrace_MegaKernel(cudaScene_t* scene, Camera* camera, int width, int heig
     check(eos_render(scene->scene, &cam, &o, reinterpret_cast<uint8_t*>(image), nullptr), "rayTrace_MegaKernel");
 }
 
-void rayTrace_WarpShuffle_MegaKernel(cudaScene_t* scene, Camera* camera, int width, int height, uchar4_t* image,
+void rayTrace_WarpShuffle_MegaKernel(cudaScene_t* scene, Camera* camera, int width, int height, uchar4* image,
                                      int* blockdim, int) {
     rayTrace_MegaKernel(scene, camera, width, height, image, blockdim, nullptr);
 }
@@ -412,7 +412,7 @@ void rayTrace_WarpShuffle_MegaKernel(cudaScene_t* scene, Camera* camera, int wid
 // Renders the band of rows [gpuID*H/numGPUS, (gpuID+1)*H/numGPUS) into image
 // (src/MultiDeviceRenderer.cu:122-163): only that band's primary rays are
 // generated, so each device traces its own share of the frame.
-void rayTrace_MultiMegaKernel(cudaScene_t* scene, Camera* camera, int width, int height, uchar4_t* image,
+void rayTrace_MultiMegaKernel(cudaScene_t* scene, Camera* camera, int width, int height, uchar4* image,
                               int numGPUS, int gpuID, int*, int*) {
     if (numGPUS <= 1) {
         rayTrace_MegaKernel(scene, camera, width, height, image, nullptr, nullptr);

@@ -164,7 +164,7 @@ int main(int argc, char** argv) {
         if (hipMalloc(&band, 4 * (size_t)W * rows + 4) != hipSuccess) return 1;
         int blocks[2] = {(W + 15) / 16, (rows + 15) / 16}, threads[2] = {16, 16};
         for (int k = 0; k < n; ++k) {
-            rayTrace_MultiMegaKernel(bscene, bcam.getDeviceCamera(), W, H, static_cast<uchar4_t*>(band), n, k, blocks,
+            rayTrace_MultiMegaKernel(bscene, bcam.getDeviceCamera(), W, H, static_cast<uchar4*>(band), n, k, blocks,
                                      threads);
             gpuSynch();
             if (hipMemcpy(image4.data() + 4 * (size_t)W * rows * k, band, 4 * (size_t)W * rows,

@@ -30,6 +30,8 @@
 
 #include <stdint.h>
 
+#include <hip/hip_vector_types.h>   /* uchar4, the image type of the launchers below */
+
 #include <memory>
 #include <utility>
 #include <vector>
@@ -410,17 +412,13 @@ private:
     int m_Depth = RAYTRACER_DEFAULT_TRACE_DEPTH;
 };
 
-struct uchar4_t {
-    unsigned char x, y, z, w;
-};
-
 extern "C" {
 /* src/kernelWrapper.h:30-34; imageBuffer is device memory; launch shapes are ignored. */
-void rayTrace_MegaKernel(cudaScene_t* deviceScene, Camera* camera, int width, int height, uchar4_t* imageBuffer,
+void rayTrace_MegaKernel(cudaScene_t* deviceScene, Camera* camera, int width, int height, uchar4* imageBuffer,
                          int blockdim[], int tpblock[]);
 void rayTrace_WarpShuffle_MegaKernel(cudaScene_t* deviceScene, Camera* camera, int width, int height,
-                                     uchar4_t* imageBuffer, int blockdim[], int tpblock);
-void rayTrace_MultiMegaKernel(cudaScene_t* deviceScene, Camera* camera, int width, int height, uchar4_t* imageBuffer,
+                                     uchar4* imageBuffer, int blockdim[], int tpblock);
+void rayTrace_MultiMegaKernel(cudaScene_t* deviceScene, Camera* camera, int width, int height, uchar4* imageBuffer,
                               int numGPUS, int gpuID, int blockdimp[], int tpblock[]);
 /* src/cudaWrapper.h:37-44 */
 void resetDevice(void);
