@@ -92,6 +92,8 @@ static void free_scene(eos_scene* s) {
         if (s->seq_ev[k]) (void)hipEventDestroy(s->seq_ev[k]);
     }
     if (s->seq_start) (void)hipEventDestroy(s->seq_start);
+    for (hipEvent_t e : s->first_ev) (void)hipEventDestroy(e);
+    s->first_ev.clear();
     if (s->host_ctr) (void)hipHostFree(s->host_ctr);
     for (hipEvent_t& e : s->ev)
         if (e) (void)hipEventDestroy(e);

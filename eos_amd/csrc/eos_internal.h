@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/eos_c.h"
 
@@ -208,6 +209,7 @@ struct eos_scene {
     hipEvent_t resolved[kFrameSlots] = {}, seq_start = nullptr, seq_ev[kFrameSlots] = {};
     void* host_ctr = nullptr;
     size_t host_ctr_bytes = 0;
+    std::vector<hipEvent_t> first_ev;   // frames in flight: timing events around each frame's primary launch
     // Ray queries (eos_trace_rays): staged rays and outputs for host memory,
     // hit records, fetch cursors and error flag, overflow stacks.
     eos::DevBuf q_rays, q_out, q_rec, q_ctr, q_gstack;

@@ -486,6 +486,7 @@ struct Frame {
     int row0 = 0, band_h = 0;  // single scene: camera rows [row0, row0 + band_h) of the frame
     hipEvent_t resolve_after = nullptr;   // frames in flight: resolve after this event (the previous frame's)
     hipEvent_t resolve_done = nullptr;    // and record this one after it
+    hipEvent_t first_ev[2] = {nullptr, nullptr};   // frames in flight: around the primary-ray launch
     bool count = false;
     LaunchCfg cfg;
     int num_cu = 256;
@@ -676,8 +677,10 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             ta.hit_out = fb.hitq.as<float4>();
         }
         F.c[2 * l] = rec(S);
+        if (l == 0 && f.first_ev[0]) EOS_HIP(hipEventRecord(f.first_ev[0], S));
         if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, S, ta));
         else EOS_TRY(launch_trace_any<MODE_QUEUE>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, S, ta));
+        if (l == 0 && f.first_ev[1]) EOS_HIP(hipEventRecord(f.first_ev[1], S));
         F.c[2 * l + 1] = rec(S);
         SetupArgs sa{};
         sa.sc = f.sc;
@@ -937,6 +940,17 @@ eos_status run_pipelined(eos_scene* s, std::vector<Frame>& frames, eos_render_st
         f.cfg.batch = f.cfg.frames_batch;
     }
     const int slots = std::min(K, std::max(n, 1));
+    // a pair of timing events per frame around its primary-ray launch (the
+    // dispatch duration of that launch while frames overlap)
+    while (s->first_ev.size() < 2 * (size_t)n) {
+        hipEvent_t e = nullptr;
+        EOS_HIP(hipEventCreate(&e));
+        s->first_ev.push_back(e);
+    }
+    for (int i = 0; i < n; ++i) {
+        frames[(size_t)i].first_ev[0] = s->first_ev[2 * (size_t)i];
+        frames[(size_t)i].first_ev[1] = s->first_ev[2 * (size_t)i + 1];
+    }
     // size every slot in use for every frame first (buffers only grow): no
     // buffer is reallocated while a frame in flight uses it
     for (const Frame& f : frames)
@@ -969,6 +983,9 @@ eos_status run_pipelined(eos_scene* s, std::vector<Frame>& frames, eos_render_st
                             reinterpret_cast<const unsigned long long*>(base + 4 * CTR_COUNT), nullptr, nullptr,
                             &one));
         one.render_ms = total / n;
+        float fm = 0.f;
+        if (hipEventElapsedTime(&fm, s->first_ev[2 * (size_t)i], s->first_ev[2 * (size_t)i + 1]) == hipSuccess)
+            one.trace_first_ms = fm;
         if (st) st[i] = one;
     }
     return EOS_OK;

@@ -436,6 +436,27 @@ def config_c(width: int = 3840, height: int = 2160, nu: int = 2048, nv: int = 10
     return SceneData(verts, normals, mat, mats, lights, cam, 3, "C: displaced sphere 4M")
 
 
+def config_c_dense(width: int = 3840, height: int = 2160, nu: int = 2048, nv: int = 1024,
+                   scale: float = 4.0) -> SceneData:
+    """Config C's mesh (same 4,194,304 triangles, materials and lights) scaled by
+    4 about the origin and seen from close by, so that most primary rays hit.
+    At unit scale the reference's absolute |det| < 1e-5 test (src/Triangle.cpp:65)
+    rejects every unit-length ray against about 89% of the triangles (their
+    doubled area is below 1e-5), so even a camera the sphere fills gets about 3%
+    hits; scaled by 4, 95% of the triangles can be hit and about 88% of the
+    frame's primary rays hit, each spawning four shadow rays and, on reflective
+    materials, a reflection."""
+    c = config_c(width, height, nu, nv)
+    verts = (c.verts.astype(np.float64) * scale).astype(np.float32)
+    lights = c.lights.copy()
+    lights["position"][:, :3] = (lights["position"][:, :3].astype(np.float64) * scale).astype(np.float32)
+    u = np.array([0.3, 0.45, 1.0])
+    u /= np.linalg.norm(u)
+    cam = make_camera(1.45 * scale * u, (0.0, 0.0, 0.0), 60.0, width, height)
+    return SceneData(verts, c.normals, c.mat, c.materials, lights, cam, c.depth,
+                     "C dense: displaced sphere 4M scaled x4, close camera")
+
+
 def config_d(n_scenes: int = 64, scale: float = 1.0):
     """64 soups, N log-uniform in [10k, 500k], resolution from {256,512,768,1024}^2,
     1-4 lights, depth 2.  ``scale`` shrinks triangle counts and resolutions for tests."""

@@ -208,7 +208,8 @@ eos_status eos_render_rows(eos_scene* scene, const eos_camera* camera, int32_t r
  * trace while frame i's shadow and reflection launches finish.  Each image is
  * the one eos_render gives for that camera.  opts: depth, count_visits;
  * output_memory must be EOS_MEM_DEVICE, no rgb_out / hit_out / keep_records /
- * anti-aliasing.  stats: n entries (render_ms = the sequence's GPU time / n)
+ * anti-aliasing.  stats: n entries (render_ms = the sequence's GPU time / n;
+ * trace_first_ms = that frame's primary-ray launch, while it overlaps the other frame's launches)
  * or NULL.  No counterpart in the reference, whose driver renders one frame
  * per call of its viewer loop (src/main.cpp:514-606).  Synchronous. */
 eos_status eos_render_frames(eos_scene* scene, const eos_camera* cameras, int32_t n, const eos_render_opts* opts,

@@ -5,6 +5,7 @@
 // -ffp-contract=off so every float operation is the single IEEE operation the
 // reference's glm code performs (x86-64 SSE arithmetic, no excess precision).
 #include "oracle.h"
+#include "glm_mat4.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -70,41 +71,6 @@ inline V3 xform(const float* m, V3 p, float w) {
     float r[3];
     for (int k = 0; k < 3; ++k) r[k] = (m[k] * p.x + m[4 + k] * p.y) + (m[8 + k] * p.z + m[12 + k] * w);
     return {r[0], r[1], r[2]};
-}
-
-// glm 0.9.6 compute_inverse(tmat4x4) (glm/detail/type_mat4x4.inl:37-96): 2x2
-// cofactors of the lower rows, column vectors combined as (a - b) + c, signs,
-// determinant from the first column and row, one reciprocal.
-void inverse_glm(const float* m, float* out) {
-    auto M = [&](int c, int r) { return m[4 * c + r]; };
-    const float c00 = M(2, 2) * M(3, 3) - M(3, 2) * M(2, 3), c02 = M(1, 2) * M(3, 3) - M(3, 2) * M(1, 3),
-                c03 = M(1, 2) * M(2, 3) - M(2, 2) * M(1, 3);
-    const float c04 = M(2, 1) * M(3, 3) - M(3, 1) * M(2, 3), c06 = M(1, 1) * M(3, 3) - M(3, 1) * M(1, 3),
-                c07 = M(1, 1) * M(2, 3) - M(2, 1) * M(1, 3);
-    const float c08 = M(2, 1) * M(3, 2) - M(3, 1) * M(2, 2), c10 = M(1, 1) * M(3, 2) - M(3, 1) * M(1, 2),
-                c11 = M(1, 1) * M(2, 2) - M(2, 1) * M(1, 2);
-    const float c12 = M(2, 0) * M(3, 3) - M(3, 0) * M(2, 3), c14 = M(1, 0) * M(3, 3) - M(3, 0) * M(1, 3),
-                c15 = M(1, 0) * M(2, 3) - M(2, 0) * M(1, 3);
-    const float c16 = M(2, 0) * M(3, 2) - M(3, 0) * M(2, 2), c18 = M(1, 0) * M(3, 2) - M(3, 0) * M(1, 2),
-                c19 = M(1, 0) * M(2, 2) - M(2, 0) * M(1, 2);
-    const float c20 = M(2, 0) * M(3, 1) - M(3, 0) * M(2, 1), c22 = M(1, 0) * M(3, 1) - M(3, 0) * M(1, 1),
-                c23 = M(1, 0) * M(2, 1) - M(2, 0) * M(1, 1);
-    const float f0[4] = {c00, c00, c02, c03}, f1[4] = {c04, c04, c06, c07}, f2[4] = {c08, c08, c10, c11},
-                f3[4] = {c12, c12, c14, c15}, f4[4] = {c16, c16, c18, c19}, f5[4] = {c20, c20, c22, c23};
-    const float v0[4] = {M(1, 0), M(0, 0), M(0, 0), M(0, 0)}, v1[4] = {M(1, 1), M(0, 1), M(0, 1), M(0, 1)},
-                v2[4] = {M(1, 2), M(0, 2), M(0, 2), M(0, 2)}, v3[4] = {M(1, 3), M(0, 3), M(0, 3), M(0, 3)};
-    float inv[16];
-    for (int k = 0; k < 4; ++k) {
-        const float sa = (k & 1) ? -1.0f : 1.0f, sb = -sa;
-        inv[0 + k] = ((v1[k] * f0[k] - v2[k] * f1[k]) + v3[k] * f2[k]) * sa;
-        inv[4 + k] = ((v0[k] * f0[k] - v2[k] * f3[k]) + v3[k] * f4[k]) * sb;
-        inv[8 + k] = ((v0[k] * f1[k] - v1[k] * f3[k]) + v3[k] * f5[k]) * sa;
-        inv[12 + k] = ((v0[k] * f2[k] - v1[k] * f4[k]) + v2[k] * f5[k]) * sb;
-    }
-    const float d0 = M(0, 0) * inv[0], d1 = M(0, 1) * inv[4], d2 = M(0, 2) * inv[8], d3 = M(0, 3) * inv[12];
-    const float det = (d0 + d1) + (d2 + d3);
-    const float one_over = 1.0f / det;
-    for (int k = 0; k < 16; ++k) out[k] = inv[k] * one_over;
 }
 
 inline void transpose4(const float* a, float* out) {
@@ -303,7 +269,7 @@ bool anyhit(const Accel& A, const orc_scene& s, V3 o, V3 d) {
 inline void hit_geometry(const orc_scene& s, int tri, V3 o, V3 d, float t, float u, float v,
                          V3& P, V3& N, bool renorm = true) {
     const int xi = xform_of(s, tri);
-    float inv[16], invT[16];
+    float inv[16] = {}, invT[16] = {};
     if (xi >= 0) {
         inverse_glm(s.xforms + 16 * (size_t)xi, inv);
         transpose4(inv, invT);

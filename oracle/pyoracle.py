@@ -65,43 +65,62 @@ def build() -> None:
     subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
 
 
+_FAST = None
+
+
+def lib_fast():
+    """The same restatement built with the reference's timing flags (-O3
+    -ffast-math, oracle/Makefile: liboracle_fast.so): for CPU timings only,
+    never for parity."""
+    global _FAST
+    if _FAST is None:
+        path = os.path.join(_HERE, "liboracle_fast.so")
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle_fast.so"])
+        _FAST = _bind(C.CDLL(path))
+    return _FAST
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        _LIB = C.CDLL(path)
-        L = _LIB
-        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
-        L.orc_accel_build.restype = vp
-        L.orc_accel_build.argtypes = [vp, C.c_int]
-        L.orc_accel_free.argtypes = [vp]
-        L.orc_nearest.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp]
-        L.orc_render.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
-        L.orc_nearest_linear.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp]
-        L.orc_render_lbvh.argtypes = [vp, vp, i32, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, i32]
-        L.orc_mt.argtypes = [vp, vp, vp, vp]
-        L.orc_render_aa.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp]
-        L.orc_mat4_inverse.argtypes = [vp, vp]
-        L.orc_camera_ray.argtypes = [vp, C.c_int, C.c_int, vp]
-        L.orc_camera_ray_aa.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]
-        L.orc_quantize.restype = C.c_uint8
-        L.orc_quantize.argtypes = [C.c_float]
-        L.orc_morton3.restype = C.c_uint32
-        L.orc_morton3.argtypes = [C.c_uint32] * 3
-        L.orc_shade_phong.restype = C.c_float
-        L.orc_shade_phong.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
-        L.orc_morton_codes.argtypes = [i32, vp, vp, vp]
-        L.orc_lbvh_build.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
-        L.orc_karras.argtypes = [i32, vp, vp, vp, vp, vp]
-        L.orc_lbvh_trace.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
-        L.orc_lbvh4_packet_primary.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp]
-        L.orc_loss64.restype = C.c_double
-        L.orc_loss64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, vp]
-        L.orc_grad_fd64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, C.c_double, vp, i32, vp]
-        L.orc_grad_fwd64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, vp]
+        _LIB = _bind(C.CDLL(path))
     return _LIB
+
+
+def _bind(L):
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    L.orc_accel_build.restype = vp
+    L.orc_accel_build.argtypes = [vp, C.c_int]
+    L.orc_accel_free.argtypes = [vp]
+    L.orc_nearest.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp]
+    L.orc_render.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    L.orc_nearest_linear.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp]
+    L.orc_render_lbvh.argtypes = [vp, vp, i32, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, i32]
+    L.orc_mt.argtypes = [vp, vp, vp, vp]
+    L.orc_render_aa.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp]
+    L.orc_mat4_inverse.argtypes = [vp, vp]
+    L.orc_camera_ray.argtypes = [vp, C.c_int, C.c_int, vp]
+    L.orc_camera_ray_aa.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+    L.orc_quantize.restype = C.c_uint8
+    L.orc_quantize.argtypes = [C.c_float]
+    L.orc_morton3.restype = C.c_uint32
+    L.orc_morton3.argtypes = [C.c_uint32] * 3
+    L.orc_shade_phong.restype = C.c_float
+    L.orc_shade_phong.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
+    L.orc_morton_codes.argtypes = [i32, vp, vp, vp]
+    L.orc_lbvh_build.argtypes = [i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.orc_karras.argtypes = [i32, vp, vp, vp, vp, vp]
+    L.orc_lbvh_trace.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
+    L.orc_lbvh4_packet_primary.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp]
+    L.orc_loss64.restype = C.c_double
+    L.orc_loss64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, vp]
+    L.orc_grad_fd64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, C.c_double, vp, i32, vp]
+    L.orc_grad_fwd64.argtypes = [vp, i32, i64, vp, i32, i32, vp, vp, vp]
+    return L
 
 
 def _p(a):
@@ -305,7 +324,8 @@ def camera_rays(scene, camera=None):
     return out
 
 
-def render_lbvh(scene, b, camera=None, depth=None, row_start=0, row_step=1, width=2):
+def render_lbvh(scene, b, camera=None, depth=None, row_start=0, row_step=1, width=2, fast=False):
+    """The restatement of the GPU's LBVH walk and shading; fast: the timing build (lib_fast)."""
     o = Oracle.__new__(Oracle)
     o._accel = None
     s = _orc_scene(o, scene)
@@ -319,7 +339,7 @@ def render_lbvh(scene, b, camera=None, depth=None, row_start=0, row_step=1, widt
     nodes = np.ascontiguousarray(b["nodes4"] if width == 4 else b["nodes"], np.float32)
     tris = np.ascontiguousarray(b["tris"], np.float32)
     root = b["root4"] if width == 4 else b["root"]
-    lib().orc_render_lbvh(C.byref(s), _p(nodes), root, _p(tris), cam.ctypes.data, depth,
+    (lib_fast() if fast else lib()).orc_render_lbvh(C.byref(s), _p(nodes), root, _p(tris), cam.ctypes.data, depth,
                           row_start, row_step, _p(rgba), _p(rgb), _p(hits), C.byref(cnt), width)
     del o
     return {"rgba": rgba, "rgb": rgb, "hit": hits, "counts": cnt.as_dict()}
@@ -407,8 +427,6 @@ class RefPort:
     def __init__(self, scene, parity: bool = False):
         self.L = refport_lib(parity)
         self.scene = scene
-        if _xform_arrays(scene)[0]:
-            raise ValueError("the CPU baseline port has identity transformations only")
         self._s = _orc_scene(self, scene)
         bs = np.zeros(1, np.float64)
         self.h = self.L.rp_build(C.byref(self._s), bs.ctypes.data)

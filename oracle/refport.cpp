@@ -5,6 +5,7 @@
 // librefport_parity.so (IEEE, no contraction) for the parity test against the
 // reference's own images.
 #include "refport.h"
+#include "glm_mat4.h"
 
 #include <time.h>
 
@@ -97,6 +98,9 @@ struct Port {
     std::vector<int> surf;       // the reference's surface array (sorted in place by the build)
     std::vector<RNode> nodes;    // preorder
     M4 ident;
+    // per-transformation M, glm::inverse(M) and its transpose, as the reference's
+    // triangles point at them (src/triangleMesh.cpp:246-250); sized once
+    std::vector<M4> xm, xinv, xinvT;
     const orc_scene* s;
 };
 
@@ -434,6 +438,20 @@ void* rp_build(const orc_scene* s, double* build_seconds) {
     P->s = s;
     std::memset(&P->ident, 0, sizeof P->ident);
     for (int k = 0; k < 4; ++k) P->ident.c[k][k] = 1.0f;
+    const int nx = s->xform_index ? s->n_xform : 0;
+    P->xm.resize(nx);
+    P->xinv.resize(nx);
+    P->xinvT.resize(nx);
+    for (int x = 0; x < nx; ++x) {
+        float inv[16];
+        inverse_glm(s->xforms + 16 * (size_t)x, inv);
+        for (int c = 0; c < 4; ++c)
+            for (int r = 0; r < 4; ++r) {
+                P->xm[x].c[c][r] = s->xforms[16 * (size_t)x + 4 * c + r];
+                P->xinv[x].c[c][r] = inv[4 * c + r];
+                P->xinvT[x].c[c][r] = inv[4 * r + c];
+            }
+    }
     P->tris.resize(s->n_tri);
     for (int i = 0; i < s->n_tri; ++i) {
         const float* v = s->verts + 9 * (size_t)i;
@@ -452,6 +470,26 @@ void* rp_build(const orc_scene* s, double* build_seconds) {
             }
         }
         T.M = T.Minv = T.MinvT = &P->ident;
+        const int xi = nx > 0 ? s->xform_index[i] : -1;
+        if (xi >= 0) {
+            T.M = &P->xm[xi];
+            T.Minv = &P->xinv[xi];
+            T.MinvT = &P->xinvT[xi];
+            // world centroid (the sort comparators' M * centroid, src/BVH.cpp:36-71) and world
+            // box from the two corners of the local box (Box::transformBoundingBox, src/Box.cu:84-137)
+            const float c4[4] = {T.centroid.x, T.centroid.y, T.centroid.z, 1.0f};
+            const float lo4[4] = {T.box.b[0][0], T.box.b[0][1], T.box.b[0][2], 1.0f};
+            const float hi4[4] = {T.box.b[1][0], T.box.b[1][1], T.box.b[1][2], 1.0f};
+            float wc[4], wlo[4], whi[4];
+            mulv(*T.M, c4, wc);
+            mulv(*T.M, lo4, wlo);
+            mulv(*T.M, hi4, whi);
+            T.centroid = {wc[0], wc[1], wc[2]};
+            for (int k = 0; k < 3; ++k) {
+                T.box.b[0][k] = std::min(wlo[k], whi[k]);
+                T.box.b[1][k] = std::max(whi[k], wlo[k]);
+            }
+        }
         T.mat = s->mat[i];
     }
     P->surf.resize(s->n_tri);

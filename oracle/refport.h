@@ -12,7 +12,10 @@
  *        on the split axis, slab test over [0.001, 999] (src/BVH.cpp:609-716,
  *        src/Box.cu:228-256);
  *   per triangle: ray moved to local space by the inverse mat4, hit moved back
- *        by the mat4 and the inverse-transpose (src/BVH.cpp:342-383);
+ *        by the mat4 and the inverse-transpose (src/BVH.cpp:342-383), with the
+ *        scene's transformations (identity where a triangle has none); world
+ *        centroids and two-corner world boxes for the build (src/BVH.cpp:36-71,
+ *        src/Box.cu:84-137);
  *   shading and quantisation as src/RayTracer.cpp:99-174,268-458.
  * Deviations: the SAH sweep uses prefix/suffix box unions (same costs and the
  * same first-minimum split as the reference run with one OpenMP thread, in

@@ -3,7 +3,8 @@ stratified sample of rows (the oracle restates the same LBVH walk, so its
 rows are exact references), the LBVH bit-exact, ray counts consistent.
 
 Sizes: config B 1,000,000 triangles at 1920x1080, depth 3; config C
-4,194,304 triangles at 3840x2160, depth 3; config E 1,000,000 triangles at
+4,194,304 triangles at 3840x2160, depth 3 (and its hit-dense variant: the same
+mesh scaled by 4, most primary rays hitting); config E 1,000,000 triangles at
 1024x1024, depth 2; config D (64 scenes) batched against single frames.
 """
 import numpy as np
@@ -21,7 +22,7 @@ def _u32(a):
     return np.ascontiguousarray(a).view(np.uint32)
 
 
-@pytest.mark.parametrize("name", ["config_b", "config_c", "config_e"])
+@pytest.mark.parametrize("name", ["config_b", "config_c", "config_c_dense", "config_e"])
 def test_full_size_frame_on_sampled_rows(name):
     from eos_amd import _lib as E
     s = getattr(scenes, name)()
@@ -43,6 +44,8 @@ def test_full_size_frame_on_sampled_rows(name):
     st = go["stats"]
     assert st["primary_rays"] == s.width * s.height
     assert st["hits"] >= int((go["hit"] >= 0).sum())
+    if name == "config_c_dense":
+        assert (go["hit"] >= 0).mean() > 0.8          # a frame of hits, not of misses
 
 
 def test_full_size_batch_matches_single_frames():

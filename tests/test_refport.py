@@ -16,12 +16,7 @@ from oracle import pyoracle as po
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz")
 
 
-# the port times the reference's path on the world-space workloads; transformed
-# scenes are pinned through the oracle instead
-IDENTITY_SCENES = sorted(k for k, f in golden_scenes().items() if getattr(f(), "xforms", None) is None)
-
-
-@pytest.mark.parametrize("name", IDENTITY_SCENES)
+@pytest.mark.parametrize("name", sorted(golden_scenes()))
 def test_port_reproduces_reference_images(name):
     gold = np.load(GOLD, allow_pickle=False)
     s = golden_scenes()[name]()
