@@ -148,9 +148,26 @@ def bench_batched(E, torch, reps: int):
     from eos_amd import autotune, scenes
     t0 = time.perf_counter()
     data = scenes.config_d()
-    sc = [E.Scene(d, leaf_max=4) for d in data]
-    _progress(f"batch: {len(sc)} scenes, {sum(d.n_tri for d in data)} triangles built in "
+    sc = [E.Scene(d, build=False) for d in data]
+    _progress(f"batch: {len(sc)} scenes, {sum(d.n_tri for d in data)} triangles uploaded in "
               f"{time.perf_counter() - t0:.1f} s")
+    # LBVHs: 64 eos_build_lbvh calls (each synchronous) vs one eos_build_lbvh_batch;
+    # the second of each is timed (both rebuild the same trees in place)
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        single_gpu = sum(s.build(4)["build_ms"] for s in sc)
+        t_single = time.perf_counter() - t
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        bst = E.build_batch(sc, leaf_max=4)
+        t_batch = time.perf_counter() - t
+    build = {"scenes": len(sc), "triangles": int(sum(d.n_tri for d in data)),
+             "sequential_wall_ms": round(t_single * 1e3, 3), "sequential_gpu_ms": round(single_gpu, 3),
+             "batched_wall_ms": round(t_batch * 1e3, 3), "batched_gpu_ms": round(bst[0]["build_ms"], 3),
+             "speedup_wall": round(t_single / t_batch, 2)}
+    _progress(f"batch LBVH build: {build['batched_wall_ms']} ms vs {build['sequential_wall_ms']} ms for 64 builds")
     tuned = autotune.load()
     source = "eos_amd/tuned_launch.json"
     if tuned is None:
@@ -202,7 +219,7 @@ def bench_batched(E, torch, reps: int):
     return {"scenes": len(sc), "rays_per_frame": int(rays), "sequential_ms": round(t_seq * 1e3, 3),
             "batched_ms": round(t_bat * 1e3, 3), "sequential_mrays": round(rays / t_seq / 1e6, 2),
             "batched_mrays": round(rays / t_bat / 1e6, 2), "speedup": round(t_seq / t_bat, 3),
-            "launch_config": tuned, "launch_config_source": source, "hipgraph": True,
+            "launch_config": tuned, "launch_config_source": source, "hipgraph": True, "lbvh_build": build,
             "roofline": {"bound": "hbm", "scope": "whole batched frame", "alg_bytes": int(alg),
                          "node_visits": int(visits), "tri_tests": int(tests), "hits": int(hits),
                          "gpu_ms": round(gms, 4), "achieved": round(alg / (gms * 1e-3) / 1e9, 2),

@@ -24,7 +24,7 @@ EXPORTS = ["eos_create_scene", "eos_destroy_scene", "eos_build_lbvh", "eos_build
            "eos_set_launch_config", "eos_get_launch_config", "eos_debug_read", "eos_last_error",
            "eos_abi_version", "eos_device_name", "eos_mesh_load_obj", "eos_mesh_free", "eos_set_transforms",
            "eos_set_ambient_refractive_index", "eos_render_frames", "eos_render_rows", "eos_trace_rays",
-           "eos_trace_rays_static"]
+           "eos_trace_rays_static", "eos_build_lbvh_batch"]
 STATIC_MAX_RAYS = 1 << 20   # EOS_STATIC_MAX_RAYS
 
 # eos_trace_rays query modes and the eos_ray_hit record (48 bytes)
@@ -149,6 +149,7 @@ def lib():
         L.eos_render_rows.argtypes = [vp, vp, i32, i32, vp, vp, vp]
         L.eos_trace_rays.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         L.eos_trace_rays_static.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+        L.eos_build_lbvh_batch.argtypes = [vp, i32, vp, vp]
         for name in EXPORTS:
             if name != "eos_last_error" and name != "eos_abi_version":
                 getattr(L, name).restype = C.c_int
@@ -388,6 +389,20 @@ class Scene:
         out = np.zeros(shape, dt)
         check(lib().eos_debug_read(self.h, DBG[what], _ptr(out), out.nbytes))
         return out
+
+
+def build_batch(scenes, leaf_max: int = 4, width: int = 4) -> list:
+    """eos_build_lbvh_batch: the LBVHs of every scene in one launch sequence; per-scene build stats."""
+    hs = (C.c_void_p * len(scenes))(*[s.h.value for s in scenes])
+    st = (BuildStats * len(scenes))()
+    o = BuildOpts(leaf_max, width)
+    check(lib().eos_build_lbvh_batch(hs, len(scenes), C.byref(o), st))
+    out = []
+    for s, b in zip(scenes, st):
+        s.generation += 1
+        s.build_stats = {k: getattr(b, k) for k, _ in b._fields_}
+        out.append(s.build_stats)
+    return out
 
 
 def render_batch(scenes, cameras=None, depth=2, use_graph=False, device_ptrs=None):

@@ -339,6 +339,23 @@ eos_status eos_build_lbvh_ex(eos_scene* s, const eos_build_opts* o, eos_build_st
     return build_lbvh(s, o->leaf_max, o->width, st);
 }
 
+eos_status eos_build_lbvh_batch(eos_scene* const* scenes, int32_t n, const eos_build_opts* o, eos_build_stats* st) {
+    if (!scenes || !o || n <= 0) return fail(EOS_ERR_INVALID, "eos_build_lbvh_batch: bad argument");
+    if (o->leaf_max < 1 || o->leaf_max > 8)
+        return fail(EOS_ERR_INVALID, "eos_build_lbvh_batch: leaf_max must be in [1, 8]");
+    if (o->width != 2 && o->width != 4) return fail(EOS_ERR_INVALID, "eos_build_lbvh_batch: width must be 2 or 4");
+    if (n > 65536) return fail(EOS_ERR_LIMIT, "eos_build_lbvh_batch: at most 65536 scenes");
+    for (int i = 0; i < n; ++i) {
+        if (!scenes[i]) return fail(EOS_ERR_INVALID, "eos_build_lbvh_batch: null scene");
+        if (scenes[i]->device != scenes[0]->device)
+            return fail(EOS_ERR_INVALID, "eos_build_lbvh_batch: scenes on different devices");
+        for (int j = 0; j < i; ++j)
+            if (scenes[j] == scenes[i]) return fail(EOS_ERR_INVALID, "eos_build_lbvh_batch: a scene appears twice");
+    }
+    EOS_HIP(hipSetDevice(scenes[0]->device));
+    return build_lbvh_batch(scenes, n, o->leaf_max, o->width, st);
+}
+
 eos_status eos_build_lbvh(eos_scene* s, int32_t leaf_max, eos_build_stats* st) {
     eos_build_opts o{leaf_max, 4};
     return eos_build_lbvh_ex(s, &o, st);
@@ -526,6 +543,8 @@ eos_status eos_debug_read(const eos_scene* s, int32_t what, void* dst, int64_t b
         default: return fail(EOS_ERR_INVALID, "eos_debug_read: unknown array");
     }
     if (!s->built) return fail(EOS_ERR_STATE, "eos_debug_read: no LBVH built");
+    if (s->batch_built && (what == EOS_DBG_KCHILD || what == EOS_DBG_KPARENT || what == EOS_DBG_IBOX))
+        return fail(EOS_ERR_STATE, "eos_debug_read: a batched build keeps no per-scene Karras arrays or boxes");
     if ((size_t)bytes < need) return fail(EOS_ERR_INVALID, "eos_debug_read: destination too small");
     if (need == 0) return EOS_OK;
     EOS_HIP(hipSetDevice(s->device));

@@ -194,6 +194,7 @@ struct eos_scene {
     int width = 4;
     int leaf_max = 1;
     bool built = false;
+    bool batch_built = false;     // built by eos_build_lbvh_batch: the build's intermediate arrays are not kept
     eos::LaunchCfg cfg;
     int num_cu = 256;
     eos_frame_buffers fb;
@@ -218,6 +219,7 @@ struct eos_scene {
 namespace eos {
 // Implemented in lbvh.hip
 eos_status build_lbvh(eos_scene* s, int leaf_max, int width, eos_build_stats* st);
+eos_status build_lbvh_batch(eos_scene* const* scenes, int n, int leaf_max, int width, eos_build_stats* st);
 // Implemented in render.hip
 // Rows [row0, row1) of the camera's frame (row1 < 0: to the last row).
 eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,

@@ -156,6 +156,19 @@ eos_status eos_build_lbvh(eos_scene* scene, int32_t leaf_max, eos_build_stats* s
 /* Same, choosing the tree width (2 = binary nodes, 4 = 4-wide nodes). */
 eos_status eos_build_lbvh_ex(eos_scene* scene, const eos_build_opts* opts, eos_build_stats* stats);
 
+/* The LBVHs of n scenes built together: every stage of eos_build_lbvh_ex is one
+ * launch over the concatenated triangles of all scenes (one sort of (scene,
+ * Morton code, index), one Karras pass, one refit sequence, one 4-wide
+ * collapse), and each scene gets exactly the tree eos_build_lbvh_ex would build
+ * for it alone.  For many-scene workloads (eos_render_batch); the reference
+ * builds each scene's BVH on the CPU in Scene::flush (src/Scene.cpp:204-209).
+ * stats: n entries (build_ms = the whole batch's GPU
+ * time) or NULL.  After it, eos_debug_read gives the scene's Morton codes,
+ * permutation, nodes and triangle records, not the Karras arrays or boxes.
+ * Synchronous; calls must not overlap. */
+eos_status eos_build_lbvh_batch(eos_scene* const* scenes, int32_t n, const eos_build_opts* opts,
+                                eos_build_stats* stats);
+
 /* Per-triangle local-to-world transformations.  Replaces Scene::addTransformation
  * (src/Scene.cpp:96-105) with Triangle::setTransformationIndex (src/Triangle.h:59),
  * which createCudaScene copies to the device (copyTransformations and the
