@@ -123,6 +123,7 @@ struct LaunchCfg {
     int batch = 64;        // most items a traversal wave claims per fetch
     int packets = 0;       // primary rays as one 8x8 packet per wave (4-wide tree)
     int coop = 3;          // quad-cooperative walk for launches with at most one item per lane
+    int group = 1;         // group walk (16 lanes per ray) for launches with at most one item per 16 lanes
     // frames two in flight share the GPU: fewer waves per launch, larger claims
     int frames_waves_per_cu = 14;
     int frames_refill = 8;

@@ -313,6 +313,7 @@ eos_status trace_rays(eos_scene* s, int n, const float* o, const float* d, const
         a.refill = cfg.refill;
         a.batch = cfg.batch;
         a.coop = cfg.coop;
+        a.group = cfg.group;
         a.ray_o = ro;
         a.ray_d = rd;
         if (mode == EOS_QUERY_OCCLUDED) {

@@ -261,6 +261,7 @@ struct TraceArgs {
     int refill;
     int batch;               // most items a wave claims per fetch
     int coop;                // quad-cooperative walk: 0 never, 1..3 at most lanes/4, /2, /1 items, 4 always
+    int group;               // group walk (16 lanes per ray) for launches of at most group x lanes/16 items (0: never)
 };
 
 // Compare-exchange on (entry distance, slot): a total order, so the result does
@@ -628,6 +629,198 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
     }
 }
 
+// Group walk of the 4-wide tree: kGroupLanes (16) lanes share one ray and its
+// stack of pending nodes in LDS.  Every step pops up to four entries from the
+// top and quad q takes entry q: at an internal node lane k tests child k and
+// the quad pushes its hits farthest first, at a leaf lane k tests triangles k
+// and k + 4.  The group's least (t, original index) hit, or its any-hit flag,
+// is reduced over the 16 lanes after every step, so four nodes cost one
+// dependent step.  It visits nodes in another order than the per-lane walk (so
+// launches that count visits never take it), but every node whose box the ray
+// meets before its nearest hit so far is visited, so the result is the same
+// exact nearest hit, ties to the least original index, or the same occlusion
+// flag, bit for bit.  For launches with few rays, whose time is the longest
+// ray's chain of dependent steps.
+constexpr int kGroupLanes = 16;
+constexpr int kGroupQuads = kGroupLanes / 4;
+constexpr int kGroupStack = 512;   // entries per group: at most 16 pushes per step; overflow is an error
+
+// Items per launch at or below which the group walk is used, as a multiple of
+// the launch's lanes / 16 (1: one group per ray in one round; 4: four rounds...).
+__device__ __forceinline__ bool use_group(int rounds, long long n, long long lanes) {
+    return rounds > 0 && n * kGroupLanes <= lanes * (long long)rounds;
+}
+
+template <int MODE, int WG, bool BATCH, bool XF>
+__device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* wave_stack) {
+    constexpr bool ANY = any_hit_mode(MODE);
+    constexpr int kGroups = 64 / kGroupLanes;
+    const int lane = __lane_id();
+    const int gsub = lane & (kGroupLanes - 1), q = gsub >> 2, k = lane & 3;
+    const int grp = lane / kGroupLanes;
+    const int gbit = kGroupLanes * grp;
+    const uint64_t gmask = 0xFFFFull << gbit;
+    const int qbit = lane & ~3;                     // first lane of this lane's quad
+    int32_t* stk = wave_stack + grp * kGroupStack;
+    const int waves_total = (int)gridDim.x * (WG / 64);
+    Fetch f = fetch_start();
+    int item = -1;
+    size_t out_idx = 0;
+    F3 o = {0, 0, 0}, d = {0, 0, 0};
+    float ix = 0, iy = 0, iz = 0, px = 0, py = 0, pz = 0;
+    float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
+    int best_slot = -1, best_orig = 0x7FFFFFFF;
+    int sp = 0, budget = 0;
+    const Node4* nodes4 = a.sc.nodes4;
+    const float4* geo = a.sc.geo;
+    const float4* xf = a.sc.xf;
+    for (;;) {
+        // one bit per idle group, at its first lane
+        const uint64_t idle = __ballot(gsub == 0 && item < 0);
+        const int nidle = __popcll(idle);
+        if (nidle >= 1 && (!f.drained || f.next < f.end)) {
+            const int rank = __popcll(idle & ((1ull << gbit) - 1));
+            const int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
+            if (item < 0 && my >= 0) {
+                RayItem r;
+                setup_item<MODE, BATCH, 4>(a, my, r);
+                item = my;
+                out_idx = r.out_idx;
+                o = r.o;
+                d = r.d;
+                nodes4 = r.nodes4;
+                geo = r.geo;
+                xf = r.xf;
+                ix = safe_inv(d.x);
+                iy = safe_inv(d.y);
+                iz = safe_inv(d.z);
+                px = o.x * ix;
+                py = o.y * iy;
+                pz = o.z * iz;
+                best_t = INFINITY;
+                best_slot = -1;
+                best_orig = 0x7FFFFFFF;
+                bu = bv = 0.0f;
+                budget = r.max_visits;
+                sp = 0;
+                if (r.valid) {
+                    if (gsub == 0) stk[0] = r.root;
+                    sp = 1;
+                }
+            }
+        }
+        if (__ballot(item >= 0) == 0) {
+            if (f.drained && f.next >= f.end) break;
+            continue;
+        }
+        const int need = (f.drained && f.next >= f.end) ? kGroups : 1;
+        for (;;) {
+            if (item >= 0) {  // uniform within a group from here on
+                const int npop = sp < kGroupQuads ? sp : kGroupQuads;
+                const int ref = q < npop ? stk[sp - 1 - q] : kDone;
+                sp -= npop;
+                const bool inner = ref >= 0 && ref != kDone;
+                budget -= __popcll(__ballot(inner && k == 0) & gmask);
+                bool h = false, anyhit = false;
+                int rf = kEmpty;
+                float key = INFINITY;
+                float lt = INFINITY, lu = 0.0f, lv = 0.0f;
+                int lo_ = 0x7FFFFFFF, ls = -1;
+                if (inner) {
+                    const float* nd = reinterpret_cast<const float*>(nodes4 + ref);
+                    const float lx = nd[k], hx = nd[4 + k], ly = nd[8 + k], hy = nd[12 + k], lz = nd[16 + k],
+                                hz = nd[20 + k];
+                    rf = __float_as_int(nd[24 + k]);
+                    float t;
+                    h = slab(lx, hx, ly, hy, lz, hz, ix, iy, iz, px, py, pz, ANY ? INFINITY : best_t, t) &
+                        (rf != kEmpty);
+                    key = h ? t : INFINITY;
+                } else if (ref != kDone) {
+                    const int e = ~ref;
+                    const int first = e & ((1 << kLeafShift) - 1);
+                    const int cnt = ((e >> kLeafShift) & 7) + 1;
+                    for (int b = 0; b < cnt; b += 4) {
+                        const int kk = b + k;
+                        const bool act = kk < cnt;
+                        const int sl = first + (act ? kk : 0);
+                        const float4 g0 = geo[3 * (size_t)sl], g1 = geo[3 * (size_t)sl + 1], g2 = geo[3 * (size_t)sl + 2];
+                        float tt, uu, vv;
+                        F3 lo = o, ld = d;
+                        to_local<XF>(xf, __float_as_int(g2.z), lo, ld);
+                        const bool hit = moller_trumbore(lo, ld, f3(g0.x, g0.y, g0.z), f3(g0.w, g1.x, g1.y),
+                                                         f3(g1.z, g1.w, g2.x), tt, uu, vv) & act;
+                        if (ANY) {
+                            anyhit |= hit;
+                        } else {
+                            const int og = __float_as_int(g2.y);
+                            if (hit && (tt < lt || (tt == lt && og < lo_))) {
+                                lt = tt;
+                                lo_ = og;
+                                ls = sl;
+                                lu = uu;
+                                lv = vv;
+                            }
+                        }
+                    }
+                }
+                // pushes: each quad its hits farthest first, quads in order, above the popped entries
+                const float k1 = __shfl_xor(key, 1), k2 = __shfl_xor(key, 2), k3 = __shfl_xor(key, 3);
+                const int rank = (k1 < key || (k1 == key && (k ^ 1) < k)) + (k2 < key || (k2 == key && (k ^ 2) < k)) +
+                                 (k3 < key || (k3 == key && (k ^ 3) < k));
+                const uint64_t hm = __ballot(h);
+                const int nh_q = __popcll((hm >> qbit) & 0xFull);
+                const int before = __popcll(hm & gmask & ((1ull << qbit) - 1));
+                const int total = __popcll(hm & gmask);
+                bool done = false;
+                if (budget < 0) {
+                    if (gsub == 0) atomicOr(a.err, 2);
+                    done = true;
+                } else if (sp + total > kGroupStack) {
+                    if (gsub == 0) atomicOr(a.err, 1);
+                    done = true;
+                } else {
+                    if (h) stk[sp + before + (nh_q - 1 - rank)] = rf;
+                    sp += total;
+                }
+                if (ANY) {
+                    if (__ballot(anyhit) & gmask) {
+                        best_slot = 0;  // marks "hit" for the any-hit result
+                        done = true;
+                    }
+                } else {
+                    // least (t, original index) over the group, then against the ray's best
+                    for (int off = 1; off < kGroupLanes; off <<= 1) {
+                        const float ot = __shfl_xor(lt, off), ou = __shfl_xor(lu, off), ov = __shfl_xor(lv, off);
+                        const int oo = __shfl_xor(lo_, off), os = __shfl_xor(ls, off);
+                        if (ot < lt || (ot == lt && oo < lo_)) {
+                            lt = ot;
+                            lo_ = oo;
+                            ls = os;
+                            lu = ou;
+                            lv = ov;
+                        }
+                    }
+                    if (ls >= 0 && (lt < best_t || (lt == best_t && lo_ < best_orig))) {
+                        best_t = lt;
+                        best_orig = lo_;
+                        best_slot = ls;
+                        bu = lu;
+                        bv = lv;
+                    }
+                }
+                if (done || sp == 0) {
+                    if (gsub == 0) {
+                        if (ANY) a.vis_out[out_idx] = (best_slot >= 0) == (MODE == MODE_RAYS_ANY) ? 1 : 0;
+                        else a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
+                    }
+                    item = -1;
+                }
+            }
+            if ((int)__popcll(__ballot(gsub == 0 && item < 0)) >= need) break;
+        }
+    }
+}
+
 // Entries of a lane's traversal stack kept in registers, above the LDS and
 // global parts (0: the whole stack is in LDS / global memory).  Measured on the
 // 4K frame of the 4M-triangle sphere: 0 entries 2.18 ms for the primary launch,
@@ -704,7 +897,11 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     constexpr bool ANY = any_hit_mode(MODE);
     constexpr int MAXS = max_stack<WIDTH>();
     constexpr int LDS_LANE = STACK_LDS * WG, LDS_COOP = WIDTH == 4 ? (WG / 4) * kMaxStack4 : 0;
-    __shared__ int32_t lds[LDS_LANE > LDS_COOP ? LDS_LANE : LDS_COOP];
+    // the group walk serves the short launches after the primary rays, never the primary launch
+    constexpr bool GROUP = WIDTH == 4 && !COUNT && MODE != MODE_PRIMARY;
+    constexpr int LDS_GROUP = GROUP ? (WG / kGroupLanes) * kGroupStack : 0;
+    constexpr int LDS_A = LDS_LANE > LDS_COOP ? LDS_LANE : LDS_COOP;
+    __shared__ int32_t lds[LDS_A > LDS_GROUP ? LDS_A : LDS_GROUP];
     const int tid = threadIdx.x;
     const int lane = __lane_id();
     int n = a.n_static;
@@ -714,7 +911,21 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         for (int k = 0; k < nlev; ++k) n += a.count_ptr[k] * a.count_mul;
     }
     unsigned long long visits = 0, tests = 0;
-    if (WIDTH == 4 && use_coop(a.coop, n, (long long)gridDim.x * WG)) {
+    const long long lanes = (long long)gridDim.x * WG;
+    const bool grouped = GROUP && use_group(a.group, n, lanes);
+    const bool coop = !grouped && WIDTH == 4 && use_coop(a.coop, n, lanes);
+    // Waves beyond what the launch's items can occupy at once leave now: a
+    // launch with a few items on the full persistent grid would otherwise have
+    // every wave claim from every range cursor before it finds nothing left.
+    {
+        const long long per_wave = grouped ? 64 / kGroupLanes : (coop ? 16 : 64);
+        const long long need = (n + per_wave - 1) / per_wave;
+        const int wave_id = (int)blockIdx.x * (WG / 64) + (tid >> 6);
+        if (wave_id >= (need > 0 ? need : 1)) return;
+    }
+    if (grouped) {
+        trace_group<MODE, WG, BATCH, XF>(a, n, lds + (tid >> 6) * (64 / kGroupLanes) * kGroupStack);
+    } else if (coop) {
         trace_coop<MODE, WG, COUNT, BATCH, XF>(a, n, lds + (tid >> 6) * 16 * kMaxStack4, visits, tests);
     } else {
         int32_t* gst = a.gstack + (size_t)(blockIdx.x * WG + tid) * (MAXS - STACK_LDS);

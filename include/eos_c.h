@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EOS_ABI_VERSION 2
+#define EOS_ABI_VERSION 3
 
 typedef enum eos_status {
     EOS_OK = 0,
@@ -359,6 +359,9 @@ typedef struct eos_launch_config {
     int32_t frames_waves_per_cu;
     int32_t frames_refill;
     int32_t frames_batch;
+    int32_t group;            /* launches walked by groups of 16 lanes per ray (4-wide tree): 0 never; g > 0:
+                                 launches with at most g x lanes/16 items (the short launches after the primary
+                                 rays, whose time is their longest ray's chain of steps) */
 } eos_launch_config;
 
 eos_status eos_set_launch_config(eos_scene* scene, const eos_launch_config* cfg);

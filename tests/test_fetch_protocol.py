@@ -19,13 +19,17 @@ def _c_div(a, b):
     return q if (a >= 0) == (b > 0) else -q
 
 
-def _run(n, n_waves, thresh, batch, seed):
+def _run(n, n_waves, thresh, batch, seed, grid_waves=None):
+    """n_waves take part (the launch's other waves, up to grid_waves, left at
+    once because the items cannot occupy them); range of wave w: its
+    workgroup's index mod 8 (four waves per workgroup)."""
     rnd = random.Random(seed)
     cursors = [0] * RANGES
     issued = [0] * n
-    waves = [dict(item=[-1] * 64, drained=False, nxt=0, end=0, seen=0, rng=w % RANGES, tried=0, done=False)
+    waves = [dict(item=[-1] * 64, drained=False, nxt=0, end=0, seen=0,
+                  rng=(w // 4 if grid_waves else w) % RANGES, tried=0, done=False)
              for w in range(n_waves)]
-    wpr = max(1, n_waves // RANGES)
+    wpr = max(1, (grid_waves or n_waves) // RANGES)
     steps = 0
     while not all(w["done"] for w in waves):
         steps += 1
@@ -84,3 +88,13 @@ def test_every_item_once_and_every_wave_ends(n, n_waves):
     for thresh in (1, 16, 64):
         for batch in (16, 128):
             _run(n, n_waves, thresh, batch, n * 31 + n_waves * 7 + thresh + batch)
+
+
+@pytest.mark.parametrize("n", [1, 3, 40, 200, 5000])
+def test_launches_with_few_items_use_only_the_waves_they_need(n):
+    """A launch with few items on the full persistent grid (5120 waves): only
+    the first ceil(n / items per wave) waves take part (ranges by workgroup),
+    chunk sizes still computed for the whole grid; every item once."""
+    for per_wave in (4, 16, 64):
+        need = max(1, -(-n // per_wave))
+        _run(n, min(need, 64), 1, 128, n + per_wave, grid_waves=5120)
