@@ -122,12 +122,14 @@ struct LaunchCfg {
     int refill = 16;
     int batch = 64;        // most items a traversal wave claims per fetch
     int packets = 0;       // primary rays as one 8x8 packet per wave (4-wide tree)
-    int coop = 3;          // quad-cooperative walk for launches with at most one item per lane
+    int coop = 0;          // quad-cooperative walk (0: never; work stealing serves the short launches)
     int group = 1;         // group walk (16 lanes per ray) for launches with at most one item per 16 lanes
+    int steal = 1;         // per-lane walk: idle lanes of a drained wave help its remaining rays
+    int scatter = 4;       // launches of at most four items per lane claim them in scattered order
     // frames two in flight share the GPU: fewer waves per launch, larger claims
     int frames_waves_per_cu = 14;
     int frames_refill = 8;
-    int frames_batch = 128;
+    int frames_batch = 64;
 };
 
 }  // namespace eos

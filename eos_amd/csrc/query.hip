@@ -314,6 +314,8 @@ eos_status trace_rays(eos_scene* s, int n, const float* o, const float* d, const
         a.batch = cfg.batch;
         a.coop = cfg.coop;
         a.group = cfg.group;
+        a.steal = cfg.steal;
+        a.scatter = cfg.scatter;
         a.ray_o = ro;
         a.ray_d = rd;
         if (mode == EOS_QUERY_OCCLUDED) {

@@ -614,6 +614,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             sh.batch = f.cfg.batch;
             sh.coop = f.cfg.coop;
             sh.group = f.cfg.group;
+            sh.steal = f.count ? 0 : f.cfg.steal;
+            sh.scatter = f.cfg.scatter;
             sh.cursor = cursors(16 + l0);
             sh.stats = stats + 2 * (16 + l0);
             sh.count_ptr = ctr + CTR_HITS + l0;
@@ -665,6 +667,8 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         ta.batch = f.cfg.batch;
         ta.coop = f.cfg.coop;
         ta.group = f.cfg.group;
+        ta.steal = f.count ? 0 : f.cfg.steal;
+        ta.scatter = f.cfg.scatter;
         ta.cursor = cursors(l);
         ta.stats = stats + 2 * l;
         if (l == 0) {
@@ -1117,7 +1121,8 @@ bool same_batch(eos_scene* const* scenes, const eos_camera* cams, int n, int dep
     if ((int)b.scenes.size() != n || b.depth != depth) return false;
     if (b.cfg.workgroup != cfg.workgroup || b.cfg.stack_lds != cfg.stack_lds || b.cfg.waves_per_cu != cfg.waves_per_cu ||
         b.cfg.refill != cfg.refill || b.cfg.batch != cfg.batch || b.cfg.packets != cfg.packets ||
-        b.cfg.coop != cfg.coop || b.cfg.group != cfg.group)
+        b.cfg.coop != cfg.coop || b.cfg.group != cfg.group || b.cfg.steal != cfg.steal ||
+        b.cfg.scatter != cfg.scatter)
         return false;
     for (int i = 0; i < n; ++i) {
         if (b.scenes[i] != scenes[i]) return false;

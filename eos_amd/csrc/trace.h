@@ -155,6 +155,23 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Position of the r-th set bit (from 0) of m, for r < popcount(m).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
+    int base = 0;
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint64_t lo = m & ((1ull << w) - 1);
+        const int c = __popcll(lo);
+        if (r >= c) {
+            r -= c;
+            m >>= w;
+            base += w;
+        } else {
+            m = lo;
+        }
+    }
+    return base;
+}
+
 // Wave-aggregated append: one atomic per wave, lanes get consecutive slots in
 // lane order.  Every lane of the wave must call it.
 __device__ __forceinline__ int wave_append(int* ctr, bool pred) {
@@ -262,6 +279,8 @@ struct TraceArgs {
     int batch;               // most items a wave claims per fetch
     int coop;                // quad-cooperative walk: 0 never, 1..3 at most lanes/4, /2, /1 items, 4 always
     int group;               // group walk (16 lanes per ray) for launches of at most group x lanes/16 items (0: never)
+    int steal;               // per-lane walk: idle lanes of a drained wave take pending subtrees of its rays
+    int scatter;             // launches of at most scatter x lanes items claim them in scattered order (0: never)
 };
 
 // Compare-exchange on (entry distance, slot): a total order, so the result does
@@ -346,6 +365,20 @@ __device__ __forceinline__ int fetch_items(Fetch& f, const TraceArgs& a, int n, 
         f.end = take > used ? base + take : 0;
     }
     return my;
+}
+
+// Scattered claim order of a short launch: claim i is item (i * p) mod n, with
+// p a prime that does not divide n (1000003 and 999983 cannot both divide an
+// n below 2^31), a bijection on [0, n).  Items next to each other -- often all
+// long, as where shadow rays graze a surface -- go to different waves, so each
+// wave holds few long rays and its idle lanes can share them out.  The item
+// number keeps naming the ray and its outputs; only the order of claims changes.
+__device__ __forceinline__ int scatter_item(int i, int n) {
+    const long long p = (n % 1000003 != 0) ? 1000003 : 999983;
+    return (int)(((long long)i * p) % n);
+}
+__device__ __forceinline__ bool use_scatter(int rounds, long long n, long long lanes) {
+    return rounds > 0 && n <= lanes * (long long)rounds;
 }
 
 // One traversal item, set up: the ray and the scene it runs in.
@@ -468,6 +501,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
     const int gbit = 4 * grp;
     int32_t* stk = wave_stack + grp * kMaxStack4;
     const int waves_total = (int)gridDim.x * (WG / 64);
+    const bool scat = use_scatter(a.scatter, n, (long long)gridDim.x * WG);
     int gthresh = (a.refill + 3) / 4;
     gthresh = gthresh < 1 ? 1 : (gthresh > 16 ? 16 : gthresh);
     Fetch f = fetch_start();
@@ -487,7 +521,8 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
         const int nidle = __popcll(idle);
         if (nidle >= gthresh && (!f.drained || f.next < f.end)) {
             const int rank = __popcll(idle & ((1ull << gbit) - 1));
-            const int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
+            int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
+            if (scat && my >= 0) my = scatter_item(my, n);
             if (item < 0 && my >= 0) {
                 RayItem r;
                 setup_item<MODE, BATCH, 4>(a, my, r);
@@ -663,6 +698,7 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
     const int qbit = lane & ~3;                     // first lane of this lane's quad
     int32_t* stk = wave_stack + grp * kGroupStack;
     const int waves_total = (int)gridDim.x * (WG / 64);
+    const bool scat = use_scatter(a.scatter, n, (long long)gridDim.x * WG);
     Fetch f = fetch_start();
     int item = -1;
     size_t out_idx = 0;
@@ -680,7 +716,8 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
         const int nidle = __popcll(idle);
         if (nidle >= 1 && (!f.drained || f.next < f.end)) {
             const int rank = __popcll(idle & ((1ull << gbit) - 1));
-            const int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
+            int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
+            if (scat && my >= 0) my = scatter_item(my, n);
             if (item < 0 && my >= 0) {
                 RayItem r;
                 setup_item<MODE, BATCH, 4>(a, my, r);
@@ -877,6 +914,20 @@ struct LaneStack {
         else --nreg;
         return top;
     }
+    // Removes and returns the bottom entry (the oldest: the farthest sibling
+    // pushed nearest the root, usually the largest pending subtree); the
+    // entries above move down one place, keeping their order.  depth() > 0,
+    // memory-only stacks (R == 0).
+    __device__ __forceinline__ int take_bottom(int32_t* lds, int32_t* gst, int tid) {
+        const int v = STACK_LDS > 0 ? lds[tid] : gst[0];
+        for (int j = 1; j < sp; ++j) {
+            const int x = j < STACK_LDS ? lds[j * WG + tid] : gst[j - STACK_LDS];
+            if (j - 1 < STACK_LDS) lds[(j - 1) * WG + tid] = x;
+            else gst[j - 1 - STACK_LDS] = x;
+        }
+        --sp;
+        return v;
+    }
 };
 
 // Persistent traversal.  Each wave keeps up to 64 independent rays in flight
@@ -899,6 +950,8 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     constexpr int LDS_LANE = STACK_LDS * WG, LDS_COOP = WIDTH == 4 ? (WG / 4) * kMaxStack4 : 0;
     // the group walk serves the short launches after the primary rays, never the primary launch
     constexpr bool GROUP = WIDTH == 4 && !COUNT && MODE != MODE_PRIMARY;
+    // work stealing visits nodes in another order (never in launches that count visits)
+    constexpr bool STEAL = !COUNT && kRegStack == 0;
     constexpr int LDS_GROUP = GROUP ? (WG / kGroupLanes) * kGroupStack : 0;
     constexpr int LDS_A = LDS_LANE > LDS_COOP ? LDS_LANE : LDS_COOP;
     __shared__ int32_t lds[LDS_A > LDS_GROUP ? LDS_A : LDS_GROUP];
@@ -937,6 +990,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         size_t out_idx = 0;
         Fetch f = fetch_start();
         const int waves_total = (int)gridDim.x * (WG / 64);
+        const bool scat = use_scatter(a.scatter, n, (long long)gridDim.x * WG);
         F3 o = {0, 0, 0}, d = {0, 0, 0};
         float ix = 0, iy = 0, iz = 0, px = 0, py = 0, pz = 0;
         float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
@@ -950,13 +1004,33 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         int lxi = -1;
         F3 lo_o = {0, 0, 0}, lo_d = {0, 0, 0};
         const int thresh = a.refill < 1 ? 1 : (a.refill > 64 ? 64 : a.refill);
+        // Work stealing once the wave's items are all claimed: a lane with a ray
+        // (its home lane) lends the bottom entry of its stack to an idle lane,
+        // which walks that subtree for the same ray as a helper and hands its
+        // result back.  home: the home lane of the ray a helper works on (-1: this
+        // lane is the ray's home, or idle); pending: helpers of a home's ray not yet
+        // merged; fin: a helper whose part is walked.
+        const bool steal = STEAL && a.steal != 0;
+        int home = -1, pending = 0;
+        bool fin = false;
+        // the ray's answer, written by its home lane
+        auto emit = [&]() {
+            if (ANY) {
+                a.vis_out[out_idx] = (best_slot >= 0) == (MODE == MODE_RAYS_ANY) ? 1 : 0;
+            } else {
+                a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
+            }
+            item = -1;
+            pending = 0;
+        };
 
         for (;;) {
             // ---- refill idle lanes
             const uint64_t idle = __ballot(item < 0);
             const int nidle = __popcll(idle);
             if (nidle >= thresh && (!f.drained || f.next < f.end)) {
-                const int my = fetch_items(f, a, n, waves_total, idle, nidle, __popcll(idle & lanemask_lt()));
+                int my = fetch_items(f, a, n, waves_total, idle, nidle, __popcll(idle & lanemask_lt()));
+                if (scat && my >= 0) my = scatter_item(my, n);
                 if (item < 0 && my >= 0) {
                     RayItem r;
                     setup_item<MODE, BATCH, WIDTH>(a, my, r);
@@ -983,6 +1057,8 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     lxi = -1;
                     budget = r.max_visits;
                     cur = r.valid ? r.root : kDone;
+                    home = -1;
+                    pending = 0;
                 }
             }
             if (__ballot(item >= 0) == 0) {
@@ -993,7 +1069,19 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
             // left to claim), so finished lanes get new items without waiting for the
             // longest ray in the wave
             const int need = (f.drained && f.next >= f.end) ? 64 : thresh;
+            const bool stealing = steal && need == 64;
             for (;;) {
+                if (ANY && stealing) {
+                    // a helper stops once its ray is answered: the home lane left
+                    // it (an occluder was found) or took another role
+                    const int hl = home >= 0 ? home : lane;
+                    const int hi = __shfl(item, hl), hh = __shfl(home, hl);
+                    if (home >= 0 && item >= 0 && (hi != item || hh >= 0)) {
+                        cur = kDone;
+                        stk.clear();
+                        lpos = -1;
+                    }
+                }
                 if (item >= 0) {
                     if (cur >= 0 && cur != kDone && --budget < 0) {
                         atomicOr(a.err, 2);
@@ -1134,12 +1222,109 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                         }
                     }
                     if (cur == kDone) {
-                        if (ANY) {
-                            a.vis_out[out_idx] = (best_slot >= 0) == (MODE == MODE_RAYS_ANY) ? 1 : 0;
-                        } else {
-                            a.hit_out[item] = make_float4(best_t, __int_as_float(best_slot), bu, bv);
+                        if (STEAL && home >= 0) fin = true;   // merged below
+                        else if (!STEAL || pending == 0 || (ANY && best_slot >= 0)) emit();
+                        // else: a home whose helpers still walk parts of its ray waits for them
+                    }
+                }
+                if (STEAL && stealing) {
+                    // ---- merge the helpers that finished into their home lanes, one at a time
+                    uint64_t fm = __ballot(fin);
+                    while (fm) {
+                        const int src = __ffsll((unsigned long long)fm) - 1;
+                        fm &= fm - 1;
+                        const int h = __builtin_amdgcn_readlane(home, src);
+                        const int it = __builtin_amdgcn_readlane(item, src);
+                        const int fs = __builtin_amdgcn_readlane(best_slot, src);
+                        const int fo = __builtin_amdgcn_readlane(best_orig, src);
+                        const float ft = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(best_t), src));
+                        const float fu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bu), src));
+                        const float fv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bv), src));
+                        if (lane == h && item == it && home < 0) {
+                            --pending;
+                            if (ANY) {
+                                if (fs >= 0) {
+                                    best_slot = 0;
+                                    cur = kDone;
+                                    stk.clear();
+                                    lpos = -1;
+                                }
+                            } else if (fs >= 0 && (ft < best_t || (ft == best_t && fo < best_orig))) {
+                                best_t = ft;
+                                best_orig = fo;
+                                best_slot = fs;
+                                bu = fu;
+                                bv = fv;
+                            }
+                            if (cur == kDone && (pending == 0 || (ANY && best_slot >= 0))) emit();
                         }
+                    }
+                    if (fin) {
+                        fin = false;
+                        home = -1;
                         item = -1;
+                    }
+                    // ---- idle lanes take the bottom stack entry of walking lanes
+                    const uint64_t idle = __ballot(item < 0);
+                    const uint64_t dm = __ballot(item >= 0 && cur != kDone && stk.depth() > 0);
+                    if (idle != 0 && dm != 0) {
+                        const int ni = __popcll(idle), nd = __popcll(dm);
+                        const int np = ni < nd ? ni : nd;
+                        const int irank = __popcll(idle & lanemask_lt());
+                        const bool thief = ((idle >> lane) & 1) && irank < np;
+                        const bool donor = ((dm >> lane) & 1) && __popcll(dm & lanemask_lt()) < np;
+                        const int src = thief ? nth_set_bit(dm, irank) : lane;
+                        int give = kDone;
+                        if (donor) give = stk.take_bottom(lds, gst, tid);
+                        const int g_ref = __shfl(give, src), g_item = __shfl(item, src);
+                        const int g_home = __shfl(home >= 0 ? home : lane, src);
+                        const float g_ox = __shfl(o.x, src), g_oy = __shfl(o.y, src), g_oz = __shfl(o.z, src);
+                        const float g_dx = __shfl(d.x, src), g_dy = __shfl(d.y, src), g_dz = __shfl(d.z, src);
+                        const float g_t = __shfl(best_t, src);
+                        const int g_orig = __shfl(best_orig, src), g_budget = __shfl(budget, src);
+                        const size_t g_out = (size_t)__shfl((long long)out_idx, src);
+                        if (BATCH) {
+                            const Node* g_nodes = (const Node*)__shfl((long long)nodes, src);
+                            const Node4* g_nodes4 = (const Node4*)__shfl((long long)nodes4, src);
+                            const float4* g_geo = (const float4*)__shfl((long long)geo, src);
+                            const float4* g_xf = (const float4*)__shfl((long long)xf, src);
+                            if (thief) {
+                                nodes = g_nodes;
+                                nodes4 = g_nodes4;
+                                geo = g_geo;
+                                xf = g_xf;
+                            }
+                        }
+                        if (thief) {
+                            item = g_item;
+                            home = g_home;
+                            out_idx = g_out;
+                            cur = g_ref;
+                            o = f3(g_ox, g_oy, g_oz);
+                            d = f3(g_dx, g_dy, g_dz);
+                            ix = safe_inv(d.x);
+                            iy = safe_inv(d.y);
+                            iz = safe_inv(d.z);
+                            px = o.x * ix;
+                            py = o.y * iy;
+                            pz = o.z * iz;
+                            // only hits better than the lender's best so far count
+                            best_t = g_t;
+                            best_orig = g_orig;
+                            best_slot = -1;
+                            bu = bv = 0.0f;
+                            budget = g_budget;
+                            stk.clear();
+                            lpos = -1;
+                            lxi = -1;
+                            pending = 0;
+                        }
+                        uint64_t tm = __ballot(thief);
+                        while (tm) {
+                            const int s2 = __ffsll((unsigned long long)tm) - 1;
+                            tm &= tm - 1;
+                            if (lane == __builtin_amdgcn_readlane(home, s2)) ++pending;
+                        }
                     }
                 }
                 if ((int)__popcll(__ballot(item < 0)) >= need) break;

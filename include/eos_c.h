@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EOS_ABI_VERSION 3
+#define EOS_ABI_VERSION 4
 
 typedef enum eos_status {
     EOS_OK = 0,
@@ -362,6 +362,12 @@ typedef struct eos_launch_config {
     int32_t group;            /* launches walked by groups of 16 lanes per ray (4-wide tree): 0 never; g > 0:
                                  launches with at most g x lanes/16 items (the short launches after the primary
                                  rays, whose time is their longest ray's chain of steps) */
+    int32_t steal;            /* 1: once a traversal wave has claimed its last rays, its idle lanes walk
+                                 pending subtrees of the rays still walking (launches that count visits
+                                 never do); 0: each ray is walked by one lane to the end */
+    int32_t scatter;          /* launches with at most scatter x lanes items claim them in a scattered
+                                 order (item i * p mod n, p prime), so neighbouring rays -- often long
+                                 together -- go to different waves; 0: never.  Results do not change */
 } eos_launch_config;
 
 eos_status eos_set_launch_config(eos_scene* scene, const eos_launch_config* cfg);

@@ -256,3 +256,70 @@ def test_group_walk_in_batches():
     outs, _ = E.render_batch(sc, depth=2, use_graph=True)
     for a, b in zip(outs, ref):
         assert np.array_equal(a, b)
+
+
+STEAL_CFGS = (
+    {"steal": 1, "scatter": 0},
+    {"steal": 0, "scatter": 64},
+    {"steal": 1, "scatter": 64},
+    {"steal": 1, "scatter": 64, "waves_per_cu": 1, "refill": 64, "batch": 4096},
+    {"steal": 1, "scatter": 1, "waves_per_cu": 2, "refill": 1, "batch": 1},
+    {"steal": 1, "scatter": 4, "group": 1, "coop": 0},
+)
+
+
+@pytest.mark.parametrize("name", sorted(GROUP_CASES))
+def test_work_stealing_equals_plain_walk(name):
+    """Work stealing (idle lanes of a drained wave walk pending subtrees of its
+    remaining rays, results merged into each ray's home lane) and scattered
+    claim order give bit-identical frames, ray counts and query answers, with
+    every launch shape from one ray per wave to many rounds of rays per lane."""
+    from eos_amd import _lib as E
+    s = GROUP_CASES[name]()
+    print(f"work stealing vs plain walk case={name}", flush=True)
+    g = E.Scene(s, leaf_max=4)
+    plain = {"steal": 0, "scatter": 0, "group": 0, "coop": 0}
+    g.set_launch_config(**plain)
+    base = g.render(want_rgb=True, want_hits=True)
+    rng = np.random.default_rng(5)
+    v = s.verts.reshape(-1, 3)
+    o = (v.min(0) + (v.max(0) - v.min(0)) * rng.random((3000, 3))).astype(np.float32)
+    d = rng.normal(size=(3000, 3)).astype(np.float32)
+    qa, qo = g.trace_rays(o, d, "nearest"), g.trace_rays(o, d, "occluded")
+    for cfg in STEAL_CFGS:
+        g.set_launch_config()
+        g.set_launch_config(**{**plain, **cfg})
+        print(f"  cfg {cfg}", flush=True)
+        r = g.render(want_rgb=True, want_hits=True)
+        assert np.array_equal(r["hit"], base["hit"]), cfg
+        assert np.array_equal(r["rgba"], base["rgba"]), cfg
+        assert np.array_equal(r["rgb"].view(np.uint32), base["rgb"].view(np.uint32)), cfg
+        for k in ("shadow_rays", "secondary_rays", "hits"):
+            assert r["stats"][k] == base["stats"][k], (cfg, k)
+        assert np.array_equal(g.trace_rays(o, d, "nearest").view(np.uint8), qa.view(np.uint8)), cfg
+        assert np.array_equal(g.trace_rays(o, d, "occluded"), qo), cfg
+    g.set_launch_config()
+
+
+def test_work_stealing_full_frame_and_batches():
+    """The 4K frame of the 4M-triangle sphere (long grazing rays: the case work
+    stealing is for) and batched frames: identical with and without stealing."""
+    from eos_amd import _lib as E
+    s = scenes.config_c()
+    print("work stealing on the 4K sphere frame", flush=True)
+    g = E.Scene(s, leaf_max=4)
+    g.set_launch_config(steal=0, scatter=0)
+    a = g.render(want_hits=True)
+    g.set_launch_config()
+    b = g.render(want_hits=True)
+    assert np.array_equal(a["hit"], b["hit"]) and np.array_equal(a["rgba"], b["rgba"])
+    assert a["stats"]["shadow_rays"] == b["stats"]["shadow_rays"]
+    del g
+    data = scenes.config_d(n_scenes=6, scale=0.02)
+    sc = [E.Scene(d, leaf_max=4) for d in data]
+    sc[0].set_launch_config(steal=0, scatter=0)
+    ref, _ = E.render_batch(sc, depth=2)
+    sc[0].set_launch_config(steal=1, scatter=64)
+    outs, _ = E.render_batch(sc, depth=2, use_graph=True)
+    for x, y in zip(outs, ref):
+        assert np.array_equal(x, y)
