@@ -367,19 +367,51 @@ __device__ __forceinline__ int fetch_items(Fetch& f, const TraceArgs& a, int n, 
     return my;
 }
 
-// Scattered claim order of a short launch: claim i is item (i * p) mod n, with
-// p a prime that does not divide n (1000003 and 999983 cannot both divide an
-// n below 2^31), a bijection on [0, n).  Items next to each other -- often all
-// long, as where shadow rays graze a surface -- go to different waves, so each
-// wave holds few long rays and its idle lanes can share them out.  The item
-// number keeps naming the ray and its outputs; only the order of claims changes.
-__device__ __forceinline__ int scatter_item(int i, int n) {
-    const long long p = (n % 1000003 != 0) ? 1000003 : 999983;
-    return (int)(((long long)i * p) % n);
+// Scattered claim order of a short launch.  Items come in blocks of b (a
+// shadow launch: the light slots of one hit surface, which share their first
+// nodes); claim i of block k is item (k * p mod m) * b + i mod b for the m
+// whole blocks (the items after them keep their place), with p the first
+// integer from floor(m / golden ratio) up that is coprime to m: a bijection on [0, n) whose
+// neighbouring claims land far apart and evenly spread (Fibonacci hashing).
+// Items next to each other -- often all long, as where shadow rays graze a
+// surface -- so go to different waves, each wave holds few long rays and its
+// idle lanes can share them out.  The item number keeps naming the ray and its
+// outputs; only the order of claims changes.
+__device__ __forceinline__ unsigned gcd_u32(unsigned a, unsigned b) {  // binary (Stein) gcd
+    if (a == 0) return b;
+    if (b == 0) return a;
+    const int sh = __builtin_ctz(a | b);
+    a >>= __builtin_ctz(a);
+    do {
+        b >>= __builtin_ctz(b);
+        if (a > b) {
+            const unsigned t = a;
+            a = b;
+            b = t;
+        }
+        b -= a;
+    } while (b != 0);
+    return a << sh;
 }
-__device__ __forceinline__ bool use_scatter(int rounds, long long n, long long lanes) {
-    return rounds > 0 && n <= lanes * (long long)rounds;
-}
+struct Scatter {
+    bool on = false;
+    int b = 1, m = 0;
+    long long p = 1;
+    __device__ __forceinline__ void init(int rounds, int n, long long lanes, int block) {
+        on = rounds > 0 && n > 1 && (long long)n <= lanes * (long long)rounds;
+        b = block > 1 ? block : 1;
+        m = n / b;
+        unsigned q = (unsigned)((double)m * 0.6180339887498949);
+        q = q < 1 ? 1 : q;
+        while (m > 1 && gcd_u32(q, (unsigned)m) != 1) ++q;   // ends at m - 1 at the latest
+        p = q;
+    }
+    __device__ __forceinline__ int map(int i) const {
+        if (!on || i < 0 || i >= m * b) return i;
+        const int k = i / b, w = i - k * b;
+        return (int)(((long long)k * p) % m) * b + w;
+    }
+};
 
 // One traversal item, set up: the ray and the scene it runs in.
 struct RayItem {
@@ -501,7 +533,8 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
     const int gbit = 4 * grp;
     int32_t* stk = wave_stack + grp * kMaxStack4;
     const int waves_total = (int)gridDim.x * (WG / 64);
-    const bool scat = use_scatter(a.scatter, n, (long long)gridDim.x * WG);
+    Scatter scat;
+    scat.init(a.scatter, n, (long long)gridDim.x * WG, MODE == MODE_SHADOW ? a.count_mul : 1);
     int gthresh = (a.refill + 3) / 4;
     gthresh = gthresh < 1 ? 1 : (gthresh > 16 ? 16 : gthresh);
     Fetch f = fetch_start();
@@ -522,7 +555,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
         if (nidle >= gthresh && (!f.drained || f.next < f.end)) {
             const int rank = __popcll(idle & ((1ull << gbit) - 1));
             int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
-            if (scat && my >= 0) my = scatter_item(my, n);
+            my = scat.map(my);
             if (item < 0 && my >= 0) {
                 RayItem r;
                 setup_item<MODE, BATCH, 4>(a, my, r);
@@ -698,7 +731,8 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
     const int qbit = lane & ~3;                     // first lane of this lane's quad
     int32_t* stk = wave_stack + grp * kGroupStack;
     const int waves_total = (int)gridDim.x * (WG / 64);
-    const bool scat = use_scatter(a.scatter, n, (long long)gridDim.x * WG);
+    Scatter scat;
+    scat.init(a.scatter, n, (long long)gridDim.x * WG, MODE == MODE_SHADOW ? a.count_mul : 1);
     Fetch f = fetch_start();
     int item = -1;
     size_t out_idx = 0;
@@ -717,7 +751,7 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
         if (nidle >= 1 && (!f.drained || f.next < f.end)) {
             const int rank = __popcll(idle & ((1ull << gbit) - 1));
             int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
-            if (scat && my >= 0) my = scatter_item(my, n);
+            my = scat.map(my);
             if (item < 0 && my >= 0) {
                 RayItem r;
                 setup_item<MODE, BATCH, 4>(a, my, r);
@@ -990,7 +1024,8 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         size_t out_idx = 0;
         Fetch f = fetch_start();
         const int waves_total = (int)gridDim.x * (WG / 64);
-        const bool scat = use_scatter(a.scatter, n, (long long)gridDim.x * WG);
+        Scatter scat;
+        scat.init(a.scatter, n, (long long)gridDim.x * WG, MODE == MODE_SHADOW ? a.count_mul : 1);
         F3 o = {0, 0, 0}, d = {0, 0, 0};
         float ix = 0, iy = 0, iz = 0, px = 0, py = 0, pz = 0;
         float best_t = INFINITY, bu = 0.0f, bv = 0.0f;
@@ -1030,7 +1065,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
             const int nidle = __popcll(idle);
             if (nidle >= thresh && (!f.drained || f.next < f.end)) {
                 int my = fetch_items(f, a, n, waves_total, idle, nidle, __popcll(idle & lanemask_lt()));
-                if (scat && my >= 0) my = scatter_item(my, n);
+                my = scat.map(my);
                 if (item < 0 && my >= 0) {
                     RayItem r;
                     setup_item<MODE, BATCH, WIDTH>(a, my, r);

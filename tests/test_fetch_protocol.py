@@ -6,6 +6,7 @@ and every wave terminates, for item counts from 0 up, fewer waves than ranges,
 every refill threshold and ray batch.  (The GPU tests check every item's
 result; this pins the protocol itself, including schedules the GPU rarely
 produces.)"""
+import math
 import random
 
 import pytest
@@ -98,3 +99,35 @@ def test_launches_with_few_items_use_only_the_waves_they_need(n):
     for per_wave in (4, 16, 64):
         need = max(1, -(-n // per_wave))
         _run(n, min(need, 64), 1, 128, n + per_wave, grid_waves=5120)
+
+
+def _scatter_map(n, b, rounds=1, lanes=1 << 30):
+    """Restatement of the scattered claim order (trace.h Scatter): claim ->
+    item for a launch of n items in blocks of b."""
+    on = rounds > 0 and n > 1 and n <= lanes * rounds
+    b = max(b, 1)
+    m = n // b
+    q = max(int(m * 0.6180339887498949), 1)
+    while m > 1 and math.gcd(q, m) != 1:
+        q += 1
+    out = []
+    for i in range(n):
+        if not on or i >= m * b:
+            out.append(i)
+        else:
+            k, w = divmod(i, b)
+            out.append((k * q) % m * b + w)
+    return out
+
+
+@pytest.mark.parametrize("b", [1, 2, 4, 7, 32])
+def test_scattered_claim_order_is_a_bijection(b):
+    """Every item is claimed exactly once under the scattered order, for item
+    counts that are and are not multiples of the block, including counts whose
+    golden-ratio multiplier needs stepping to become coprime."""
+    for n in list(range(0, 70)) + [97, 128, 1000, 1024, 4096, 65536 + 3, 266688]:
+        got = _scatter_map(n, b)
+        assert sorted(got) == list(range(n)), (n, b)
+        if n >= 64 and b == 1:
+            # neighbouring claims land about n / golden ratio^2 apart or more
+            assert min(abs(got[i + 1] - got[i]) for i in range(min(n - 1, 200))) >= n // 4
