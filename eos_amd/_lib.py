@@ -260,7 +260,7 @@ class Scene:
         check(lib().eos_update_lights(self.h, _ptr(self._lights), len(self._lights)))
 
     DEFAULT_LAUNCH = {"workgroup": 256, "stack_lds": 16, "waves_per_cu": 20, "refill": 16, "batch": 64,
-                      "primary_packets": 0, "coop": 0, "frames_waves_per_cu": 14, "frames_refill": 8,
+                      "primary_packets": 0, "coop": 0, "frames_waves_per_cu": 16, "frames_refill": 16,
                       "frames_batch": 64, "group": 1, "steal": 1, "scatter": 4}
 
     def set_launch_config(self, *args, **kw):

@@ -127,8 +127,8 @@ struct LaunchCfg {
     int steal = 1;         // per-lane walk: idle lanes of a drained wave help its remaining rays
     int scatter = 4;       // launches of at most four items per lane claim them in scattered order
     // frames two in flight share the GPU: fewer waves per launch, larger claims
-    int frames_waves_per_cu = 14;
-    int frames_refill = 8;
+    int frames_waves_per_cu = 16;
+    int frames_refill = 16;
     int frames_batch = 64;
 };
 
