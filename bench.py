@@ -322,6 +322,19 @@ def _primary_roofline(counted, per, width_bytes, n_ids):
             "frac": round(alg / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def _with_traffic(roof: dict, prefix: str) -> dict:
+    """Adds the measured HBM bytes and L2 hit rate of the launch (rocprofv3
+    --pmc, profiles/traffic_config_c.json) to a roofline block."""
+    prof = _profile("config_c")
+    traffic = prof.get(prefix + roof["kernel"])
+    t = roof["launch_ms"]
+    roof.update({"traffic": traffic, "l2_hit_rate": prof.get(prefix + roof["kernel"] + "_l2_hit_rate"),
+                 "traffic_source": prof.get("source"),
+                 "hbm_achieved": round(traffic / (t * 1e-3) / 1e9, 2) if traffic else None,
+                 "hbm_frac": round(traffic / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None})
+    return roof
+
+
 def bench_dense(E, torch, reps: int):
     """The config C mesh scaled by 4, camera close: a 4M-triangle frame of hits."""
     import numpy as np
@@ -354,7 +367,7 @@ def bench_dense(E, torch, reps: int):
                        "trace_closest": round(float(np.mean([p["trace_closest_ms"] for p in per])), 4),
                        "trace_shadow": round(float(np.mean([p["trace_shadow_ms"] for p in per])), 4)},
             "hits_all_levels": hits0, "primary_hit_fraction": round(primary_hit_frac, 4),
-            "roofline": _primary_roofline(counted, per, NODE_BYTES[4], n_ids),
+            "roofline": _with_traffic(_primary_roofline(counted, per, NODE_BYTES[4], n_ids), "dense:"),
             "roofline_in_flight_launch_ms": round(float(np.mean([p["trace_first_ms"] for p in seq])), 4)}
 
 
