@@ -78,10 +78,15 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
     // every wave of the workgroup runs every round (the round's slots are
     // claimed once per workgroup, below), so the trip count is the workgroup's
     __shared__ int s_cnt[3][4], s_base[3][4];
+    // the next round's hit record is loaded while this round is set up (a
+    // level-0 pass is many rounds of mostly misses, bound by that load)
+    const float4 kMiss = make_float4(INFINITY, __int_as_float(-1), 0.f, 0.f);
+    float4 h_next = (int)(blockIdx.x * blockDim.x + threadIdx.x) < n ? a.hit_in[blockIdx.x * blockDim.x + threadIdx.x] : kMiss;
     for (int bbase = blockIdx.x * blockDim.x; bbase < n; bbase += stride) {
         const int i = bbase + (int)threadIdx.x;
         const bool valid = i < n;
-        float4 h = valid ? a.hit_in[i] : make_float4(INFINITY, __int_as_float(-1), 0.f, 0.f);
+        const float4 h = h_next;
+        h_next = (long long)i + stride < n ? a.hit_in[i + stride] : kMiss;
         const int slot = __float_as_int(h.y);
         int pixel = 0, scene = 0;
         F3 o = {0, 0, 0}, d = {0, 0, 0};
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
                     pix_ok = tile_pixel(i - B.id_off, B.tiles_x, B.cam.width, B.cam.height, x, y);
                     pixel = B.pix_off + y * B.cam.width + x;
                     o = f3(B.cam.position[0], B.cam.position[1], B.cam.position[2]);
-                    if (pix_ok) d = camera_dir(B.cam, x, y, 0.0f, 0.0f);
+                    if (pix_ok && slot >= 0) d = camera_dir(B.cam, x, y, 0.0f, 0.0f);   // hits only use d
                     shade = B.sc.shade;
                     geo = B.sc.geo;
                     xf = B.sc.xf;
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
                 pix_ok = valid && tile_pixel(i, a.tiles_x, a.cam.width, a.band_h, x, y);
                 pixel = y * a.cam.width + x;
                 o = f3(a.cam.position[0], a.cam.position[1], a.cam.position[2]);
-                if (pix_ok) d = camera_dir(a.cam, x, y + a.row0, a.aa_ox, a.aa_oy);
+                if (pix_ok && slot >= 0) d = camera_dir(a.cam, x, y + a.row0, a.aa_ox, a.aa_oy);
             }
         } else if (valid) {
             const float4 q0 = a.q_in[2 * (size_t)i], q1 = a.q_in[2 * (size_t)i + 1];
@@ -148,32 +153,37 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
         const bool child = transp && a.level < a.depth;
         const uint64_t lt = lanemask_lt();
         const uint64_t mh = __ballot(hit), mr = __ballot(refl), mc = __ballot(child);
-        int nl = hit ? (BATCH ? a.table[scene].sc.n_light : a.sc.n_light) : 0;
-        for (int off = 32; off > 0; off >>= 1) nl += __shfl_xor(nl, off);
-        const int w = (int)threadIdx.x >> 6;
-        if (__lane_id() == 0) {
-            s_cnt[0][w] = __popcll(mh);
-            s_cnt[1][w] = __popcll(mr) + __popcll(mc);
-            s_cnt[2][w] = nl;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int* const ctrs[3] = {a.hits_ctr, a.q_out_ctr, a.shadow_ctr};
-            for (int c = 0; c < 3; ++c) {
-                int tot = 0;
-                for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-                    s_base[c][k] = tot;
-                    tot += s_cnt[c][k];
-                }
-                const int b0 = tot > 0 ? atomicAdd(ctrs[c], tot) : 0;
-                for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s_base[c][k] += b0;
+        int j = -1, k = -1, kt = -1;
+        // a round in which no ray of the workgroup hit claims nothing (most
+        // rounds of a level-0 pass over a frame of misses)
+        if (__syncthreads_or(hit ? 1 : 0)) {
+            int nl = hit ? (BATCH ? a.table[scene].sc.n_light : a.sc.n_light) : 0;
+            for (int off = 32; off > 0; off >>= 1) nl += __shfl_xor(nl, off);
+            const int w = (int)threadIdx.x >> 6;
+            if (__lane_id() == 0) {
+                s_cnt[0][w] = __popcll(mh);
+                s_cnt[1][w] = __popcll(mr) + __popcll(mc);
+                s_cnt[2][w] = nl;
             }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int* const ctrs[3] = {a.hits_ctr, a.q_out_ctr, a.shadow_ctr};
+                for (int c = 0; c < 3; ++c) {
+                    int tot = 0;
+                    for (int kk = 0; kk < (int)(blockDim.x >> 6); ++kk) {
+                        s_base[c][kk] = tot;
+                        tot += s_cnt[c][kk];
+                    }
+                    const int b0 = tot > 0 ? atomicAdd(ctrs[c], tot) : 0;
+                    for (int kk = 0; kk < (int)(blockDim.x >> 6); ++kk) s_base[c][kk] += b0;
+                }
+            }
+            __syncthreads();
+            j = hit ? s_base[0][w] + __popcll(mh & lt) : -1;
+            const int qb = s_base[1][w];
+            k = refl ? qb + __popcll(mr & lt) : -1;
+            kt = child ? qb + __popcll(mr) + __popcll(mc & lt) : -1;
         }
-        __syncthreads();
-        const int j = hit ? s_base[0][w] + __popcll(mh & lt) : -1;
-        const int qb = s_base[1][w];
-        const int k = refl ? qb + __popcll(mr & lt) : -1;
-        const int kt = child ? qb + __popcll(mr) + __popcll(mc & lt) : -1;
         if (hit) {
             a.surf_out[2 * (size_t)j] = make_float4(P.x, P.y, P.z, __int_as_float(pixel));
             a.surf_out[2 * (size_t)j + 1] = make_float4(N.x, N.y, N.z, __int_as_float(pack_mat(mat, scene)));
