@@ -260,36 +260,45 @@ struct PhongArgs {
     DevCamera cam;
     const BatchScene* table;
     int level, npix, lstride;
-    const int* count_ptr;
-    const float4* surf;
-    const uint8_t* vis;
+    int nlev;                 // consecutive levels level .. level + nlev - 1 (chains; ray trees: 1)
+    const int* count_ptr;     // [nlev] hits per level
+    const float4* surf;       // first level's surfaces; the next levels' at surf_stride float4s apart
+    const uint8_t* vis;       // likewise, vis_stride bytes apart
+    long long surf_stride, vis_stride;
     float4* levcol;
     float4* hitcol;           // ray trees: colour per hit of this level instead of per pixel
 };
 
+// Phong of the hits of one or more levels (one launch, the levels one after
+// another in the same threads: independent records, so no ordering needed).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_phong(PhongArgs a) {
-    const int n = *a.count_ptr;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const float4 s0 = a.surf[2 * (size_t)j], s1 = a.surf[2 * (size_t)j + 1];
-        const F3 P = f3(s0.x, s0.y, s0.z), N = f3(s1.x, s1.y, s1.z);
-        const int pixel = __float_as_int(s0.w), packed = __float_as_int(s1.w);
-        const SceneView& sc = BATCH ? a.table[unpack_scene(packed)].sc : a.sc;
-        const DevCamera& cm = BATCH ? a.table[unpack_scene(packed)].cam : a.cam;
-        const F3 cam = f3(cm.position[0], cm.position[1], cm.position[2]);
-        const DevMaterial& m = sc.mats[unpack_mat(packed)];
-        const int L = sc.n_light;
-        float acc[3] = {0.0f, 0.0f, 0.0f};
-        for (int i = 0; i < L; ++i) {
-            if (!a.vis[(size_t)j * a.lstride + i]) continue;
-            float c[3];
-            phong(m, sc.lights[i], cam, P, N, c);
+    for (int li = 0; li < a.nlev; ++li) {
+        const int n = a.count_ptr[li];
+        const int level = a.level + li;
+        const float4* surf = a.surf + li * a.surf_stride;
+        const uint8_t* vis = a.vis + li * a.vis_stride;
+        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+            const float4 s0 = surf[2 * (size_t)j], s1 = surf[2 * (size_t)j + 1];
+            const F3 P = f3(s0.x, s0.y, s0.z), N = f3(s1.x, s1.y, s1.z);
+            const int pixel = __float_as_int(s0.w), packed = __float_as_int(s1.w);
+            const SceneView& sc = BATCH ? a.table[unpack_scene(packed)].sc : a.sc;
+            const DevCamera& cm = BATCH ? a.table[unpack_scene(packed)].cam : a.cam;
+            const F3 cam = f3(cm.position[0], cm.position[1], cm.position[2]);
+            const DevMaterial& m = sc.mats[unpack_mat(packed)];
+            const int L = sc.n_light;
+            float acc[3] = {0.0f, 0.0f, 0.0f};
+            for (int i = 0; i < L; ++i) {
+                if (!vis[(size_t)j * a.lstride + i]) continue;
+                float c[3];
+                phong(m, sc.lights[i], cam, P, N, c);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) acc[k] = acc[k] + c[k];
+                for (int k = 0; k < 3; ++k) acc[k] = acc[k] + c[k];
+            }
+            const float w = m.reflective ? m.reflectivity : 0.0f;
+            if (a.hitcol) a.hitcol[j] = make_float4(acc[0], acc[1], acc[2], w);
+            else a.levcol[(size_t)level * a.npix + pixel] = make_float4(acc[0], acc[1], acc[2], w);
         }
-        const float w = m.reflective ? m.reflectivity : 0.0f;
-        if (a.hitcol) a.hitcol[j] = make_float4(acc[0], acc[1], acc[2], w);
-        else a.levcol[(size_t)a.level * a.npix + pixel] = make_float4(acc[0], acc[1], acc[2], w);
     }
 }
 
@@ -638,24 +647,27 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, st, sh));
         }
         F.s[2 * slot + 1] = rec(st);
-        for (int l = l0; l <= l1; ++l) {
-            PhongArgs pa{};
-            pa.sc = f.sc;
-            pa.cam = f.cam;
-            pa.table = f.table;
-            pa.level = l;
-            pa.npix = (int)f.npix;
-            pa.lstride = f.lstride;
-            pa.count_ptr = ctr + CTR_HITS + l;
-            pa.surf = fb.surf_at(f.tree ? 0 : l);
-            pa.vis = fb.vis_at(f.tree ? 0 : l);
-            pa.levcol = fb.levcol.as<float4>();
-            pa.hitcol = f.tree ? fb.hitcol.as<float4>() + fb.toff[l] : nullptr;
-            F.h[2 * l] = rec(st);
-            if (f.batch) hipLaunchKernelGGL(k_phong<true>, dim3(shade_grid), dim3(256), 0, st, pa);
-            else hipLaunchKernelGGL(k_phong<false>, dim3(shade_grid), dim3(256), 0, st, pa);
-            F.h[2 * l + 1] = rec(st);
-        }
+        // Phong of levels [l0, l1] in one launch (ray trees shade one level at a time)
+        PhongArgs pa{};
+        pa.sc = f.sc;
+        pa.cam = f.cam;
+        pa.table = f.table;
+        pa.level = l0;
+        pa.nlev = l1 - l0 + 1;
+        pa.npix = (int)f.npix;
+        pa.lstride = f.lstride;
+        pa.count_ptr = ctr + CTR_HITS + l0;
+        pa.surf = fb.surf_at(f.tree ? 0 : l0);
+        pa.vis = fb.vis_at(f.tree ? 0 : l0);
+        pa.surf_stride = (long long)fb.surf_stride();
+        pa.vis_stride = (long long)fb.vis_stride();
+        pa.levcol = fb.levcol.as<float4>();
+        pa.hitcol = f.tree ? fb.hitcol.as<float4>() + fb.toff[l0] : nullptr;
+        F.h[2 * l0] = rec(st);
+        if (f.batch) hipLaunchKernelGGL(k_phong<true>, dim3(shade_grid), dim3(256), 0, st, pa);
+        else hipLaunchKernelGGL(k_phong<false>, dim3(shade_grid), dim3(256), 0, st, pa);
+        F.h[2 * l0 + 1] = rec(st);
+        for (int l = l0 + 1; l <= l1; ++l) F.h[2 * l] = F.h[2 * l + 1] = F.h[2 * l0 + 1];   // timed with level l0
         return EOS_OK;
     };
     for (int l = 0; l <= f.depth; ++l) {
