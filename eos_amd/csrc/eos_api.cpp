@@ -499,6 +499,7 @@ eos_status eos_set_launch_config(eos_scene* s, const eos_launch_config* c) {
     if (c->group < 0 || c->group > 64) return fail(EOS_ERR_INVALID, "group must be in [0, 64]");
     if (c->steal != 0 && c->steal != 1) return fail(EOS_ERR_INVALID, "steal must be 0 or 1");
     if (c->scatter < 0 || c->scatter > 64) return fail(EOS_ERR_INVALID, "scatter must be in [0, 64]");
+    if (c->group_lanes != 16 && c->group_lanes != 64) return fail(EOS_ERR_INVALID, "group_lanes must be 16 or 64");
     s->cfg.workgroup = c->workgroup;
     s->cfg.stack_lds = c->stack_lds;
     s->cfg.waves_per_cu = c->waves_per_cu;
@@ -509,6 +510,7 @@ eos_status eos_set_launch_config(eos_scene* s, const eos_launch_config* c) {
     s->cfg.group = c->group;
     s->cfg.steal = c->steal;
     s->cfg.scatter = c->scatter;
+    s->cfg.group_lanes = c->group_lanes;
     s->cfg.frames_waves_per_cu = c->frames_waves_per_cu;
     s->cfg.frames_refill = c->frames_refill;
     s->cfg.frames_batch = c->frames_batch;
@@ -527,6 +529,7 @@ eos_status eos_get_launch_config(const eos_scene* s, eos_launch_config* c) {
     c->group = s->cfg.group;
     c->steal = s->cfg.steal;
     c->scatter = s->cfg.scatter;
+    c->group_lanes = s->cfg.group_lanes;
     c->frames_waves_per_cu = s->cfg.frames_waves_per_cu;
     c->frames_refill = s->cfg.frames_refill;
     c->frames_batch = s->cfg.frames_batch;

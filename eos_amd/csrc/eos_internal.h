@@ -124,6 +124,7 @@ struct LaunchCfg {
     int packets = 0;       // primary rays as one 8x8 packet per wave (4-wide tree)
     int coop = 0;          // quad-cooperative walk (0: never; work stealing serves the short launches)
     int group = 1;         // group walk (16 lanes per ray) for launches with at most one item per 16 lanes
+    int group_lanes = 64;  // widest group: a whole wave per ray for launches of at most one item per wave
     int steal = 1;         // per-lane walk: idle lanes of a drained wave help its remaining rays
     int scatter = 4;       // launches of at most four items per lane claim them in scattered order
     // frames two in flight share the GPU: fewer waves per launch, larger claims

@@ -314,6 +314,7 @@ eos_status trace_rays(eos_scene* s, int n, const float* o, const float* d, const
         a.batch = cfg.batch;
         a.coop = cfg.coop;
         a.group = cfg.group;
+        a.group_lanes = cfg.group_lanes;
         a.steal = cfg.steal;
         a.scatter = cfg.scatter;
         a.ray_o = ro;

@@ -633,6 +633,7 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             sh.batch = f.cfg.batch;
             sh.coop = f.cfg.coop;
             sh.group = f.cfg.group;
+            sh.group_lanes = f.cfg.group_lanes;
             sh.steal = f.count ? 0 : f.cfg.steal;
             sh.scatter = f.cfg.scatter;
             sh.cursor = cursors(16 + l0);
@@ -689,6 +690,7 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         ta.batch = f.cfg.batch;
         ta.coop = f.cfg.coop;
         ta.group = f.cfg.group;
+        ta.group_lanes = f.cfg.group_lanes;
         ta.steal = f.count ? 0 : f.cfg.steal;
         ta.scatter = f.cfg.scatter;
         ta.cursor = cursors(l);
@@ -1143,7 +1145,7 @@ bool same_batch(eos_scene* const* scenes, const eos_camera* cams, int n, int dep
     if ((int)b.scenes.size() != n || b.depth != depth) return false;
     if (b.cfg.workgroup != cfg.workgroup || b.cfg.stack_lds != cfg.stack_lds || b.cfg.waves_per_cu != cfg.waves_per_cu ||
         b.cfg.refill != cfg.refill || b.cfg.batch != cfg.batch || b.cfg.packets != cfg.packets ||
-        b.cfg.coop != cfg.coop || b.cfg.group != cfg.group || b.cfg.steal != cfg.steal ||
+        b.cfg.coop != cfg.coop || b.cfg.group != cfg.group || b.cfg.steal != cfg.steal || b.cfg.group_lanes != cfg.group_lanes ||
         b.cfg.scatter != cfg.scatter)
         return false;
     for (int i = 0; i < n; ++i) {

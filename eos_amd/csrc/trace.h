@@ -279,6 +279,7 @@ struct TraceArgs {
     int batch;               // most items a wave claims per fetch
     int coop;                // quad-cooperative walk: 0 never, 1..3 at most lanes/4, /2, /1 items, 4 always
     int group;               // group walk (16 lanes per ray) for launches of at most group x lanes/16 items (0: never)
+    int group_lanes;         // 64: launches of at most group x lanes/64 items walk each ray with a whole wave
     int steal;               // per-lane walk: idle lanes of a drained wave take pending subtrees of its rays
     int scatter;             // launches of at most scatter x lanes items claim them in scattered order (0: never)
 };
@@ -710,26 +711,30 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
 // flag, bit for bit.  For launches with few rays, whose time is the longest
 // ray's chain of dependent steps.
 constexpr int kGroupLanes = 16;
-constexpr int kGroupQuads = kGroupLanes / 4;
-constexpr int kGroupStack = 512;   // entries per group: at most 16 pushes per step; overflow is an error
+constexpr int kWaveGroupStack = 2048;   // stack entries per wave, shared by its groups: 512 per group of
+                                        // 16 lanes (at most 16 pushes per step), 2048 for a wave-wide
+                                        // group (at most 64); overflow is an error
 
 // Items per launch at or below which the group walk is used, as a multiple of
-// the launch's lanes / 16 (1: one group per ray in one round; 4: four rounds...).
-__device__ __forceinline__ bool use_group(int rounds, long long n, long long lanes) {
-    return rounds > 0 && n * kGroupLanes <= lanes * (long long)rounds;
+// the launch's lanes / g (1: one group per ray in one round; 4: four rounds...).
+__device__ __forceinline__ bool use_group(int rounds, long long n, long long lanes, int g) {
+    return rounds > 0 && n * g <= lanes * (long long)rounds;
 }
 
-template <int MODE, int WG, bool BATCH, bool XF>
+template <int MODE, int WG, bool BATCH, bool XF, int G>
 __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* wave_stack) {
     constexpr bool ANY = any_hit_mode(MODE);
-    constexpr int kGroups = 64 / kGroupLanes;
+    static_assert(G == 16 || G == 64, "groups of 16 lanes or whole waves");
+    constexpr int kGroups = 64 / G;
+    constexpr int kQuads = G / 4;
+    constexpr int kStack = kWaveGroupStack / kGroups;   // entries per group
     const int lane = __lane_id();
-    const int gsub = lane & (kGroupLanes - 1), q = gsub >> 2, k = lane & 3;
-    const int grp = lane / kGroupLanes;
-    const int gbit = kGroupLanes * grp;
-    const uint64_t gmask = 0xFFFFull << gbit;
+    const int gsub = lane & (G - 1), q = gsub >> 2, k = lane & 3;
+    const int grp = lane / G;
+    const int gbit = G * grp;
+    const uint64_t gmask = G == 64 ? ~0ull : (0xFFFFull << gbit);
     const int qbit = lane & ~3;                     // first lane of this lane's quad
-    int32_t* stk = wave_stack + grp * kGroupStack;
+    int32_t* stk = wave_stack + grp * kStack;
     const int waves_total = (int)gridDim.x * (WG / 64);
     Scatter scat;
     scat.init(a.scatter, n, (long long)gridDim.x * WG, MODE == MODE_SHADOW ? a.count_mul : 1);
@@ -787,7 +792,7 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
         const int need = (f.drained && f.next >= f.end) ? kGroups : 1;
         for (;;) {
             if (item >= 0) {  // uniform within a group from here on
-                const int npop = sp < kGroupQuads ? sp : kGroupQuads;
+                const int npop = sp < kQuads ? sp : kQuads;
                 const int ref = q < npop ? stk[sp - 1 - q] : kDone;
                 sp -= npop;
                 const bool inner = ref >= 0 && ref != kDone;
@@ -846,7 +851,7 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
                 if (budget < 0) {
                     if (gsub == 0) atomicOr(a.err, 2);
                     done = true;
-                } else if (sp + total > kGroupStack) {
+                } else if (sp + total > kStack) {
                     if (gsub == 0) atomicOr(a.err, 1);
                     done = true;
                 } else {
@@ -860,7 +865,7 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
                     }
                 } else {
                     // least (t, original index) over the group, then against the ray's best
-                    for (int off = 1; off < kGroupLanes; off <<= 1) {
+                    for (int off = 1; off < G; off <<= 1) {
                         const float ot = __shfl_xor(lt, off), ou = __shfl_xor(lu, off), ov = __shfl_xor(lv, off);
                         const int oo = __shfl_xor(lo_, off), os = __shfl_xor(ls, off);
                         if (ot < lt || (ot == lt && oo < lo_)) {
@@ -1007,7 +1012,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     constexpr bool GROUP = WIDTH == 4 && !COUNT && MODE != MODE_PRIMARY;
     // work stealing visits nodes in another order (never in launches that count visits)
     constexpr bool STEAL = !COUNT && kRegStack == 0;
-    constexpr int LDS_GROUP = GROUP ? (WG / kGroupLanes) * kGroupStack : 0;
+    constexpr int LDS_GROUP = GROUP ? (WG / 64) * kWaveGroupStack : 0;
     constexpr int LDS_A = LDS_LANE > LDS_COOP ? LDS_LANE : LDS_COOP;
     __shared__ int32_t lds[LDS_A > LDS_GROUP ? LDS_A : LDS_GROUP];
     const int tid = threadIdx.x;
@@ -1020,19 +1025,23 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     }
     unsigned long long visits = 0, tests = 0;
     const long long lanes = (long long)gridDim.x * WG;
-    const bool grouped = GROUP && use_group(a.group, n, lanes);
+    // a whole wave per ray for launches of a few thousand rays, else 16 lanes
+    const bool wide = GROUP && a.group_lanes == 64 && use_group(a.group, n, lanes, 64);
+    const bool grouped = GROUP && (wide || use_group(a.group, n, lanes, kGroupLanes));
     const bool coop = !grouped && WIDTH == 4 && use_coop(a.coop, n, lanes);
     // Waves beyond what the launch's items can occupy at once leave now: a
     // launch with a few items on the full persistent grid would otherwise have
     // every wave claim from every range cursor before it finds nothing left.
     {
-        const long long per_wave = grouped ? 64 / kGroupLanes : (coop ? 16 : 64);
+        const long long per_wave = wide ? 1 : (grouped ? 64 / kGroupLanes : (coop ? 16 : 64));
         const long long need = (n + per_wave - 1) / per_wave;
         const int wave_id = (int)blockIdx.x * (WG / 64) + (tid >> 6);
         if (wave_id >= (need > 0 ? need : 1)) return;
     }
-    if (grouped) {
-        trace_group<MODE, WG, BATCH, XF>(a, n, lds + (tid >> 6) * (64 / kGroupLanes) * kGroupStack);
+    if (wide) {
+        trace_group<MODE, WG, BATCH, XF, 64>(a, n, lds + (tid >> 6) * kWaveGroupStack);
+    } else if (grouped) {
+        trace_group<MODE, WG, BATCH, XF, kGroupLanes>(a, n, lds + (tid >> 6) * kWaveGroupStack);
     } else if (coop) {
         trace_coop<MODE, WG, COUNT, BATCH, XF>(a, n, lds + (tid >> 6) * 16 * kMaxStack4, visits, tests);
     } else {

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EOS_ABI_VERSION 4
+#define EOS_ABI_VERSION 5
 
 typedef enum eos_status {
     EOS_OK = 0,
@@ -368,6 +368,8 @@ typedef struct eos_launch_config {
     int32_t scatter;          /* launches with at most scatter x lanes items claim them in a scattered
                                  order (item i * p mod n, p prime), so neighbouring rays -- often long
                                  together -- go to different waves; 0: never.  Results do not change */
+    int32_t group_lanes;      /* widest group walk: 64 (launches with at most group x lanes/64 items walk
+                                 each ray with a whole wave, 16 nodes per step) or 16 (never wider) */
 } eos_launch_config;
 
 eos_status eos_set_launch_config(eos_scene* scene, const eos_launch_config* cfg);

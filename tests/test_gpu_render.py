@@ -214,18 +214,19 @@ GROUP_CASES = {
 
 @pytest.mark.parametrize("name", sorted(GROUP_CASES))
 def test_group_walk_equals_per_lane_walk(name):
-    """The group walk (16 lanes per ray, four nodes per step) and the per-lane
-    and quad walks give bit-identical frames: closest-hit and any-hit launches,
-    chains and ray trees, transformed triangles, queries."""
+    """The group walks (16 lanes per ray and four nodes per step, or a whole
+    wave per ray and sixteen nodes per step) and the per-lane and quad walks
+    give bit-identical frames: closest-hit and any-hit launches, chains and ray
+    trees, transformed triangles, queries."""
     from eos_amd import _lib as E
     s = GROUP_CASES[name]()
     print(f"group walk vs per-lane case={name}", flush=True)
     g = E.Scene(s, leaf_max=4)
     out = {}
-    for grp, coop in ((0, 0), (0, 4), (64, 0), (1, 3)):
-        g.set_launch_config(group=grp, coop=coop)
-        out[(grp, coop)] = g.render(want_rgb=True, want_hits=True)
-    base = out[(0, 0)]
+    for grp, coop, gl in ((0, 0, 64), (0, 4, 64), (64, 0, 64), (64, 0, 16), (1, 3, 64), (1, 0, 16)):
+        g.set_launch_config(group=grp, coop=coop, group_lanes=gl)
+        out[(grp, coop, gl)] = g.render(want_rgb=True, want_hits=True)
+    base = out[(0, 0, 64)]
     for key, r in out.items():
         assert np.array_equal(r["hit"], base["hit"]), key
         assert np.array_equal(r["rgba"], base["rgba"]), key
@@ -239,10 +240,11 @@ def test_group_walk_equals_per_lane_walk(name):
     g.set_launch_config(group=0, coop=0)
     a = g.trace_rays(o, d, "nearest")
     ao = g.trace_rays(o, d, "occluded")
-    g.set_launch_config(group=64, coop=0)
-    b = g.trace_rays(o, d, "nearest")
-    assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
-    assert np.array_equal(ao, g.trace_rays(o, d, "occluded"))
+    for gl in (64, 16):
+        g.set_launch_config(group=64, coop=0, group_lanes=gl)
+        b = g.trace_rays(o, d, "nearest")
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), gl
+        assert np.array_equal(ao, g.trace_rays(o, d, "occluded")), gl
 
 
 def test_group_walk_in_batches():
@@ -252,10 +254,11 @@ def test_group_walk_in_batches():
     sc = [E.Scene(d, leaf_max=4) for d in data]
     sc[0].set_launch_config(group=0, coop=0)
     ref, _ = E.render_batch(sc, depth=2)
-    sc[0].set_launch_config(group=64, coop=0)
-    outs, _ = E.render_batch(sc, depth=2, use_graph=True)
-    for a, b in zip(outs, ref):
-        assert np.array_equal(a, b)
+    for gl in (64, 16):
+        sc[0].set_launch_config(group=64, coop=0, group_lanes=gl)
+        outs, _ = E.render_batch(sc, depth=2, use_graph=True)
+        for a, b in zip(outs, ref):
+            assert np.array_equal(a, b), gl
 
 
 STEAL_CFGS = (
