@@ -137,6 +137,27 @@ def cpu_same_algorithm(scene, stride: int, fast: bool = False):
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "sample": f"every {stride}th row", "seconds": dt}
 
 
+def hbm_copy_gbs(torch, mib: int = 2048, reps: int = 5) -> float:
+    """Device-to-device copy bandwidth (bytes read + written per second, median
+    of reps): the box's attainable HBM rate beside the published 8 TB/s."""
+    n = mib << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    rates = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        torch.cuda.synchronize()
+        rates.append(2 * n / (a.elapsed_time(b) * 1e-3) / 1e9)
+    del src, dst
+    torch.cuda.empty_cache()
+    return float(sorted(rates)[len(rates) // 2])
+
+
 def _progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -663,6 +684,11 @@ def main() -> int:
         result["device"] = E.device_name()
     except E.EosError:
         pass
+    copy = hbm_copy_gbs(torch)
+    result["roofline"]["copy_gbs"] = round(copy, 1)
+    result["roofline"]["copy_note"] = ("measured device-to-device copy of 2 GiB (read + write bytes per second, "
+                                       "torch's copy kernel): the attainable HBM rate beside the published peak")
+    _progress(f"HBM copy: {copy:.0f} GB/s")
     if not args.skip_batch:
         result["batched"] = bench_batched(E, torch, reps=3)
         _progress(f"batched: {result['batched']['speedup']}x over sequential")

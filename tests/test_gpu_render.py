@@ -271,16 +271,17 @@ STEAL_CFGS = (
 )
 
 
+@pytest.mark.parametrize("width", [4, 2])
 @pytest.mark.parametrize("name", sorted(GROUP_CASES))
-def test_work_stealing_equals_plain_walk(name):
+def test_work_stealing_equals_plain_walk(name, width):
     """Work stealing (idle lanes of a drained wave walk pending subtrees of its
     remaining rays, results merged into each ray's home lane) and scattered
     claim order give bit-identical frames, ray counts and query answers, with
     every launch shape from one ray per wave to many rounds of rays per lane."""
     from eos_amd import _lib as E
     s = GROUP_CASES[name]()
-    print(f"work stealing vs plain walk case={name}", flush=True)
-    g = E.Scene(s, leaf_max=4)
+    print(f"work stealing vs plain walk case={name} width={width}", flush=True)
+    g = E.Scene(s, leaf_max=4, width=width)
     plain = {"steal": 0, "scatter": 0, "group": 0, "coop": 0}
     g.set_launch_config(**plain)
     base = g.render(want_rgb=True, want_hits=True)
