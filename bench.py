@@ -368,7 +368,7 @@ def bench_dense(E, torch, reps: int):
     counted = g.render_device(s.camera, s.depth, out.data_ptr(), hit_dev_ptr=hit.data_ptr(), count_visits=True)
     primary_hit_frac = float((hit >= 0).float().mean().item())
     wall, st = _frames(g, s, out, reps)
-    per = [g.render_device(s.camera, s.depth, out.data_ptr()) for _ in range(reps)]
+    per = [g.render_device(s.camera, s.depth, out.data_ptr(), launch_times=True) for _ in range(reps)]
     cams = np.stack([np.array(s.camera, copy=True)] * reps)
     g.render_frames(cams[:1], [out.data_ptr()], depth=s.depth)
     torch.cuda.synchronize()
@@ -598,6 +598,11 @@ def main() -> int:
     torch.cuda.synchronize()
     wall_single = time.perf_counter() - t_start
     _progress(f"{args.steps} single frames: {wall_single * 1e3 / args.steps:.2f} ms/frame")
+    # the same frames with every launch timed (GPU time per launch kind; the
+    # timing events between dependent launches add a few microseconds each, so
+    # these frames are not the latency figure)
+    per_t = [g.render_device(cam, depth, out.data_ptr(), launch_times=True) for _ in range(max(3, args.steps // 2))]
+    torch.cuda.synchronize()
     # the timed steps: the same frames with two in flight (eos_render_frames)
     cams = np.stack([np.array(cam, copy=True)] * args.steps)
     if args.warmup:
@@ -665,8 +670,12 @@ def main() -> int:
                      "hbm_frac": round(traffic / (t_first * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None},
         "rays_per_frame": {"primary": st["primary_rays"], "shadow": st["shadow_rays"],
                            "secondary": st["secondary_rays"], "hits": st["hits"]},
-        "gpu_ms": {"render": round(mean("render_ms"), 4), "trace_closest": round(mean("trace_closest_ms"), 4),
-                   "trace_shadow": round(mean("trace_shadow_ms"), 4), "shade": round(mean("shade_ms"), 4)},
+        "gpu_ms": {"render": round(mean("render_ms"), 4),
+                   "trace_closest": round(float(np.mean([p["trace_closest_ms"] for p in per_t])), 4),
+                   "trace_shadow": round(float(np.mean([p["trace_shadow_ms"] for p in per_t])), 4),
+                   "shade": round(float(np.mean([p["shade_ms"] for p in per_t])), 4),
+                   "note": "render: frames timed only at their ends (the latency frames); the per-launch "
+                           "sums come from frames with every launch timed"},
         "single_frame": {"ms_per_frame": round(wall_single / args.steps * 1e3, 4),
                          "mrays_per_s": round(rays * args.steps / wall_single / 1e6, 3),
                          "note": "one eos_render per frame (frame latency); value keeps two frames in flight"},
@@ -708,7 +717,7 @@ def main() -> int:
     host = host_cpu()
     threads = host["omp_threads_used"]
     if not args.skip_driver:
-        result["driver_scene"] = bench_driver(E, torch, reps=5, cpu_threads=0 if args.no_cpu else threads,
+        result["driver_scene"] = bench_driver(E, torch, reps=40, cpu_threads=0 if args.no_cpu else threads,
                                               cpu_stride=4)
         _progress(f"driver scene: {result['driver_scene']['mrays_per_s']} Mrays/s, GPU speedup "
                   f"{result['driver_scene'].get('gpu_speedup_single_frame')}")

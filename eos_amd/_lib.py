@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libeos_amd.so")
 
 EOS_OK, EOS_ERR_INVALID, EOS_ERR_HIP, EOS_ERR_NOMEM, EOS_ERR_STATE, EOS_ERR_LIMIT, EOS_ERR_NODEVICE = range(7)
-ABI_VERSION = 5   # EOS_ABI_VERSION of include/eos_c.h these structures follow
+ABI_VERSION = 6   # EOS_ABI_VERSION of include/eos_c.h these structures follow
 EOS_MEM_HOST, EOS_MEM_DEVICE = 0, 1
 DBG = {"morton_sorted": 1, "perm": 2, "kchild": 3, "kparent": 4, "ibox": 5, "nodes": 6, "tris": 7,
        "morton": 8, "nodes4": 9, "xforms": 10}
@@ -59,7 +59,7 @@ class BuildOpts(C.Structure):
 class RenderOpts(C.Structure):
     _fields_ = [("depth", C.c_int32), ("output_memory", C.c_int32), ("rgb_out", C.c_void_p),
                 ("hit_out", C.c_void_p), ("keep_records", C.c_int32), ("count_visits", C.c_int32),
-                ("aa_samples", C.c_int32)]
+                ("aa_samples", C.c_int32), ("launch_times", C.c_int32)]
 
 
 class RenderStats(C.Structure):
@@ -279,7 +279,7 @@ class Scene:
         return {k: getattr(c, k) for k, _ in c._fields_}
 
     def render(self, camera=None, depth=None, want_rgb=False, want_hits=False, keep_records=False,
-               count_visits=False, want_rgba=True, aa_samples=0):
+               count_visits=False, want_rgba=True, aa_samples=0, launch_times=False):
         self.generation += 1
         cam = np.array(self.data.camera if camera is None else camera, copy=True)
         depth = self.data.depth if depth is None else depth
@@ -288,18 +288,19 @@ class Scene:
         rgb = np.zeros((H, W, 3), np.float32) if want_rgb else None
         hits = np.zeros((H, W), np.int32) if want_hits else None
         o = RenderOpts(depth, EOS_MEM_HOST, _ptr(rgb), _ptr(hits), int(keep_records), int(count_visits),
-                       int(aa_samples))
+                       int(aa_samples), int(launch_times))
         st = RenderStats()
         check(lib().eos_render(self.h, cam.ctypes.data, C.byref(o), _ptr(rgba), C.byref(st)))
         return {"rgba": rgba, "rgb": rgb, "hit": hits, "stats": st.as_dict()}
 
     def render_device(self, camera, depth, rgba_dev_ptr, rgb_dev_ptr=None, hit_dev_ptr=None,
-                      keep_records=False, count_visits=False, aa_samples=0) -> dict:
-        """Render into device buffers (raw pointers, e.g. torch tensors' data_ptr())."""
+                      keep_records=False, count_visits=False, aa_samples=0, launch_times=False) -> dict:
+        """Render into device buffers (raw pointers, e.g. torch tensors' data_ptr()).
+        launch_times: also time every launch (trace_closest_ms, trace_shadow_ms, shade_ms)."""
         self.generation += 1
         cam = np.array(camera, copy=True)
         o = RenderOpts(depth, EOS_MEM_DEVICE, rgb_dev_ptr, hit_dev_ptr, int(keep_records), int(count_visits),
-                       int(aa_samples))
+                       int(aa_samples), int(launch_times))
         st = RenderStats()
         check(lib().eos_render(self.h, cam.ctypes.data, C.byref(o), rgba_dev_ptr, C.byref(st)))
         return st.as_dict()

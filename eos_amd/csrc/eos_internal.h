@@ -250,7 +250,9 @@ struct AaSample {
     float4* acc = nullptr;        // [npix] running sums
 };
 // One frame of the wavefront pipeline (render.hip); outputs may be null.
+// launch_times: timing events around every launch (else only around the frame
+// and its primary-ray launch).
 eos_status run_frame(eos_scene* s, const eos_camera* cam, int depth, bool count, uchar4* rgba_dev, float* rgb_dev,
                      int32_t* hits_dev, eos_render_stats* st, const AaSample& aa = AaSample(), int row0 = 0,
-                     int row1 = -1);
+                     int row1 = -1, bool launch_times = true);
 }  // namespace eos

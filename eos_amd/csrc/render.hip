@@ -911,13 +911,35 @@ int trace_grid_of(const Frame& f) { return std::max(1, f.num_cu * f.cfg.waves_pe
 }  // namespace
 
 eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count, uchar4* rgba_dev,
-                     float* rgb_dev, int32_t* hits_dev, eos_render_stats* st, const AaSample& aa, int row0, int row1) {
-    const Frame f = make_frame(s, camp, depth, count, rgba_dev, rgb_dev, hits_dev, aa, row0, row1);
+                     float* rgb_dev, int32_t* hits_dev, eos_render_stats* st, const AaSample& aa, int row0, int row1,
+                     bool launch_times) {
+    Frame f = make_frame(s, camp, depth, count, rgba_dev, rgb_dev, hits_dev, aa, row0, row1);
     EOS_TRY(ensure_buffers(s->fb, f, trace_grid_of(f)));
     FrameEvents fe;
     FrameStreams fs{s->stream, s->stream2, s->sync};
-    EOS_TRY(enqueue_frame(s->fb, f, fs, s->ev, &fe));
-    return finish_frame(s->fb, f, s->stream, s->ev, &fe, st);
+    if (launch_times) {
+        EOS_TRY(enqueue_frame(s->fb, f, fs, s->ev, &fe));
+        return finish_frame(s->fb, f, s->stream, s->ev, &fe, st);
+    }
+    // four events: around the frame and around its primary-ray launch (an event
+    // between two dependent launches delays the second by a few microseconds)
+    while (s->first_ev.size() < 2) {
+        hipEvent_t e = nullptr;
+        EOS_HIP(hipEventCreate(&e));
+        s->first_ev.push_back(e);
+    }
+    f.first_ev[0] = s->first_ev[0];
+    f.first_ev[1] = s->first_ev[1];
+    EOS_HIP(hipEventRecord(s->ev[0], s->stream));
+    EOS_TRY(enqueue_frame(s->fb, f, fs, nullptr, nullptr));
+    EOS_HIP(hipEventRecord(s->ev[1], s->stream));
+    EOS_TRY(finish_frame(s->fb, f, s->stream, nullptr, nullptr, st));
+    if (st) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, s->ev[0], s->ev[1]) == hipSuccess) st->render_ms = ms;
+        if (hipEventElapsedTime(&ms, s->first_ev[0], s->first_ev[1]) == hipSuccess) st->trace_first_ms = ms;
+    }
+    return EOS_OK;
 }
 
 // Frames in flight: frame i runs on slot i % kFrameSlots (its own frame
@@ -1067,7 +1089,7 @@ eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o,
     const int n = o->aa_samples > 1 ? o->aa_samples : 1;
     if (n == 1) {
         EOS_TRY(run_frame(s, cam, o->depth, o->count_visits != 0, rgba_dev, rgb_dev, hit_dev, st, AaSample(), row0,
-                          row1));
+                          row1, o->launch_times != 0));
     } else {
         // one frame per sample, in the reference's (q, p) order, summed by the
         // resolves (ordered across the two slots), two sample frames in flight

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EOS_ABI_VERSION 5
+#define EOS_ABI_VERSION 6
 
 typedef enum eos_status {
     EOS_OK = 0,
@@ -119,6 +119,9 @@ typedef struct eos_render_opts {
     int32_t aa_samples;       /* RayTracer::setAASamples (src/RayTracer.cpp:176-266): n x n samples per
                                  pixel on a regular grid, averaged; 0 or 1: one ray through the pixel
                                  corner.  hit_out then holds sample (0, 0); keep_records needs 0 or 1 */
+    int32_t launch_times;     /* 1: time every launch of the frame (trace_closest_ms, trace_shadow_ms,
+                                 shade_ms); each timing event between dependent launches costs a few
+                                 microseconds of GPU time.  0: only render_ms and trace_first_ms */
 } eos_render_opts;
 
 typedef struct eos_render_stats {
@@ -129,9 +132,9 @@ typedef struct eos_render_stats {
     int64_t node_visits;      /* count_visits only */
     int64_t tri_tests;        /* count_visits only */
     double render_ms;         /* GPU time, first to last kernel of the frame */
-    double trace_closest_ms;  /* summed over the frame's closest-hit launches */
-    double trace_shadow_ms;   /* summed over the frame's any-hit launches */
-    double shade_ms;          /* hit setup + Phong + resolve */
+    double trace_closest_ms;  /* launch_times: summed over the frame's closest-hit launches */
+    double trace_shadow_ms;   /* launch_times: summed over the frame's any-hit launches */
+    double shade_ms;          /* launch_times: hit setup + Phong + resolve */
     int32_t trace_closest_launches;
     int32_t trace_shadow_launches;
     int64_t closest_visits_first;  /* count_visits: node visits of the primary-ray launch */
