@@ -320,14 +320,14 @@ class Scene:
         check(lib().eos_render_rows(self.h, cam.ctypes.data, row_begin, row_end, C.byref(o), _ptr(rgba), C.byref(st)))
         return {"rgba": rgba, "rgb": rgb, "hit": hits, "stats": st.as_dict()}
 
-    def render_frames(self, cameras, rgba_dev_ptrs, depth=None, count_visits=False) -> list:
+    def render_frames(self, cameras, rgba_dev_ptrs, depth=None, count_visits=False, launch_times=False) -> list:
         """eos_render_frames: frame i from cameras[i] into the device buffer rgba_dev_ptrs[i]
-        (two frames in flight); per-frame stats."""
+        (two frames in flight); per-frame stats (launch_times: with each primary-ray launch timed)."""
         self.generation += 1
         cams = np.ascontiguousarray(cameras)
         n = len(cams)
         depth = self.data.depth if depth is None else depth
-        o = RenderOpts(depth, EOS_MEM_DEVICE, None, None, 0, int(count_visits), 0)
+        o = RenderOpts(depth, EOS_MEM_DEVICE, None, None, 0, int(count_visits), 0, int(launch_times))
         ptrs = (C.c_void_p * n)(*rgba_dev_ptrs)
         st = (RenderStats * n)()
         check(lib().eos_render_frames(self.h, cams.ctypes.data, n, C.byref(o), ptrs, st))

@@ -921,24 +921,14 @@ eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count
         EOS_TRY(enqueue_frame(s->fb, f, fs, s->ev, &fe));
         return finish_frame(s->fb, f, s->stream, s->ev, &fe, st);
     }
-    // four events: around the frame and around its primary-ray launch (an event
-    // between two dependent launches delays the second by a few microseconds)
-    while (s->first_ev.size() < 2) {
-        hipEvent_t e = nullptr;
-        EOS_HIP(hipEventCreate(&e));
-        s->first_ev.push_back(e);
-    }
-    f.first_ev[0] = s->first_ev[0];
-    f.first_ev[1] = s->first_ev[1];
+    // two events, around the frame only (an event between two dependent
+    // launches delays the second by a few microseconds)
     EOS_HIP(hipEventRecord(s->ev[0], s->stream));
     EOS_TRY(enqueue_frame(s->fb, f, fs, nullptr, nullptr));
     EOS_HIP(hipEventRecord(s->ev[1], s->stream));
     EOS_TRY(finish_frame(s->fb, f, s->stream, nullptr, nullptr, st));
-    if (st) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, s->ev[0], s->ev[1]) == hipSuccess) st->render_ms = ms;
-        if (hipEventElapsedTime(&ms, s->first_ev[0], s->first_ev[1]) == hipSuccess) st->trace_first_ms = ms;
-    }
+    float ms = 0.f;
+    if (st && hipEventElapsedTime(&ms, s->ev[0], s->ev[1]) == hipSuccess) st->render_ms = ms;
     return EOS_OK;
 }
 
@@ -952,7 +942,7 @@ eos_status run_frame(eos_scene* s, const eos_camera* camp, int depth, bool count
 namespace {
 // Runs prepared frames kFrameSlots in flight; per-frame stats (render_ms: the
 // sequence's GPU time / frames) into st when non-null.
-eos_status run_pipelined(eos_scene* s, std::vector<Frame>& frames, eos_render_stats* st) {
+eos_status run_pipelined(eos_scene* s, std::vector<Frame>& frames, eos_render_stats* st, bool launch_times) {
     constexpr int K = eos_scene::kFrameSlots;
     const int n = (int)frames.size();
     if (!s->seq_start) {
@@ -992,14 +982,14 @@ eos_status run_pipelined(eos_scene* s, std::vector<Frame>& frames, eos_render_st
         f.cfg.batch = f.cfg.frames_batch;
     }
     const int slots = std::min(K, std::max(n, 1));
-    // a pair of timing events per frame around its primary-ray launch (the
-    // dispatch duration of that launch while frames overlap)
-    while (s->first_ev.size() < 2 * (size_t)n) {
+    // launch_times: a pair of timing events per frame around its primary-ray
+    // launch (the dispatch duration of that launch while frames overlap)
+    while (launch_times && s->first_ev.size() < 2 * (size_t)n) {
         hipEvent_t e = nullptr;
         EOS_HIP(hipEventCreate(&e));
         s->first_ev.push_back(e);
     }
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; launch_times && i < n; ++i) {
         frames[(size_t)i].first_ev[0] = s->first_ev[2 * (size_t)i];
         frames[(size_t)i].first_ev[1] = s->first_ev[2 * (size_t)i + 1];
     }
@@ -1036,7 +1026,8 @@ eos_status run_pipelined(eos_scene* s, std::vector<Frame>& frames, eos_render_st
                             &one));
         one.render_ms = total / n;
         float fm = 0.f;
-        if (hipEventElapsedTime(&fm, s->first_ev[2 * (size_t)i], s->first_ev[2 * (size_t)i + 1]) == hipSuccess)
+        if (launch_times &&
+            hipEventElapsedTime(&fm, s->first_ev[2 * (size_t)i], s->first_ev[2 * (size_t)i + 1]) == hipSuccess)
             one.trace_first_ms = fm;
         if (st) st[i] = one;
     }
@@ -1051,7 +1042,7 @@ eos_status render_frames(eos_scene* s, const eos_camera* cams, int n, const eos_
     for (int i = 0; i < n; ++i)
         frames.push_back(make_frame(s, &cams[i], o->depth, o->count_visits != 0, reinterpret_cast<uchar4*>(rgba[i]),
                                     nullptr, nullptr, AaSample()));
-    return run_pipelined(s, frames, st);
+    return run_pipelined(s, frames, st, o->launch_times != 0);
 }
 
 eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o, uint8_t* rgba,
@@ -1111,7 +1102,7 @@ eos_status render(eos_scene* s, const eos_camera* cam, const eos_render_opts* o,
                                             row0, row1));
             }
         std::vector<eos_render_stats> per(frames.size());
-        EOS_TRY(run_pipelined(s, frames, per.data()));
+        EOS_TRY(run_pipelined(s, frames, per.data(), o->launch_times != 0));
         eos_render_stats sum{};
         for (const eos_render_stats& one : per) {
             sum.primary_rays += one.primary_rays;

@@ -119,9 +119,10 @@ typedef struct eos_render_opts {
     int32_t aa_samples;       /* RayTracer::setAASamples (src/RayTracer.cpp:176-266): n x n samples per
                                  pixel on a regular grid, averaged; 0 or 1: one ray through the pixel
                                  corner.  hit_out then holds sample (0, 0); keep_records needs 0 or 1 */
-    int32_t launch_times;     /* 1: time every launch of the frame (trace_closest_ms, trace_shadow_ms,
-                                 shade_ms); each timing event between dependent launches costs a few
-                                 microseconds of GPU time.  0: only render_ms and trace_first_ms */
+    int32_t launch_times;     /* 1: time the launches of the frame (trace_closest_ms, trace_shadow_ms,
+                                 shade_ms, trace_first_ms; eos_render_frames: trace_first_ms); each
+                                 timing event between dependent launches costs a few microseconds of
+                                 GPU time.  0: render_ms only */
 } eos_render_opts;
 
 typedef struct eos_render_stats {
@@ -139,7 +140,7 @@ typedef struct eos_render_stats {
     int32_t trace_shadow_launches;
     int64_t closest_visits_first;  /* count_visits: node visits of the primary-ray launch */
     int64_t closest_tests_first;   /* count_visits: triangle tests of the primary-ray launch */
-    double trace_first_ms;         /* duration of the primary-ray closest-hit launch */
+    double trace_first_ms;         /* launch_times: duration of the primary-ray closest-hit launch */
 } eos_render_stats;
 
 /* ---- scene lifetime ----------------------------------------------------- */
