@@ -82,6 +82,15 @@ def test_errors(tmp_path):
     two = _write(tmp_path, "v 0 0 0\nv 1 0 0\nf 1 2\n", "two.obj")
     with pytest.raises(E.EosError):
         E.load_obj(two)
+    # indices that wrap to a valid int when cast (2^32 + 1 -> 1, -(2^32) + 1) are out of range
+    for k, idx in enumerate(("4294967297", "-4294967295", "99999999999999999999", "2147483648")):
+        big = _write(tmp_path, f"v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1 2 {idx}\n", f"big{k}.obj")
+        with pytest.raises(E.EosError) as ei:
+            E.load_obj(big)
+        assert ei.value.status == E.EOS_ERR_INVALID, idx
+        bign = _write(tmp_path, f"v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3//{idx}\n", f"bign{k}.obj")
+        with pytest.raises(E.EosError):
+            E.load_obj(bign)
     empty = _write(tmp_path, "# nothing\n", "empty.obj")
     v, n = E.load_obj(empty)
     assert v.shape == (0, 3, 3)
