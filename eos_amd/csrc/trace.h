@@ -923,6 +923,13 @@ constexpr int kRegStack = EOS_REG_STACK;
 #define EOS_ANY_NOSORT 1
 #endif
 constexpr bool kFastPush = EOS_FASTPUSH != 0;
+// Idle lanes a wave needs before its walking lanes lend subtrees (measured,
+// frames identical: at 1, 16, 32, 48 idle lanes config C's single frame took
+// 2.21, 2.11, 2.11, 2.28 ms and config D's 64 sequential frames 102, 93, 84, 87 ms).
+#ifndef EOS_STEAL_IDLE
+#define EOS_STEAL_IDLE 32
+#endif
+constexpr int kStealIdle = EOS_STEAL_IDLE;
 constexpr bool kAnyNoSort = EOS_ANY_NOSORT != 0;
 
 // A lane's traversal stack: the top kRegStack entries in registers (r[0] the
@@ -1008,8 +1015,10 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     constexpr bool ANY = any_hit_mode(MODE);
     constexpr int MAXS = max_stack<WIDTH>();
     constexpr int LDS_LANE = STACK_LDS * WG, LDS_COOP = WIDTH == 4 ? (WG / 4) * kMaxStack4 : 0;
-    // the group walk serves the short launches after the primary rays, never the primary launch
-    constexpr bool GROUP = WIDTH == 4 && !COUNT && MODE != MODE_PRIMARY;
+    // the group walk serves the short launches after the primary rays, never the
+    // primary launch, and not batched frames, whose launches hold every scene's
+    // rays (its LDS stacks, 32 KiB per workgroup, cost a batched frame 4%)
+    constexpr bool GROUP = WIDTH == 4 && !COUNT && MODE != MODE_PRIMARY && !BATCH;
     // work stealing visits nodes in another order (never in launches that count visits)
     constexpr bool STEAL = !COUNT && kRegStack == 0;
     constexpr int LDS_GROUP = GROUP ? (WG / 64) * kWaveGroupStack : 0;
@@ -1136,7 +1145,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
             const int need = (f.drained && f.next >= f.end) ? 64 : thresh;
             const bool stealing = steal && need == 64;
             for (;;) {
-                if (ANY && stealing) {
+                if (ANY && stealing && __ballot(home >= 0) != 0) {
                     // a helper stops once its ray is answered: the home lane left
                     // it (an occluder was found) or took another role
                     const int hl = home >= 0 ? home : lane;
@@ -1349,9 +1358,11 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                         item = -1;
                     }
                     // ---- idle lanes take the bottom stack entry of walking lanes
+                    // (once at least kStealIdle lanes are idle: while most lanes still
+                    // walk, lending costs the walkers more than the idle lanes gain)
                     const uint64_t idle = __ballot(item < 0);
                     const uint64_t dm = __ballot(item >= 0 && cur != kDone && stk.depth() > 0);
-                    if (idle != 0 && dm != 0) {
+                    if (__popcll(idle) >= kStealIdle && dm != 0) {
                         const int ni = __popcll(idle), nd = __popcll(dm);
                         const int np = ni < nd ? ni : nd;
                         const int irank = __popcll(idle & lanemask_lt());
