@@ -414,6 +414,13 @@ struct Scatter {
     }
 };
 
+// Any-hit rays sent into their own surface walk the farthest child first (see
+// setup_item); a switch for measuring it.
+#ifndef EOS_FAR_FIRST
+#define EOS_FAR_FIRST 1
+#endif
+constexpr bool kFarFirst = EOS_FAR_FIRST != 0;
+
 // One traversal item, set up: the ray and the scene it runs in.
 struct RayItem {
     F3 o, d;
@@ -424,6 +431,7 @@ struct RayItem {
     const float4* geo;
     const float4* xf;        // transformations of the scene (null: none)
     size_t out_idx;          // any hit: result byte of the item
+    bool far_first;          // shadow ray into its own surface: walk the farthest child first
 };
 
 template <int MODE, bool BATCH, int WIDTH>
@@ -436,6 +444,7 @@ __device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& 
     r.root = WIDTH == 4 ? a.sc.root4 : a.sc.root;
     r.max_visits = a.sc.n_tri + 64;
     r.out_idx = 0;
+    r.far_first = false;
     auto use_scene = [&](const BatchScene& B) {
         r.nodes = B.sc.nodes;
         r.nodes4 = B.sc.nodes4;
@@ -494,6 +503,11 @@ __device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& 
             const DevLight& L = lights[li];
             r.d = normalize3(sub(f3(L.position[0], L.position[1], L.position[2]), P));
             r.o = add(P, scale(1e-3f, r.d));
+            // a light behind the surface sends the ray into the object: its
+            // occluder, if any, is most likely far away (the far wall), so that
+            // ray walks the farthest child first (any order gives the same answer)
+            const float4 s1 = surf[2 * (size_t)j + 1];
+            r.far_first = kFarFirst && dot3(f3(s1.x, s1.y, s1.z), r.d) < 0.0f;
         } else {
             r.valid = false;
             r.d = f3(0.0f, 0.0f, 1.0f);
@@ -1072,6 +1086,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         int cur = kDone;
         LaneStack<kRegStack, WG, STACK_LDS> stk;
         int lpos = -1;   // next triangle of the current leaf (-1: not in a leaf)
+        bool farf = false;   // any hit: walk the farthest hit child first (RayItem::far_first)
         int budget = 0;  // node visits left for this ray: a tree visits each internal node at most once
         // XF: the ray in the local space of transformation lxi, kept while the
         // walk stays among that transformation's triangles (-1: none yet)
@@ -1131,6 +1146,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     lxi = -1;
                     budget = r.max_visits;
                     cur = r.valid ? r.root : kDone;
+                    farf = ANY && !COUNT && r.far_first;
                     home = -1;
                     pending = 0;
                 }
@@ -1179,7 +1195,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                             } else if (stk.depth() + nh - 1 > MAXS) {
                                 atomicOr(a.err, 1);
                                 cur = kDone;
-                            } else if (ANY && !COUNT && kAnyNoSort && stk.lds_room(4)) {
+                            } else if (ANY && !COUNT && kAnyNoSort && !farf && stk.lds_room(4)) {
                                 // any hit: the order of the children does not change the
                                 // answer; the first hit child next, the others onto the stack
                                 // in slot order (non-hits to a slot above the new top)
@@ -1199,7 +1215,22 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                                 cmpswap(e0, k0, r0, e2, k2, r2);
                                 cmpswap(e1, k1, r1, e3, k3, r3);
                                 cmpswap(e1, k1, r1, e2, k2, r2);
-                                if (kFastPush && stk.lds_room(3)) {
+                                if (ANY && !COUNT && farf) {
+                                    // farthest hit child next; the nearer ones pushed nearest
+                                    // first, so the next farthest is popped next
+                                    const int far = nh > 3 ? r3 : (nh > 2 ? r2 : (nh > 1 ? r1 : r0));
+                                    if (kFastPush && stk.lds_room(3)) {
+                                        stk.put(lds, tid, 0, r0);
+                                        stk.put(lds, tid, 1, r1);
+                                        stk.put(lds, tid, 2, r2);
+                                        stk.grow(nh - 1);
+                                    } else {
+                                        if (nh > 1) stk.push(lds, gst, tid, r0);
+                                        if (nh > 2) stk.push(lds, gst, tid, r1);
+                                        if (nh > 3) stk.push(lds, gst, tid, r2);
+                                    }
+                                    r0 = far;
+                                } else if (kFastPush && stk.lds_room(3)) {
                                     // farthest first, all three slots written (those above
                                     // the new top are overwritten later)
                                     stk.put(lds, tid, 0, nh > 3 ? r3 : (nh > 2 ? r2 : r1));
@@ -1411,6 +1442,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                             budget = g_budget;
                             stk.clear();
                             lpos = -1;
+                            farf = false;
                             lxi = -1;
                             pending = 0;
                         }
