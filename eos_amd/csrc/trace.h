@@ -150,6 +150,22 @@ __device__ __forceinline__ bool slab(float lox, float hix, float loy, float hiy,
     return a <= b;
 }
 
+// The same test with each axis's near and far planes chosen by the sign of the
+// inverse direction beforehand (lo first where the component is positive):
+// fma(x, inv, -p) is monotone in x for a fixed inverse, so the near plane's
+// value is min(lo, hi)'s and the far plane's max(lo, hi)'s, and tmin, tmax and
+// the answer equal slab()'s bit for bit with six fewer min/max per box.
+__device__ __forceinline__ bool slab_nf(float nx, float fx, float ny, float fy, float nz, float fz, float ix,
+                                        float iy, float iz, float px, float py, float pz, float tmax, float& tmin) {
+    const float x0 = __builtin_fmaf(nx, ix, -px), x1 = __builtin_fmaf(fx, ix, -px);
+    const float y0 = __builtin_fmaf(ny, iy, -py), y1 = __builtin_fmaf(fy, iy, -py);
+    const float z0 = __builtin_fmaf(nz, iz, -pz), z1 = __builtin_fmaf(fz, iz, -pz);
+    const float a = fmaxf(fmaxf(x0, y0), fmaxf(z0, 0.0f));
+    const float b = fminf(fminf(x1, y1), fminf(z1, tmax));
+    tmin = a;
+    return a <= b;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = __lane_id();
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -1087,6 +1103,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         LaneStack<kRegStack, WG, STACK_LDS> stk;
         int lpos = -1;   // next triangle of the current leaf (-1: not in a leaf)
         bool farf = false;   // any hit: walk the farthest hit child first (RayItem::far_first)
+        int sgx = 0, sgy = 0, sgz = 0;   // 1 where the inverse direction is negative (hi plane nearer)
         int budget = 0;  // node visits left for this ray: a tree visits each internal node at most once
         // XF: the ray in the local space of transformation lxi, kept while the
         // walk stays among that transformation's triangles (-1: none yet)
@@ -1137,6 +1154,9 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                     px = o.x * ix;
                     py = o.y * iy;
                     pz = o.z * iz;
+                    sgx = ix < 0.0f ? 1 : 0;
+                    sgy = iy < 0.0f ? 1 : 0;
+                    sgz = iz < 0.0f ? 1 : 0;
                     best_t = INFINITY;
                     best_slot = -1;
                     best_orig = 0x7FFFFFFF;
@@ -1180,15 +1200,28 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                         if (COUNT) ++visits;
                         const float tmax = ANY ? INFINITY : best_t;
                         if (WIDTH == 4) {
-                            const Node4* nd = nodes4 + cur;
-                            const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
-                            const int4 rf = nd->ref;
+                            // the near and far planes of each axis fetched by the ray's
+                            // direction signs (float4 0/1: x lo/hi, 2/3: y, 4/5: z);
+                            // batched frames keep the plain test (measured faster there)
+                            const float4* nd = reinterpret_cast<const float4*>(nodes4 + cur);
+                            const int4 rf = reinterpret_cast<const int4*>(nd)[6];
                             float t0, t1, t2, t3;
+                            bool h0, h1, h2, h3;
                             // all four tests unconditionally (no load sinking), empty slots masked after
-                            const bool h0 = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ix, iy, iz, px, py, pz, tmax, t0) & (rf.x != kEmpty);
-                            const bool h1 = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ix, iy, iz, px, py, pz, tmax, t1) & (rf.y != kEmpty);
-                            const bool h2 = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ix, iy, iz, px, py, pz, tmax, t2) & (rf.z != kEmpty);
-                            const bool h3 = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ix, iy, iz, px, py, pz, tmax, t3) & (rf.w != kEmpty);
+                            if (BATCH) {
+                                const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+                                h0 = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ix, iy, iz, px, py, pz, tmax, t0) & (rf.x != kEmpty);
+                                h1 = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ix, iy, iz, px, py, pz, tmax, t1) & (rf.y != kEmpty);
+                                h2 = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ix, iy, iz, px, py, pz, tmax, t2) & (rf.z != kEmpty);
+                                h3 = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ix, iy, iz, px, py, pz, tmax, t3) & (rf.w != kEmpty);
+                            } else {
+                                const float4 nx = nd[sgx], fx = nd[1 - sgx], ny = nd[2 + sgy], fy = nd[3 - sgy],
+                                             nz = nd[4 + sgz], fz = nd[5 - sgz];
+                                h0 = slab_nf(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, ix, iy, iz, px, py, pz, tmax, t0) & (rf.x != kEmpty);
+                                h1 = slab_nf(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, ix, iy, iz, px, py, pz, tmax, t1) & (rf.y != kEmpty);
+                                h2 = slab_nf(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, ix, iy, iz, px, py, pz, tmax, t2) & (rf.z != kEmpty);
+                                h3 = slab_nf(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, ix, iy, iz, px, py, pz, tmax, t3) & (rf.w != kEmpty);
+                            }
                             const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
                             if (nh == 0) {
                                 cur = stk.pop(lds, gst, tid);
@@ -1434,6 +1467,9 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                             px = o.x * ix;
                             py = o.y * iy;
                             pz = o.z * iz;
+                            sgx = ix < 0.0f ? 1 : 0;
+                            sgy = iy < 0.0f ? 1 : 0;
+                            sgz = iz < 0.0f ? 1 : 0;
                             // only hits better than the lender's best so far count
                             best_t = g_t;
                             best_orig = g_orig;
