@@ -436,8 +436,12 @@ __global__ void k_zero_i32(int* p, int n) {
 // default configuration only (they exist for comparison and parity checks).
 // Scenes with transformations: the default configuration (256 threads, 16 LDS
 // stack entries), no packets; the frame sets its configuration to match.
+// lean (shadow rays, single scenes): the default configuration's kernel without
+// the group walk (less LDS per workgroup, for a launch that shares the GPU with
+// another stream's).
 template <int MODE, bool COUNT, bool BATCH>
-eos_status launch_trace(const LaunchCfg& cfg, int width, bool xf, int grid, hipStream_t s, const TraceArgs& a) {
+eos_status launch_trace(const LaunchCfg& cfg, int width, bool xf, int grid, hipStream_t s, const TraceArgs& a,
+                        bool lean) {
 #define EOS_TRACE_CASE(WG, SL)                                                                             \
     case WG * 100 + SL:                                                                                     \
         hipLaunchKernelGGL((k_trace<MODE, WG, SL, COUNT, BATCH, 4, false>), dim3(grid), dim3(WG), 0, s, a); \
@@ -454,6 +458,9 @@ eos_status launch_trace(const LaunchCfg& cfg, int width, bool xf, int grid, hipS
         hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 2, false>), dim3(grid), dim3(256), 0, s, a);
     } else if (COUNT) {
         hipLaunchKernelGGL((k_trace<MODE, 256, 16, COUNT, BATCH, 4, false>), dim3(grid), dim3(256), 0, s, a);
+    } else if (MODE == MODE_SHADOW && !BATCH && lean && cfg.workgroup == 256 && cfg.stack_lds == 16) {
+        if constexpr (MODE == MODE_SHADOW && !BATCH)
+            hipLaunchKernelGGL((k_trace<MODE, 256, 16, false, false, 4, false, true>), dim3(grid), dim3(256), 0, s, a);
     } else {
         switch (cfg.workgroup * 100 + cfg.stack_lds) {
             EOS_TRACE_CASE(64, 8)
@@ -475,10 +482,10 @@ eos_status launch_trace(const LaunchCfg& cfg, int width, bool xf, int grid, hipS
 
 template <int MODE>
 eos_status launch_trace_any(bool count, bool batch, int width, bool xf, const LaunchCfg& cfg, int grid, hipStream_t s,
-                            const TraceArgs& a) {
-    if (batch) return launch_trace<MODE, false, true>(cfg, width, xf, grid, s, a);
-    return count ? launch_trace<MODE, true, false>(cfg, width, xf, grid, s, a)
-                 : launch_trace<MODE, false, false>(cfg, width, xf, grid, s, a);
+                            const TraceArgs& a, bool lean = false) {
+    if (batch) return launch_trace<MODE, false, true>(cfg, width, xf, grid, s, a, false);
+    return count ? launch_trace<MODE, true, false>(cfg, width, xf, grid, s, a, false)
+                 : launch_trace<MODE, false, false>(cfg, width, xf, grid, s, a, lean);
 }
 
 inline unsigned blocks_for(size_t n, unsigned t, unsigned cap) {
@@ -645,7 +652,14 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             sh.vis_out = fb.vis_at(f.tree ? 0 : l0);
             sh.surf_stride = (long long)fb.surf_stride();
             sh.vis_stride = (long long)fb.vis_stride();
-            EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, st, sh));
+            // level 0's shadow launch on the second stream shares the CUs with the
+            // reflection chain: it takes the kernel without the group walk, whose
+            // 24 KiB of LDS per workgroup (32 KiB with it) leave the chain's
+            // workgroups room (hit-dense 4K frame 17.3 -> 16.5 ms, config E
+            // 1.84 -> 1.80 ms); its launches are large, so the group walk would
+            // not be taken there anyway
+            EOS_TRY(launch_trace_any<MODE_SHADOW>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, st, sh,
+                                                  st != S));
         }
         F.s[2 * slot + 1] = rec(st);
         // Phong of levels [l0, l1] in one launch (ray trees shade one level at a time)

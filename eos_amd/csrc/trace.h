@@ -1039,8 +1039,10 @@ struct LaneStack {
 // tree, a.coop) takes the quad-cooperative walk instead.  BATCH: every ray looks
 // up its own scene (BVH, triangles, lights, camera) in the batch table.  XF:
 // triangles may carry transformations; a ray meets such a triangle in the
-// triangle's local space.
-template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH, int WIDTH, bool XF>
+// triangle's local space.  LEAN: no group walk, and so no LDS for its stacks
+// (24 instead of 32 KiB per 256-thread workgroup), for a launch that runs
+// beside another stream's launches.
+template <int MODE, int WG, int STACK_LDS, bool COUNT, bool BATCH, int WIDTH, bool XF, bool LEAN = false>
 __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     constexpr bool ANY = any_hit_mode(MODE);
     constexpr int MAXS = max_stack<WIDTH>();
@@ -1048,7 +1050,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
     // the group walk serves the short launches after the primary rays, never the
     // primary launch, and not batched frames, whose launches hold every scene's
     // rays (its LDS stacks, 32 KiB per workgroup, cost a batched frame 4%)
-    constexpr bool GROUP = WIDTH == 4 && !COUNT && MODE != MODE_PRIMARY && !BATCH;
+    constexpr bool GROUP = WIDTH == 4 && !COUNT && MODE != MODE_PRIMARY && !BATCH && !LEAN;
     // work stealing visits nodes in another order (never in launches that count visits)
     constexpr bool STEAL = !COUNT && kRegStack == 0;
     constexpr int LDS_GROUP = GROUP ? (WG / 64) * kWaveGroupStack : 0;
