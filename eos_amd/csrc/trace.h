@@ -443,9 +443,13 @@ constexpr bool kFarFirst = EOS_FAR_FIRST != 0;
 // the tile), so that lanes refilled together trace neighbouring surfaces toward
 // one light.  Measured against surface by surface (the light slots of a surface
 // next to each other): hit-dense 4K frame 16.5 -> 14.65 ms, glass frames in
-// flight 3.66 -> 3.51 ms, config D's batched frame 33.9 -> 33.1 ms, config C
-// unchanged; whole levels light by light (no tiles) cost config E's frame 5%.
-// 0: surface by surface (a switch for measuring it).
+// flight 3.66 -> 3.51 ms, config D's batched frame 33.9 -> 33.1 ms; whole
+// levels light by light (no tiles) cost config E's frame 5%.  Launches whose
+// claims are scattered (Scatter) keep surface by surface: a block then holds
+// one surface's light slots, whose walks share their first nodes (config C's
+// level-0 shadow launch alone: 0.38 ms, against 0.44 ms for blocks of four
+// surfaces toward one light).  0: surface by surface always (a switch for
+// measuring it).
 #ifndef EOS_LIGHT_TILE
 #define EOS_LIGHT_TILE 32
 #endif
@@ -464,8 +468,10 @@ struct RayItem {
     bool far_first;          // shadow ray into its own surface: walk the farthest child first
 };
 
+// light_tiles: a shadow launch's items come in light tiles (kLightTile), else
+// surface by surface
 template <int MODE, bool BATCH, int WIDTH>
-__device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& r) {
+__device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& r, bool light_tiles) {
     r.valid = true;
     r.nodes = a.sc.nodes;
     r.nodes4 = a.sc.nodes4;
@@ -518,7 +524,7 @@ __device__ __forceinline__ void setup_item(const TraceArgs& a, int my, RayItem& 
         }
         const float4* surf = a.surf + (size_t)lev * (size_t)a.surf_stride;
         int j, li;
-        if (kLightTile > 0) {
+        if (kLightTile > 0 && light_tiles) {
             // tile t holds surfaces [t * kLightTile, +hb) (hb < kLightTile in the
             // level's last tile), its items light slot by light slot
             const int nh = a.count_ptr[lev], L = a.count_mul;
@@ -617,7 +623,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
             my = scat.map(my);
             if (item < 0 && my >= 0) {
                 RayItem r;
-                setup_item<MODE, BATCH, 4>(a, my, r);
+                setup_item<MODE, BATCH, 4>(a, my, r, !scat.on);
                 item = my;
                 out_idx = r.out_idx;
                 o = r.o;
@@ -817,7 +823,7 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
             my = scat.map(my);
             if (item < 0 && my >= 0) {
                 RayItem r;
-                setup_item<MODE, BATCH, 4>(a, my, r);
+                setup_item<MODE, BATCH, 4>(a, my, r, !scat.on);
                 item = my;
                 out_idx = r.out_idx;
                 o = r.o;
@@ -1169,7 +1175,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
                 my = scat.map(my);
                 if (item < 0 && my >= 0) {
                     RayItem r;
-                    setup_item<MODE, BATCH, WIDTH>(a, my, r);
+                    setup_item<MODE, BATCH, WIDTH>(a, my, r, !scat.on);
                     item = my;
                     out_idx = r.out_idx;
                     o = r.o;
