@@ -277,7 +277,9 @@ def test_work_stealing_equals_plain_walk(name, width):
     """Work stealing (idle lanes of a drained wave walk pending subtrees of its
     remaining rays, results merged into each ray's home lane) and scattered
     claim order give bit-identical frames, ray counts and query answers, with
-    every launch shape from one ray per wave to many rounds of rays per lane."""
+    every launch shape from one ray per wave to many rounds of rays per lane.
+    Scatter 0 traces shadow items in light tiles (neighbouring surfaces toward
+    one light), scattered launches surface by surface: both orders are here."""
     from eos_amd import _lib as E
     s = GROUP_CASES[name]()
     print(f"work stealing vs plain walk case={name} width={width}", flush=True)
