@@ -16,8 +16,10 @@ Also reported:
   roofline      dominant kernel (the primary-ray closest-hit launch): algorithmic
                 bytes per launch = 128 B per 4-wide node visit (64 B per binary
                 node visit) + 36 B per triangle test + 16 B hit record per ray, from the counting variant of the same
-                traversal (deterministic, so equal to the timed launches), divided
-                by the launch's mean duration from HIP events on its stream.
+                traversal (deterministic, so equal to the timed launches), less the
+                root visit and hit record of each primary ray the tile test resolved
+                (the timed launch does neither for them), divided by the launch's
+                mean duration (tile test included) from HIP events on its stream.
   cpu_baseline  the timing-faithful port of the reference's OpenMP CPU path on a
                 stratified pixel sample of the same frame.
   aa            the config C frame with 2x2 anti-aliasing samples per pixel
@@ -336,8 +338,11 @@ def _primary_roofline(counted, per, width_bytes, n_ids):
     """Algorithmic bytes of the primary-ray launch over its mean dispatch time."""
     import numpy as np
     t = float(np.mean([p["trace_first_ms"] for p in per]))
-    alg = counted["closest_visits_first"] * width_bytes + counted["closest_tests_first"] * TRI_BYTES + n_ids * HIT_BYTES
+    culled = int(per[0]["primary_culled"])
+    alg = (counted["closest_visits_first"] * width_bytes + counted["closest_tests_first"] * TRI_BYTES + n_ids * HIT_BYTES
+           - culled * (width_bytes + HIT_BYTES))
     return {"bound": "hbm", "kernel": "k_trace<MODE_PRIMARY>", "launch_ms": round(t, 4), "alg_bytes_per_launch": int(alg),
+            "primary_culled": culled,
             "node_visits": int(counted["closest_visits_first"]), "tri_tests": int(counted["closest_tests_first"]),
             "achieved": round(alg / (t * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(alg / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -624,11 +629,16 @@ def main() -> int:
     mean = lambda k: float(np.mean([p[k] for p in per]))  # noqa: E731
     t_first = float(np.mean([p["trace_first_ms"] for p in per_t]))
     n_ids = ((W + 7) // 8) * ((H + 7) // 8) * 64
+    # primary rays the tile test resolved: the timed launch neither visits the
+    # root for them nor writes their hit records (the counting frame walks them)
+    culled = int(per_t[0]["primary_culled"])
+    assert all(int(p["primary_culled"]) == culled for p in per_t + per_seq_t)
     alg_bytes = (counted["closest_visits_first"] * NODE_BYTES[build["width"]]
-                 + counted["closest_tests_first"] * TRI_BYTES + n_ids * HIT_BYTES)
+                 + counted["closest_tests_first"] * TRI_BYTES + n_ids * HIT_BYTES
+                 - culled * (NODE_BYTES[build["width"]] + HIT_BYTES))
     achieved = alg_bytes / (t_first * 1e-3) / 1e9
     frame_bytes = (counted["node_visits"] * NODE_BYTES[build["width"]] + counted["tri_tests"] * TRI_BYTES
-                   + st["hits"] * 72 + W * H * 4)
+                   + st["hits"] * 72 + W * H * 4 - culled * NODE_BYTES[build["width"]])
     kernel = "k_trace<MODE_PRIMARY>"
     prof = _profile("config_c")
     traffic = prof.get(kernel)
@@ -664,6 +674,7 @@ def main() -> int:
                      "l2_hit_rate": l2_hit,
                      "traffic_source": prof.get("source"),
                      "alg_bytes_per_launch": int(alg_bytes),
+                     "primary_culled": culled,
                      "node_visits": int(counted["closest_visits_first"]),
                      "tri_tests": int(counted["closest_tests_first"]),
                      # the measured HBM bytes of the launch at its measured time: node and

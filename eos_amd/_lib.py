@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libeos_amd.so")
 
 EOS_OK, EOS_ERR_INVALID, EOS_ERR_HIP, EOS_ERR_NOMEM, EOS_ERR_STATE, EOS_ERR_LIMIT, EOS_ERR_NODEVICE = range(7)
-ABI_VERSION = 6   # EOS_ABI_VERSION of include/eos_c.h these structures follow
+ABI_VERSION = 7   # EOS_ABI_VERSION of include/eos_c.h these structures follow
 EOS_MEM_HOST, EOS_MEM_DEVICE = 0, 1
 DBG = {"morton_sorted": 1, "perm": 2, "kchild": 3, "kparent": 4, "ibox": 5, "nodes": 6, "tris": 7,
        "morton": 8, "nodes4": 9, "xforms": 10}
@@ -69,7 +69,7 @@ class RenderStats(C.Structure):
                 ("trace_shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("trace_closest_launches", C.c_int32), ("trace_shadow_launches", C.c_int32),
                 ("closest_visits_first", C.c_int64), ("closest_tests_first", C.c_int64),
-                ("trace_first_ms", C.c_double)]
+                ("trace_first_ms", C.c_double), ("primary_culled", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -106,7 +106,7 @@ class LaunchConfig(C.Structure):
                 ("refill", C.c_int32), ("batch", C.c_int32), ("primary_packets", C.c_int32), ("coop", C.c_int32),
                 ("frames_waves_per_cu", C.c_int32), ("frames_refill", C.c_int32), ("frames_batch", C.c_int32),
                 ("group", C.c_int32), ("steal", C.c_int32), ("scatter", C.c_int32),
-                ("group_lanes", C.c_int32)]
+                ("group_lanes", C.c_int32), ("cull", C.c_int32)]
 
 
 _LIB = None
@@ -262,7 +262,8 @@ class Scene:
 
     DEFAULT_LAUNCH = {"workgroup": 256, "stack_lds": 16, "waves_per_cu": 20, "refill": 16, "batch": 64,
                       "primary_packets": 0, "coop": 0, "frames_waves_per_cu": 16, "frames_refill": 16,
-                      "frames_batch": 64, "group": 1, "steal": 1, "scatter": 4, "group_lanes": 64}
+                      "frames_batch": 64, "group": 1, "steal": 1, "scatter": 4, "group_lanes": 64,
+                      "cull": 1}
 
     def set_launch_config(self, *args, **kw):
         """Positional fields in eos_launch_config order, or keywords; fields not

@@ -73,14 +73,16 @@ static void free_scene(eos_scene* s) {
                       &s->sort_tmp_v, &s->sort_hist, &s->bounds_part, &s->nodes4, &s->flags4, &s->scan_tmp,
                       &s->xf, &s->xf_index, &s->wverts, &s->q_rays, &s->q_out, &s->q_rec, &s->q_ctr, &s->q_gstack,
                       &s->fb.hit0, &s->fb.q[0], &s->fb.q[1],
-                      &s->fb.hitq, &s->fb.levcol, &s->fb.levels, &s->fb.counters, &s->fb.gstack, &s->fb.aa_acc};
+                      &s->fb.hitq, &s->fb.levcol, &s->fb.levels, &s->fb.counters, &s->fb.gstack, &s->fb.aa_acc,
+                      &s->fb.tiles};
     for (DevBuf* b : bufs) b->release();
     s->fb.surf.release();
     s->fb.vis.release();
     for (int k = 0; k < eos_scene::kFrameSlots; ++k) {
         eos_frame_buffers* fb = k == 0 ? &s->fb : &s->fbx[k - 1];
         for (DevBuf* b : {&fb->hit0, &fb->q[0], &fb->q[1], &fb->hitq, &fb->levcol, &fb->levels, &fb->counters,
-                          &fb->gstack, &fb->aa_acc, &fb->surf, &fb->vis, &fb->rayhit, &fb->aux, &fb->hitcol})
+                          &fb->gstack, &fb->aa_acc, &fb->surf, &fb->vis, &fb->rayhit, &fb->aux, &fb->hitcol,
+                          &fb->tiles})
             b->release();
         if (k > 0) {
             for (hipEvent_t e : s->xsync[k - 1])
@@ -500,6 +502,7 @@ eos_status eos_set_launch_config(eos_scene* s, const eos_launch_config* c) {
     if (c->steal != 0 && c->steal != 1) return fail(EOS_ERR_INVALID, "steal must be 0 or 1");
     if (c->scatter < 0 || c->scatter > 64) return fail(EOS_ERR_INVALID, "scatter must be in [0, 64]");
     if (c->group_lanes != 16 && c->group_lanes != 64) return fail(EOS_ERR_INVALID, "group_lanes must be 16 or 64");
+    if (c->cull != 0 && c->cull != 1) return fail(EOS_ERR_INVALID, "cull must be 0 or 1");
     s->cfg.workgroup = c->workgroup;
     s->cfg.stack_lds = c->stack_lds;
     s->cfg.waves_per_cu = c->waves_per_cu;
@@ -511,6 +514,7 @@ eos_status eos_set_launch_config(eos_scene* s, const eos_launch_config* c) {
     s->cfg.steal = c->steal;
     s->cfg.scatter = c->scatter;
     s->cfg.group_lanes = c->group_lanes;
+    s->cfg.cull = c->cull;
     s->cfg.frames_waves_per_cu = c->frames_waves_per_cu;
     s->cfg.frames_refill = c->frames_refill;
     s->cfg.frames_batch = c->frames_batch;
@@ -530,6 +534,7 @@ eos_status eos_get_launch_config(const eos_scene* s, eos_launch_config* c) {
     c->steal = s->cfg.steal;
     c->scatter = s->cfg.scatter;
     c->group_lanes = s->cfg.group_lanes;
+    c->cull = s->cfg.cull;
     c->frames_waves_per_cu = s->cfg.frames_waves_per_cu;
     c->frames_refill = s->cfg.frames_refill;
     c->frames_batch = s->cfg.frames_batch;

@@ -48,6 +48,7 @@ struct SetupArgs {
     int* shadow_ctr;
     uint8_t* levels;
     int32_t* hit_ids;         // level 0, optional
+    const uint8_t* tile_live; // level 0, single scene, optional: 0 = the tile test resolved the tile (misses)
     // ray trees (scenes with transparent materials): the hit index of every
     // ray of this level (level 0: per pixel; -1: no hit), and per hit the
     // Fresnel weight and the queue slots of its children
@@ -81,12 +82,15 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
     // the next round's hit record is loaded while this round is set up (a
     // level-0 pass is many rounds of mostly misses, bound by that load)
     const float4 kMiss = make_float4(INFINITY, __int_as_float(-1), 0.f, 0.f);
-    float4 h_next = (int)(blockIdx.x * blockDim.x + threadIdx.x) < n ? a.hit_in[blockIdx.x * blockDim.x + threadIdx.x] : kMiss;
+    // tiles the tile test resolved were not traced: their rays are misses
+    auto traced = [&](int j) { return !L0 || BATCH || a.tile_live == nullptr || a.tile_live[j >> 6] != 0; };
+    const int i0 = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    float4 h_next = i0 < n && traced(i0) ? a.hit_in[i0] : kMiss;
     for (int bbase = blockIdx.x * blockDim.x; bbase < n; bbase += stride) {
         const int i = bbase + (int)threadIdx.x;
         const bool valid = i < n;
         const float4 h = h_next;
-        h_next = (long long)i + stride < n ? a.hit_in[i + stride] : kMiss;
+        h_next = (long long)i + stride < n && traced(i + stride) ? a.hit_in[i + stride] : kMiss;
         const int slot = __float_as_int(h.y);
         int pixel = 0, scene = 0;
         F3 o = {0, 0, 0}, d = {0, 0, 0};
@@ -431,6 +435,182 @@ __global__ void k_zero_i32(int* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
+// ---- tile test of the primary rays --------------------------------------------
+struct CullArgs {
+    const Node4* nodes4;
+    int root;                 // internal root node of the 4-wide tree
+    double org[3];            // camera position
+    double dx[3], dy[3], d0[3];   // unnormalised direction through pixel position (px, py): px dx + py dy + d0
+    int width;
+    int tiles_x, n_tiles;
+    int row0, band_h;         // the frame covers camera rows [row0, row0 + band_h)
+    int32_t* live;            // out: tiles left to trace, in tile order, [CTR_LIVE] of them
+    int32_t* block_live;      // per 256 tiles: tiles left (k_cull_tiles), then their list offset (k_cull_scan)
+    uint8_t* flags;           // out: per tile 1 = traced, 0 = resolved here
+    int* live_ctr;
+    int* culled_ctr;          // rays resolved here (pixels of the frame in resolved tiles)
+    int scanned;              // k_cull_compact: block_live holds the scanned offsets (k_cull_scan ran)
+};
+constexpr int kCullBlock = 256;   // tiles per workgroup of the tile test
+constexpr int kCullSumBlocks = 4096;   // up to this many workgroups each sums the counts before its own
+
+// Primary rays of an 8x8 tile leave the camera position through pixel
+// positions (x + ox, y + oy), ox and oy in [0, 1) (camera_dir); their
+// directions are positive multiples of a(px) right + b(py) up + view, which is
+// affine in the pixel position, so every one lies in the pyramid spanned by
+// the directions through the corners of [x0 - 1, x0 + 9] x [y0 - 1, y0 + 9]
+// (the tile's pixels with a pixel to spare on each side).  A box lying wholly
+// on the outer side of one of the pyramid's four side planes, or behind the
+// camera's plane normal to the tile's centre direction, meets no ray of the
+// tile at any t >= 0.  The test asks for a clearance of 1e-3 of the
+// scene's scale (camera distance plus coordinate magnitudes) -- far beyond the
+// rounding of a ray's direction and of the slab test's fma terms -- and is
+// computed in double precision.  A tile whose rays miss all child boxes of the
+// root node is resolved as misses: the per-ray walk would visit the root, find
+// no child hit and end there with no hit.  Everything else is traced.
+__global__ __launch_bounds__(256) void k_cull_tiles(CullArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool traced = true;
+    int culled = 0;
+    if (t < a.n_tiles) {
+        const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
+        const double x0 = tx * 8.0 - 1.0, x1 = tx * 8.0 + 9.0;
+        const double y0 = (double)(ty * 8 + a.row0) - 1.0, y1 = (double)(ty * 8 + a.row0) + 9.0;
+        double D[4][3], C[3];
+        auto dir = [&](double px, double py, double* out) {
+            for (int k = 0; k < 3; ++k) out[k] = fma(px, a.dx[k], fma(py, a.dy[k], a.d0[k]));
+        };
+        dir(x0, y0, D[0]);
+        dir(x1, y0, D[1]);
+        dir(x1, y1, D[2]);
+        dir(x0, y1, D[3]);
+        dir(0.5 * (x0 + x1), 0.5 * (y0 + y1), C);
+        const double* O = a.org;
+        // side planes through the camera, normals towards the pyramid's inside;
+        // fifth, the plane through the camera normal to the centre direction when
+        // every corner direction lies in front of it (then so does every ray)
+        double N[5][3];
+        bool front = true;
+        for (int k = 0; k < 4; ++k) front = front && D[k][0] * C[0] + D[k][1] * C[1] + D[k][2] * C[2] > 0.0;
+        for (int c = 0; c < 3; ++c) N[4][c] = front ? C[c] : 0.0;
+        for (int k = 0; k < 4; ++k) {
+            const double* p = D[k];
+            const double* q = D[(k + 1) & 3];
+            N[k][0] = p[1] * q[2] - p[2] * q[1];
+            N[k][1] = p[2] * q[0] - p[0] * q[2];
+            N[k][2] = p[0] * q[1] - p[1] * q[0];
+            if (N[k][0] * C[0] + N[k][1] * C[1] + N[k][2] * C[2] < 0.0)
+                for (int c = 0; c < 3; ++c) N[k][c] = -N[k][c];
+        }
+        const Node4& nd = a.nodes4[a.root];
+        const float* lo[3] = {&nd.lox.x, &nd.loy.x, &nd.loz.x};
+        const float* hi[3] = {&nd.hix.x, &nd.hiy.x, &nd.hiz.x};
+        const int* ref = &nd.ref.x;
+        bool all_missed = true;
+        for (int ch = 0; ch < 4 && all_missed; ++ch) {
+            if (ref[ch] == kEmpty) continue;
+            double L[3], H[3], scale = fabs(O[0]) + fabs(O[1]) + fabs(O[2]);
+            for (int c = 0; c < 3; ++c) {
+                L[c] = (double)lo[c][ch];
+                H[c] = (double)hi[c][ch];
+                scale += fmax(fabs(L[c] - O[c]), fabs(H[c] - O[c])) + fmax(fabs(L[c]), fabs(H[c]));
+            }
+            // outside plane k: its farthest corner lies below it by more than
+            // 1e-3 scale (|n| far > 1e-3 scale |n|, compared squared)
+            bool outside = false;
+            for (int k = 0; k < 5 && !outside; ++k) {
+                double far = 0.0, nn = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    far += N[k][c] * ((N[k][c] >= 0.0 ? H[c] : L[c]) - O[c]);
+                    nn += N[k][c] * N[k][c];
+                }
+                outside = far < 0.0 && far * far > 1e-6 * nn * scale * scale;
+            }
+            all_missed = outside;
+        }
+        traced = !all_missed;
+        if (!traced) {
+            const int w = min(8, a.width - tx * 8), h = min(8, a.band_h - ty * 8);
+            culled = max(w, 0) * max(h, 0);
+        }
+        a.flags[t] = traced ? 1 : 0;
+    }
+    // tiles left in this workgroup's 256 (their list keeps tile order: the
+    // primary launch's claim ranges stay image bands, one per XCD)
+    __shared__ int s_n;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const uint64_t m = __ballot(t < a.n_tiles && traced);
+    if (__lane_id() == 0 && m != 0) atomicAdd(&s_n, __popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) a.block_live[blockIdx.x] = s_n;
+    wave_sum_add(a.culled_ctr, culled);
+}
+
+// Exclusive scan of the workgroups' counts (one workgroup of 1024 threads,
+// 1024 counts per step) and the list length.
+__global__ __launch_bounds__(1024) void k_cull_scan(int32_t* block_live, int nblocks, int* live_ctr) {
+    __shared__ int s_wave[16];
+    __shared__ int s_total;
+    const int tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
+    int running = 0;
+    for (int base = 0; base < nblocks; base += 1024) {
+        const int i = base + tid;
+        const int v = i < nblocks ? block_live[i] : 0;
+        int x = v;   // inclusive scan in the wave
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_wave[w] = x;
+        __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int k = 0; k < 16; ++k) {
+                const int c = s_wave[k];
+                s_wave[k] = acc;
+                acc += c;
+            }
+            s_total = acc;
+        }
+        __syncthreads();
+        if (i < nblocks) block_live[i] = running + s_wave[w] + x - v;
+        running += s_total;
+        __syncthreads();
+    }
+    if (tid == 0) *live_ctr = running;
+}
+
+// The list of tiles left, in tile order: workgroup b writes its tiles from
+// its scanned offset on.
+__global__ __launch_bounds__(256) void k_cull_compact(CullArgs a) {
+    __shared__ int s_wave[4];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool traced = t < a.n_tiles && a.flags[t] != 0;
+    const uint64_t m = __ballot(traced);
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0) s_wave[w] = __popcll(m);
+    __syncthreads();
+    // this workgroup's offset: scanned, or the sum of the counts before it
+    __shared__ int s_part[4];
+    int base;
+    if (a.scanned) {
+        base = a.block_live[blockIdx.x];
+    } else {
+        int v = 0;
+        for (int b = threadIdx.x; b < (int)blockIdx.x; b += blockDim.x) v += a.block_live[b];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (__lane_id() == 0) s_part[w] = v;
+        __syncthreads();
+        base = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+        if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+            *a.live_ctr = base + s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    }
+    int off = base;
+    for (int k = 0; k < w; ++k) off += s_wave[k];
+    if (traced) a.live[off + __popcll(m & lanemask_lt())] = t;
+}
+
 // ---- traversal launch table ------------------------------------------------
 // 4-wide trees: every workgroup / LDS-stack configuration; binary trees: the
 // default configuration only (they exist for comparison and parity checks).
@@ -595,6 +775,7 @@ eos_status ensure_buffers(eos_frame_buffers& fb, const Frame& f, int trace_grid)
     EOS_TRY(fb.levels.ensure(f.npix));
     EOS_TRY(fb.counters.ensure(4 * (size_t)kCounterInts));
     EOS_TRY(fb.gstack.ensure(2 * gstack_ints(f, trace_grid) * 4));
+    EOS_TRY(fb.tiles.ensure(eos_frame_buffers::tiles_bytes(f.n_ids)));
     fb.npix = f.npix;
     fb.n_ids = f.n_ids;
     fb.depth = f.depth;
@@ -623,6 +804,10 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
     auto cursors = [&](int k) { return ctr + kCursorBase + k * kRanges * kCursorStride; };
     FrameEvents local;
     FrameEvents& F = fe ? *fe : local;
+    // primary tile test: single scenes, 4-wide trees with an internal root node,
+    // launches that do not count visits (their counts restate every ray's walk)
+    const bool cull = f.cfg.cull != 0 && !f.batch && !f.count && f.width == 4 && !f.cfg.packets &&
+                      f.sc.root4 >= 0 && f.sc.root4 != kDone && f.n_ids >= 64;
     F.start = rec(S);
     // Any-hit trace of levels [l0, l1] in one launch, then Phong of each level.
     auto shadow_and_phong = [&](int l0, int l1, hipStream_t st, size_t gstack_off, int slot) -> eos_status {
@@ -722,6 +907,42 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         }
         F.c[2 * l] = rec(S);
         if (l == 0 && f.first_ev[0]) EOS_HIP(hipEventRecord(f.first_ev[0], S));
+        if (l == 0 && cull) {
+            // the tile test (timed with the primary launch): the launch then numbers
+            // only the tiles left, counted on the device
+            const int n_tiles = (int)(f.n_ids / 64);
+            const int nblocks = (n_tiles + kCullBlock - 1) / kCullBlock;
+            CullArgs ca{};
+            ca.nodes4 = f.sc.nodes4;
+            ca.root = f.sc.root4;
+            // camera_dir's direction before normalisation, affine in the pixel position:
+            // ((px - w/2) / (w/2)) right + ((py - h/2) / (h/2)) up + view
+            const double nw = f.cam.width / 2.0, nh = f.cam.height / 2.0;
+            for (int c = 0; c < 3; ++c) {
+                ca.org[c] = f.cam.position[c];
+                ca.dx[c] = (double)f.cam.right[c] / nw;
+                ca.dy[c] = (double)f.cam.up[c] / nh;
+                ca.d0[c] = (double)f.cam.view[c] - (double)f.cam.right[c] - (double)f.cam.up[c];
+            }
+            ca.width = f.cam.width;
+            ca.tiles_x = f.tiles_x;
+            ca.n_tiles = n_tiles;
+            ca.row0 = f.row0;
+            ca.band_h = f.band_h;
+            ca.live = fb.tiles.as<int32_t>();
+            ca.block_live = ca.live + n_tiles;
+            ca.flags = fb.tile_flags(f.n_ids);
+            ca.live_ctr = ctr + CTR_LIVE;
+            ca.culled_ctr = ctr + CTR_CULLED;
+            ca.scanned = nblocks > kCullSumBlocks;
+            hipLaunchKernelGGL(k_cull_tiles, dim3(nblocks), dim3(kCullBlock), 0, S, ca);
+            if (ca.scanned)
+                hipLaunchKernelGGL(k_cull_scan, dim3(1), dim3(1024), 0, S, ca.block_live, nblocks, ca.live_ctr);
+            hipLaunchKernelGGL(k_cull_compact, dim3(nblocks), dim3(kCullBlock), 0, S, ca);
+            ta.count_ptr = ctr + CTR_LIVE;
+            ta.count_mul = 64;
+            ta.live = ca.live;
+        }
         if (l == 0) EOS_TRY(launch_trace_any<MODE_PRIMARY>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, S, ta));
         else EOS_TRY(launch_trace_any<MODE_QUEUE>(f.count, f.batch, f.width, f.xf, f.cfg, trace_grid, S, ta));
         if (l == 0 && f.first_ev[1]) EOS_HIP(hipEventRecord(f.first_ev[1], S));
@@ -754,6 +975,7 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         sa.shadow_ctr = ctr + CTR_SHADOW + l;
         sa.levels = fb.levels.as<uint8_t>();
         sa.hit_ids = l == 0 ? f.hits : nullptr;
+        sa.tile_live = l == 0 && cull ? fb.tile_flags(f.n_ids) : nullptr;
         F.u[2 * l] = rec(S);
         if (l == 0) {
             if (f.batch)
@@ -848,6 +1070,7 @@ eos_status frame_stats(const Frame& f, const int* hctr, const unsigned long long
         if (l >= 1) st->secondary_rays += hctr[CTR_Q + l];
     }
     for (int l = 0; l <= f.depth; ++l) st->shadow_rays += hctr[CTR_SHADOW + l];
+    st->primary_culled = hctr[CTR_CULLED];
     if (E && F) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, E[F->start], E[F->end]);

@@ -213,10 +213,12 @@ enum : int {
     CTR_ERR = 64,      // stack overflow / invalid state
     CTR_SHADOW = 65,   // + level: shadow rays of level (sum over its hits of the hit scene's light count);
                        // one level's count stays below 2^31, the frame's total is summed on the host in 64 bits
+    CTR_LIVE = 120,    // primary tiles left to trace by the tile test (k_cull_tiles)
+    CTR_CULLED = 121,  // primary rays the tile test resolved as misses
     CTR_COUNT = 128
 };
 static_assert(CTR_Q + EOS_MAX_DEPTH + 1 < CTR_HITS && CTR_HITS + EOS_MAX_DEPTH < CTR_ERR &&
-                  CTR_SHADOW + EOS_MAX_DEPTH < CTR_COUNT,
+                  CTR_SHADOW + EOS_MAX_DEPTH < CTR_LIVE && CTR_CULLED < CTR_COUNT,
               "per-level counters overlap");
 // int64 statistics (count_visits): [2*launch] visits, [2*launch+1] tests;
 // launch = level (closest) or 16 + level (any hit).
@@ -273,6 +275,8 @@ struct TraceArgs {
     float aa_ox, aa_oy;      // anti-aliasing sample offset inside the pixel (0: pixel corner)
     int row0, band_h;        // primary rays cover camera rows [row0, row0 + band_h)
     int tiles_x;
+    const int32_t* live;     // MODE_PRIMARY, single scene: tiles left by the tile test (item i is pixel
+                             // i % 64 of tile live[i / 64]); null: item i is pixel i % 64 of tile i / 64
     const BatchScene* table;  // batched frames
     int n_scenes;
     int n_static;            // item count when count_ptr is null
@@ -468,6 +472,15 @@ struct RayItem {
     bool far_first;          // shadow ray into its own surface: walk the farthest child first
 };
 
+// Item of a primary launch whose tiles went through the tile test: the launch
+// numbers only the tiles left (TraceArgs::live); the ray keeps its pixel's
+// number, which names its hit record.
+template <int MODE, bool BATCH>
+__device__ __forceinline__ int primary_tile_item(const TraceArgs& a, int my) {
+    if (MODE != MODE_PRIMARY || BATCH || a.live == nullptr || my < 0) return my;
+    return (a.live[my >> 6] << 6) | (my & 63);
+}
+
 // light_tiles: a shadow launch's items come in light tiles (kLightTile), else
 // surface by surface
 template <int MODE, bool BATCH, int WIDTH>
@@ -620,7 +633,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
         if (nidle >= gthresh && (!f.drained || f.next < f.end)) {
             const int rank = __popcll(idle & ((1ull << gbit) - 1));
             int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
-            my = scat.map(my);
+            my = primary_tile_item<MODE, BATCH>(a, scat.map(my));
             if (item < 0 && my >= 0) {
                 RayItem r;
                 setup_item<MODE, BATCH, 4>(a, my, r, !scat.on);
@@ -820,7 +833,7 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
         if (nidle >= 1 && (!f.drained || f.next < f.end)) {
             const int rank = __popcll(idle & ((1ull << gbit) - 1));
             int my = fetch_items(f, a, n, waves_total, idle, nidle, rank);
-            my = scat.map(my);
+            my = primary_tile_item<MODE, BATCH>(a, scat.map(my));
             if (item < 0 && my >= 0) {
                 RayItem r;
                 setup_item<MODE, BATCH, 4>(a, my, r, !scat.on);
@@ -1172,7 +1185,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
             const int nidle = __popcll(idle);
             if (nidle >= thresh && (!f.drained || f.next < f.end)) {
                 int my = fetch_items(f, a, n, waves_total, idle, nidle, __popcll(idle & lanemask_lt()));
-                my = scat.map(my);
+                my = primary_tile_item<MODE, BATCH>(a, scat.map(my));
                 if (item < 0 && my >= 0) {
                     RayItem r;
                     setup_item<MODE, BATCH, WIDTH>(a, my, r, !scat.on);

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EOS_ABI_VERSION 6
+#define EOS_ABI_VERSION 7
 
 typedef enum eos_status {
     EOS_OK = 0,
@@ -140,7 +140,9 @@ typedef struct eos_render_stats {
     int32_t trace_shadow_launches;
     int64_t closest_visits_first;  /* count_visits: node visits of the primary-ray launch */
     int64_t closest_tests_first;   /* count_visits: triangle tests of the primary-ray launch */
-    double trace_first_ms;         /* launch_times: duration of the primary-ray closest-hit launch */
+    double trace_first_ms;         /* launch_times: duration of the primary-ray closest-hit launch (with the
+                                      tile test before it) */
+    int64_t primary_culled;        /* primary rays the tile test resolved as misses (eos_launch_config.cull) */
 } eos_render_stats;
 
 /* ---- scene lifetime ----------------------------------------------------- */
@@ -374,6 +376,10 @@ typedef struct eos_launch_config {
                                  together -- go to different waves; 0: never.  Results do not change */
     int32_t group_lanes;      /* widest group walk: 64 (launches with at most group x lanes/64 items walk
                                  each ray with a whole wave, 16 nodes per step) or 16 (never wider) */
+    int32_t cull;             /* 1: before a single scene's primary rays (4-wide tree) are traced, a tile
+                                 test resolves the 8x8-pixel tiles whose rays all miss the root node's child
+                                 boxes as misses, and only the other tiles are traced (frames that count
+                                 visits never do); 0: every tile is traced.  Results do not change */
 } eos_launch_config;
 
 eos_status eos_set_launch_config(eos_scene* scene, const eos_launch_config* cfg);

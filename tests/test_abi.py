@@ -69,7 +69,7 @@ def test_null_arguments_are_rejected():
     assert L.eos_render_batch(None, None, 0, None, None, None) == E.EOS_ERR_INVALID
     assert L.eos_debug_read(None, 1, None, 0) == E.EOS_ERR_INVALID
     assert L.eos_destroy_scene(None) == E.EOS_OK
-    assert L.eos_abi_version() == E.ABI_VERSION == 6
+    assert L.eos_abi_version() == E.ABI_VERSION == 7
     assert L.eos_set_transforms(None, 0, None, None, 0) == E.EOS_ERR_INVALID
     assert L.eos_render_rows(None, None, 0, 1, None, None, None) == E.EOS_ERR_INVALID
     assert L.eos_render_frames(None, None, 1, None, None, None) == E.EOS_ERR_INVALID
