@@ -157,12 +157,12 @@ struct eos_frame_buffers {
     eos::DevBuf gstack;     // int32  overflow stacks of persistent lanes
     eos::DevBuf aa_acc;     // float4 [npix]       anti-aliasing sample sums
     // primary tile test, per frame of n_ids primary items (n_ids / 64 tiles):
-    // int32 [tiles] tiles left to trace, int32 [tiles / 256 rounded up] counts /
-    // offsets per workgroup of the test, uint8 [tiles] flags
+    // int32 [tiles] tiles left to trace; per workgroup of the test (256 tiles)
+    // int32 tiles left / list offset and int32 rays resolved; uint8 [tiles] flags
     eos::DevBuf tiles;
     static size_t tile_blocks(size_t n_ids) { return (n_ids / 64 + 255) / 256; }
-    static size_t tiles_bytes(size_t n_ids) { return 4 * (n_ids / 64 + tile_blocks(n_ids)) + n_ids / 64; }
-    uint8_t* tile_flags(size_t n_ids) { return tiles.as<uint8_t>() + 4 * (n_ids / 64 + tile_blocks(n_ids)); }
+    static size_t tiles_bytes(size_t n_ids) { return 4 * (n_ids / 64 + 2 * tile_blocks(n_ids)) + n_ids / 64; }
+    uint8_t* tile_flags(size_t n_ids) { return tiles.as<uint8_t>() + 4 * (n_ids / 64 + 2 * tile_blocks(n_ids)); }
     // Ray trees (scenes with transparent materials): level l holds at most
     // tcap[l] rays / hits at offset toff[l] of rayhit (hit index per ray; level
     // 0 per pixel), aux (Fresnel weight, children) and hitcol (colour per hit);

@@ -446,6 +446,7 @@ struct CullArgs {
     int row0, band_h;         // the frame covers camera rows [row0, row0 + band_h)
     int32_t* live;            // out: tiles left to trace, in tile order, [CTR_LIVE] of them
     int32_t* block_live;      // per 256 tiles: tiles left (k_cull_tiles), then their list offset (k_cull_scan)
+    int32_t* block_culled;    // per 256 tiles: rays resolved (summed by the last workgroup of k_cull_compact)
     uint8_t* flags;           // out: per tile 1 = traced, 0 = resolved here
     int* live_ctr;
     int* culled_ctr;          // rays resolved here (pixels of the frame in resolved tiles)
@@ -468,65 +469,99 @@ constexpr int kCullSumBlocks = 4096;   // up to this many workgroups each sums t
 // computed in double precision.  A tile whose rays miss all child boxes of the
 // root node is resolved as misses: the per-ray walk would visit the root, find
 // no child hit and end there with no hit.  Everything else is traced.
+#ifndef EOS_CULL_REAL
+#define EOS_CULL_REAL double
+#endif
 __global__ __launch_bounds__(256) void k_cull_tiles(CullArgs a) {
+    using creal = EOS_CULL_REAL;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     bool traced = true;
     int culled = 0;
     if (t < a.n_tiles) {
         const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
-        const double x0 = tx * 8.0 - 1.0, x1 = tx * 8.0 + 9.0;
-        const double y0 = (double)(ty * 8 + a.row0) - 1.0, y1 = (double)(ty * 8 + a.row0) + 9.0;
-        double D[4][3], C[3];
-        auto dir = [&](double px, double py, double* out) {
-            for (int k = 0; k < 3; ++k) out[k] = fma(px, a.dx[k], fma(py, a.dy[k], a.d0[k]));
+        const creal x0 = creal(tx * 8 - 1), x1 = creal(tx * 8 + 9);
+        const creal y0 = creal(ty * 8 + a.row0 - 1), y1 = creal(ty * 8 + a.row0 + 9);
+        creal D[4][3], C[3];
+        auto dir = [&](creal px, creal py, creal* out) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) out[k] = fma(px, creal(a.dx[k]), fma(py, creal(a.dy[k]), creal(a.d0[k])));
         };
         dir(x0, y0, D[0]);
         dir(x1, y0, D[1]);
         dir(x1, y1, D[2]);
         dir(x0, y1, D[3]);
-        dir(0.5 * (x0 + x1), 0.5 * (y0 + y1), C);
-        const double* O = a.org;
+        dir(creal(0.5) * (x0 + x1), creal(0.5) * (y0 + y1), C);
+        const creal O[3] = {creal(a.org[0]), creal(a.org[1]), creal(a.org[2])};
         // side planes through the camera, normals towards the pyramid's inside;
         // fifth, the plane through the camera normal to the centre direction when
         // every corner direction lies in front of it (then so does every ray)
-        double N[5][3];
+        creal N[5][3];
         bool front = true;
-        for (int k = 0; k < 4; ++k) front = front && D[k][0] * C[0] + D[k][1] * C[1] + D[k][2] * C[2] > 0.0;
-        for (int c = 0; c < 3; ++c) N[4][c] = front ? C[c] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) front = front && D[k][0] * C[0] + D[k][1] * C[1] + D[k][2] * C[2] > creal(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) N[4][c] = front ? C[c] : creal(0);
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const double* p = D[k];
-            const double* q = D[(k + 1) & 3];
+            const creal* p = D[k];
+            const creal* q = D[(k + 1) & 3];
             N[k][0] = p[1] * q[2] - p[2] * q[1];
             N[k][1] = p[2] * q[0] - p[0] * q[2];
             N[k][2] = p[0] * q[1] - p[1] * q[0];
-            if (N[k][0] * C[0] + N[k][1] * C[1] + N[k][2] * C[2] < 0.0)
-                for (int c = 0; c < 3; ++c) N[k][c] = -N[k][c];
+            const creal sg = N[k][0] * C[0] + N[k][1] * C[1] + N[k][2] * C[2] < creal(0) ? creal(-1) : creal(1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) N[k][c] *= sg;
         }
-        const Node4& nd = a.nodes4[a.root];
-        const float* lo[3] = {&nd.lox.x, &nd.loy.x, &nd.loz.x};
-        const float* hi[3] = {&nd.hix.x, &nd.hiy.x, &nd.hiz.x};
-        const int* ref = &nd.ref.x;
-        bool all_missed = true;
-        for (int ch = 0; ch < 4 && all_missed; ++ch) {
-            if (ref[ch] == kEmpty) continue;
-            double L[3], H[3], scale = fabs(O[0]) + fabs(O[1]) + fabs(O[2]);
-            for (int c = 0; c < 3; ++c) {
-                L[c] = (double)lo[c][ch];
-                H[c] = (double)hi[c][ch];
+        creal nn[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) nn[k] = N[k][0] * N[k][0] + N[k][1] * N[k][1] + N[k][2] * N[k][2];
+        // box [L, H] wholly outside one of the planes: its farthest corner lies
+        // below the plane by more than 1e-3 of the scale (|n| far > 1e-3 scale
+        // |n|, compared squared); the scale is the camera distance plus the
+        // coordinate magnitudes of the box
+        auto outside = [&](const creal* L, const creal* H) {
+            creal scale = fabs(O[0]) + fabs(O[1]) + fabs(O[2]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
                 scale += fmax(fabs(L[c] - O[c]), fabs(H[c] - O[c])) + fmax(fabs(L[c]), fabs(H[c]));
+            bool out = false;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                creal far = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) far = fma(N[k][c], (N[k][c] >= creal(0) ? H[c] : L[c]) - O[c], far);
+                out = out || (far < creal(0) && far * far > creal(1e-6) * nn[k] * scale * scale);
             }
-            // outside plane k: its farthest corner lies below it by more than
-            // 1e-3 scale (|n| far > 1e-3 scale |n|, compared squared)
-            bool outside = false;
-            for (int k = 0; k < 5 && !outside; ++k) {
-                double far = 0.0, nn = 0.0;
-                for (int c = 0; c < 3; ++c) {
-                    far += N[k][c] * ((N[k][c] >= 0.0 ? H[c] : L[c]) - O[c]);
-                    nn += N[k][c] * N[k][c];
+            return out;
+        };
+        const Node4& nd = a.nodes4[a.root];
+        const float4 lo4[3] = {nd.lox, nd.loy, nd.loz}, hi4[3] = {nd.hix, nd.hiy, nd.hiz};
+        const int4 ref = nd.ref;
+        const bool used[4] = {ref.x != kEmpty, ref.y != kEmpty, ref.z != kEmpty, ref.w != kEmpty};
+        creal CL[4][3], CH[4][3], UL[3], UH[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float l[4] = {lo4[c].x, lo4[c].y, lo4[c].z, lo4[c].w};
+            const float h[4] = {hi4[c].x, hi4[c].y, hi4[c].z, hi4[c].w};
+            UL[c] = creal(INFINITY);
+            UH[c] = -creal(INFINITY);
+#pragma unroll
+            for (int ch = 0; ch < 4; ++ch) {
+                CL[ch][c] = l[ch];
+                CH[ch][c] = h[ch];
+                if (used[ch]) {
+                    UL[c] = fmin(UL[c], creal(l[ch]));
+                    UH[c] = fmax(UH[c], creal(h[ch]));
                 }
-                outside = far < 0.0 && far * far > 1e-6 * nn * scale * scale;
             }
-            all_missed = outside;
+        }
+        // the union of the child boxes first (outside it with its own, larger
+        // clearance means outside each child with its own), then each child
+        bool all_missed = !(used[0] || used[1] || used[2] || used[3]) || outside(UL, UH);
+        if (!all_missed) {
+            all_missed = true;
+#pragma unroll
+            for (int ch = 0; ch < 4; ++ch) all_missed = all_missed && (!used[ch] || outside(CL[ch], CH[ch]));
         }
         traced = !all_missed;
         if (!traced) {
@@ -535,16 +570,22 @@ __global__ __launch_bounds__(256) void k_cull_tiles(CullArgs a) {
         }
         a.flags[t] = traced ? 1 : 0;
     }
-    // tiles left in this workgroup's 256 (their list keeps tile order: the
-    // primary launch's claim ranges stay image bands, one per XCD)
-    __shared__ int s_n;
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
+    // tiles left and rays resolved in this workgroup's 256 tiles (the list of
+    // tiles left keeps tile order: the primary launch's claim ranges stay image
+    // bands, one per XCD); no atomics on one global address (one per wave
+    // serialised to about 20 us over a 4K frame)
+    __shared__ int s_n[4], s_c[4];
     const uint64_t m = __ballot(t < a.n_tiles && traced);
-    if (__lane_id() == 0 && m != 0) atomicAdd(&s_n, __popcll(m));
+    for (int off = 32; off > 0; off >>= 1) culled += __shfl_xor(culled, off);
+    if (__lane_id() == 0) {
+        s_n[threadIdx.x >> 6] = __popcll(m);
+        s_c[threadIdx.x >> 6] = culled;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) a.block_live[blockIdx.x] = s_n;
-    wave_sum_add(a.culled_ctr, culled);
+    if (threadIdx.x == 0) {
+        a.block_live[blockIdx.x] = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+        a.block_culled[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    }
 }
 
 // Exclusive scan of the workgroups' counts (one workgroup of 1024 threads,
@@ -605,6 +646,13 @@ __global__ __launch_bounds__(256) void k_cull_compact(CullArgs a) {
         base = s_part[0] + s_part[1] + s_part[2] + s_part[3];
         if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
             *a.live_ctr = base + s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    }
+    if (blockIdx.x == gridDim.x - 1) {
+        // the rays resolved over the frame
+        int c = 0;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) c += a.block_culled[b];
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if (__lane_id() == 0 && c != 0) atomicAdd(a.culled_ctr, c);
     }
     int off = base;
     for (int k = 0; k < w; ++k) off += s_wave[k];
@@ -931,6 +979,7 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
             ca.band_h = f.band_h;
             ca.live = fb.tiles.as<int32_t>();
             ca.block_live = ca.live + n_tiles;
+            ca.block_culled = ca.block_live + nblocks;
             ca.flags = fb.tile_flags(f.n_ids);
             ca.live_ctr = ctr + CTR_LIVE;
             ca.culled_ctr = ctr + CTR_CULLED;
