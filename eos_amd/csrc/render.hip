@@ -72,25 +72,33 @@ __device__ __forceinline__ float fresnel(F3 incident, F3 normal, float n1, float
 // Hit set-up of one level: the hit surface (hit_surface, the reference's
 // point and normal), the pixel's level count, the reflection ray and the
 // slots of the next stages.
+// 256 threads: the slot claims are one atomic per counter per workgroup
+// round, and 512 or 1024 threads (2 or 4 times fewer claims) made single
+// frames up to 3% faster but frames in flight 5-10% slower (larger workgroups
+// share the CUs worse with the other frame's launches)
+constexpr int kSetupWG = 256;
 template <bool L0, bool BATCH>
-__global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
+__global__ __launch_bounds__(kSetupWG) void k_setup(SetupArgs a) {
     const int n = L0 ? a.n_ids : *a.count_ptr;
     const int stride = gridDim.x * blockDim.x;
     // every wave of the workgroup runs every round (the round's slots are
     // claimed once per workgroup, below), so the trip count is the workgroup's
-    __shared__ int s_cnt[3][4], s_base[3][4];
+    __shared__ int s_cnt[3][kSetupWG / 64], s_base[3][kSetupWG / 64];
     // the next round's hit record is loaded while this round is set up (a
     // level-0 pass is many rounds of mostly misses, bound by that load)
     const float4 kMiss = make_float4(INFINITY, __int_as_float(-1), 0.f, 0.f);
     // tiles the tile test resolved were not traced: their rays are misses
-    auto traced = [&](int j) { return !L0 || BATCH || a.tile_live == nullptr || a.tile_live[j >> 6] != 0; };
+    const uint8_t* const tile_live = L0 && !BATCH ? a.tile_live : nullptr;
+    auto traced = [tile_live](int j) { return tile_live == nullptr || tile_live[j >> 6] != 0; };
     const int i0 = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    float4 h_next = i0 < n && traced(i0) ? a.hit_in[i0] : kMiss;
+    float4 h_next = kMiss;
+    if (i0 < n && traced(i0)) h_next = a.hit_in[i0];
     for (int bbase = blockIdx.x * blockDim.x; bbase < n; bbase += stride) {
         const int i = bbase + (int)threadIdx.x;
         const bool valid = i < n;
         const float4 h = h_next;
-        h_next = (long long)i + stride < n && traced(i + stride) ? a.hit_in[i + stride] : kMiss;
+        h_next = kMiss;
+        if ((long long)i + stride < n && traced(i + stride)) h_next = a.hit_in[i + stride];
         const int slot = __float_as_int(h.y);
         int pixel = 0, scene = 0;
         F3 o = {0, 0, 0}, d = {0, 0, 0};
@@ -172,6 +180,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
             __syncthreads();
             if (threadIdx.x == 0) {
                 int* const ctrs[3] = {a.hits_ctr, a.q_out_ctr, a.shadow_ctr};
+#pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     int tot = 0;
                     for (int kk = 0; kk < (int)(blockDim.x >> 6); ++kk) {
@@ -1028,12 +1037,12 @@ eos_status enqueue_frame(eos_frame_buffers& fb, const Frame& f, const FrameStrea
         F.u[2 * l] = rec(S);
         if (l == 0) {
             if (f.batch)
-                hipLaunchKernelGGL((k_setup<true, true>), dim3(blocks_for(f.n_ids, 256, shade_grid)), dim3(256), 0, S, sa);
+                hipLaunchKernelGGL((k_setup<true, true>), dim3(blocks_for(f.n_ids, kSetupWG, shade_grid)), dim3(kSetupWG), 0, S, sa);
             else
-                hipLaunchKernelGGL((k_setup<true, false>), dim3(blocks_for(f.n_ids, 256, shade_grid)), dim3(256), 0, S, sa);
+                hipLaunchKernelGGL((k_setup<true, false>), dim3(blocks_for(f.n_ids, kSetupWG, shade_grid)), dim3(kSetupWG), 0, S, sa);
         } else {
-            if (f.batch) hipLaunchKernelGGL((k_setup<false, true>), dim3(shade_grid), dim3(256), 0, S, sa);
-            else hipLaunchKernelGGL((k_setup<false, false>), dim3(shade_grid), dim3(256), 0, S, sa);
+            if (f.batch) hipLaunchKernelGGL((k_setup<false, true>), dim3(shade_grid), dim3(kSetupWG), 0, S, sa);
+            else hipLaunchKernelGGL((k_setup<false, false>), dim3(shade_grid), dim3(kSetupWG), 0, S, sa);
         }
         F.u[2 * l + 1] = rec(S);
         if (f.tree) {
