@@ -325,15 +325,25 @@ constexpr int max_stack() { return WIDTH == 4 ? kMaxStack4 : kMaxStack; }
 // next range when that one is used up; after every range it is drained.
 struct Fetch {
     bool drained = false;   // every range is used up
+    int batch = 64;         // most items per claim
     int next = 0, end = 0;  // unissued items of this wave's chunk
     int seen = 0;           // cursor value of the current range after this wave's last claim
     int range = 0;          // current range
     int tried = 0;          // ranges found used up
 };
 
-__device__ __forceinline__ Fetch fetch_start() {
+// A primary launch left with under half of its frame's items by the tile test
+// traces the tiles around the objects' outlines, where neighbouring rays'
+// costs differ most: it claims at most kCulledBatch items at a time (config C,
+// 35% of its tiles left: single frame 2.12 -> 1.96 ms, frames in flight
+// 1.56 -> 1.50; the same cap on the glass frame, 71% left, cost it 3%).
+constexpr int kCulledBatch = 32;
+
+__device__ __forceinline__ Fetch fetch_start(const TraceArgs& a, int n) {
     Fetch f;
     f.range = (int)(blockIdx.x % kRanges);
+    f.batch = a.batch;
+    if (a.live != nullptr && 2LL * n < (long long)a.n_static && f.batch > kCulledBatch) f.batch = kCulledBatch;
     return f;
 }
 
@@ -356,7 +366,7 @@ __device__ __forceinline__ int fetch_items(Fetch& f, const TraceArgs& a, int n, 
             const int lo = (int)((long long)n * r / kRanges), hi = (int)((long long)n * (r + 1) / kRanges);
             const int len = hi - lo;
             int size = (len - f.seen) / (2 * wpr);
-            size = size > a.batch ? a.batch : size;
+            size = size > f.batch ? f.batch : size;
             size = size < want ? want : size;
             int got = 0;
             if ((int)__lane_id() == leader) got = atomicAdd(a.cursor + r * kCursorStride, size);
@@ -615,7 +625,7 @@ __device__ __forceinline__ void trace_coop(const TraceArgs& a, int n, int32_t* w
     scat.init(a.scatter, n, (long long)gridDim.x * WG, MODE == MODE_SHADOW ? a.count_mul : 1);
     int gthresh = (a.refill + 3) / 4;
     gthresh = gthresh < 1 ? 1 : (gthresh > 16 ? 16 : gthresh);
-    Fetch f = fetch_start();
+    Fetch f = fetch_start(a, n);
     int item = -1;
     size_t out_idx = 0;
     F3 o = {0, 0, 0}, d = {0, 0, 0};
@@ -815,7 +825,7 @@ __device__ __forceinline__ void trace_group(const TraceArgs& a, int n, int32_t* 
     const int waves_total = (int)gridDim.x * (WG / 64);
     Scatter scat;
     scat.init(a.scatter, n, (long long)gridDim.x * WG, MODE == MODE_SHADOW ? a.count_mul : 1);
-    Fetch f = fetch_start();
+    Fetch f = fetch_start(a, n);
     int item = -1;
     size_t out_idx = 0;
     F3 o = {0, 0, 0}, d = {0, 0, 0};
@@ -1140,7 +1150,7 @@ __global__ __launch_bounds__(WG) void k_trace(TraceArgs a) {
         const float4* __restrict__ xf = a.sc.xf;
         int item = -1;
         size_t out_idx = 0;
-        Fetch f = fetch_start();
+        Fetch f = fetch_start(a, n);
         const int waves_total = (int)gridDim.x * (WG / 64);
         Scatter scat;
         scat.init(a.scatter, n, (long long)gridDim.x * WG, MODE == MODE_SHADOW ? a.count_mul : 1);
