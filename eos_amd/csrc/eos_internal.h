@@ -160,7 +160,8 @@ struct eos_frame_buffers {
     // int32 [tiles] tiles left to trace; per workgroup of the test (256 tiles)
     // int32 tiles left / list offset and int32 rays resolved; uint8 [tiles] flags
     eos::DevBuf tiles;
-    static size_t tile_blocks(size_t n_ids) { return (n_ids / 64 + 255) / 256; }
+    static constexpr int kTileBlock = 256;   // tiles per workgroup of the tile test
+    static size_t tile_blocks(size_t n_ids) { return (n_ids / 64 + kTileBlock - 1) / kTileBlock; }
     static size_t tiles_bytes(size_t n_ids) { return 4 * (n_ids / 64 + 2 * tile_blocks(n_ids)) + n_ids / 64; }
     uint8_t* tile_flags(size_t n_ids) { return tiles.as<uint8_t>() + 4 * (n_ids / 64 + 2 * tile_blocks(n_ids)); }
     // Ray trees (scenes with transparent materials): level l holds at most

@@ -461,7 +461,8 @@ struct CullArgs {
     int* culled_ctr;          // rays resolved here (pixels of the frame in resolved tiles)
     int scanned;              // k_cull_compact: block_live holds the scanned offsets (k_cull_scan ran)
 };
-constexpr int kCullBlock = 256;   // tiles per workgroup of the tile test
+constexpr int kCullBlock = eos_frame_buffers::kTileBlock;   // tiles per workgroup of the tile test
+static_assert(kCullBlock == 256, "the tile test's kernels are written for four waves per workgroup");
 constexpr int kCullSumBlocks = 4096;   // up to this many workgroups each sums the counts before its own
 
 // Primary rays of an 8x8 tile leave the camera position through pixel
