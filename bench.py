@@ -228,7 +228,8 @@ def bench_batched(E, torch, reps: int):
     t_bat = (time.perf_counter() - t) / reps
     # the batched frame walks every scene's tree with the same per-ray steps as
     # that scene's own frame, so its visits and tests are the sum of the scenes'
-    # counting frames: the survey's whole-frame algorithmic bytes over the
+    # counting frames: the whole frame's algorithmic bytes (128 B per node
+    # visit, 36 B per triangle test, 72 B per hit, 4 B per pixel) over the
     # batched frame's GPU time
     visits = tests = hits = pix = 0
     for s, d, p in zip(sc, data, ptrs):
