@@ -120,9 +120,11 @@ def test_tile_test_ray_trees_transforms_and_gradients():
         g.set_launch_config(cull=cull)
         g.render(keep_records=True)
         grads[cull] = g.adjoint(dl)
-    print("tile test: adjoint", flush=True)
-    assert np.array_equal(grads[0][0], grads[1][0]) and np.array_equal(grads[0][1], grads[1][1])
-    assert grads[0][2]["records"] == grads[1][2]["records"]
+        grads[(cull, "recompute")] = g.adjoint(dl, recompute=True)
+    print("tile test: adjoint (replayed and recomputed records)", flush=True)
+    for k in (1, (0, "recompute"), (1, "recompute")):
+        assert np.array_equal(grads[0][0], grads[k][0]) and np.array_equal(grads[0][1], grads[k][1]), k
+        assert grads[0][2]["records"] == grads[k][2]["records"], k
 
 
 def test_tile_test_frame_of_over_a_million_tiles():
